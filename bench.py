@@ -1,0 +1,179 @@
+"""Benchmark of the fused eSRRF path on one MI355X.
+
+A step is one fused eSRRF + temporal-correlation pass (libnpx_hip npx_esrrf_temporal) over the whole synthetic
+stack of the workload, resident in HBM before timing starts. Default workload: 1000 frames x 512x512 uint16
+camera counts, magnification 4, radius 2.0, sensitivity 2, intensity weighting, AVG + TAC2 -> 2048x2048 maps.
+
+Prints one JSON line: throughput in output Mpx*frames/s (N * H*M * W*M / 1e6 / seconds), the FP32 roofline of
+the fused kernel (algorithmic flops per launch / its HIP-event duration) and the reference CPU path timed on a
+bounded frame sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "eSRRF output Mpx·frames/s vs plain GPU & CPU; sweep speedup; served p50/p99"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ring_samples_mean(H, W, M, R):
+    """Mean in-disc ring-sample count over the M*M output phases (interior pixels)."""
+    from nanopyx_amd import _hip
+    t = _hip.plan_tables(H, W, M, R)
+    return float((t["ring"][..., 2] > 0).reshape(M * M, -1).sum(1).mean())
+
+
+def traffic_from_profile(workload):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"pmc_esrrf_fused_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(x, e, seconds):
+    """The reference CPU chain (CR xM, Roberts, CR x2M twice, RGC, temporal) on the first frames of the same
+    stack, chunk after chunk until `seconds` of wall clock have passed."""
+    from oracle import ref32
+    ref32.build()
+    kind = "port"
+    if ref32.reference_available():
+        ref32.use_reference_kernels(True)
+        kind = "reference"
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    N, H, W = x.shape
+    chunk = 4
+    done = 0
+    t0 = time.perf_counter()
+    while done < N:
+        part = x[done:done + chunk].astype(np.float32)
+        maps = ref32.esrrf(part, M, R, S, iw)
+        for mode in e["temporal"]:
+            ref32.temporal(maps, mode)
+        done += part.shape[0]
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    ref32.use_reference_kernels(False)
+    value = done * H * M * W * M / 1e6 / dt
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(value, 3), "unit": "Mpx·frames/s", "cores": ref32.num_threads(), "kind": kind,
+            "sample": f"first {done} of {N} frames ({H}x{W}, M{M}, R{R}, S{S}, {'+'.join(e['temporal'])}), "
+                      f"reference CPU chain in {dt:.1f} s; {cpu}, nproc {os.cpu_count()}, "
+                      f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large_512_u16")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the eSRRF path is built and measured on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is a design note only)", flush=True)
+        sys.exit(2)
+
+    import torch
+    from nanopyx_amd import _hip, synthetic
+
+    sc = synthetic.CONFIGS[args.workload]
+    e = sc.esrrf
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    modes = tuple(e["temporal"])
+    log(f"generating {args.workload}: {sc.frames}x{sc.height}x{sc.width} {sc.dtype}")
+    t = time.perf_counter()
+    x_host = synthetic.generate(sc)
+    log(f"generated in {time.perf_counter() - t:.1f} s")
+    N, H, W = x_host.shape
+    HM, WM = H * M, W * M
+    x = torch.from_numpy(x_host).cuda()
+    torch.cuda.synchronize()
+    log(f"device: {_hip.device_info()}")
+
+    def step():
+        return _hip.esrrf_temporal(x, M, R, S, iw, modes)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _hip.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _hip.timing(False)
+    ncalls, kern_ms, merge_ms = _hip.timing_read()
+    log(f"{args.steps} steps in {dt:.3f} s; fused kernel {kern_ms / max(ncalls, 1):.3f} ms/launch, "
+        f"merge {merge_ms / max(ncalls, 1):.3f} ms/launch")
+
+    U = N * HM * WM
+    value = U / 1e6 / (dt / args.steps)
+    s_bar = ring_samples_mean(H, W, M, R)
+    flops = U * (9.0 * s_bar + 40.0)
+    k_s = kern_ms / max(ncalls, 1) / 1e3
+    achieved = flops / k_s / 1e12
+    in_bytes = N * H * W * (2 if sc.dtype == "uint16" else 4)
+    alg_bytes = in_bytes + len(modes) * HM * WM * 4
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mpx·frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {N} frames x {H}x{W} {sc.dtype}, M{M} R{R} S{S} "
+                               f"intensity-weighted, {'+'.join(modes)} -> {HM}x{WM}",
+                   "frames": N, "height": H, "width": W, "magnification": M, "radius": R, "sensitivity": S,
+                   "temporal": list(modes), "input_dtype": sc.dtype},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profile(args.workload),
+                     "compute": "FP32 vector ALU (no MFMA on this stencil path)",
+                     "flops_per_launch": flops, "ring_samples_mean": s_bar,
+                     "kernel_ms": round(k_s * 1e3, 4), "merge_ms": round(merge_ms / max(ncalls, 1), 4),
+                     "algorithmic_bytes": alg_bytes,
+                     "hbm_gbs_algorithmic": round(alg_bytes / k_s / 1e9, 1)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("timing the reference CPU path ...")
+        result["cpu_baseline"] = cpu_baseline(x_host, e, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

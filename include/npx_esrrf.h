@@ -1,0 +1,117 @@
+/*
+ * npx_esrrf.h — C ABI of the MI355X (gfx950) eSRRF library (libnpx_hip.so).
+ *
+ * The drop-in boundary for NanoPyx's eSRRF hot path: the Python host side (nanopyx_amd, the new
+ * "HIP_gfx950" run type of the eSRRF and RadialGradientConvergence Liquid Engine classes) binds these
+ * with ctypes. Plain pointers and sizes only: every buffer is a caller-owned device pointer, `stream` is a
+ * hipStream_t (NULL = the default stream), and calls are asynchronous on that stream. Calls on distinct
+ * streams are reentrant; the library's only shared state is a mutex-protected cache of per-(size,
+ * magnification, radius) geometry tables. Status: 0 = OK, < 0 = error, message in npx_last_error()
+ * (thread-local). Paths in citations are relative to the NanoPyx repository root (v2.1.2).
+ */
+#ifndef NPX_ESRRF_H
+#define NPX_ESRRF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPX_OK 0
+#define NPX_ERR_ARG (-1)         /* bad shape / parameter (reference: ValueError / TypeError)        */
+#define NPX_ERR_HIP (-2)         /* HIP runtime error (launch, allocation, copy)                     */
+#define NPX_ERR_UNSUPPORTED (-3) /* valid for the reference but outside what these kernels build      */
+#define NPX_ERR_WORKSPACE (-4)   /* workspace pointer given but smaller than required                 */
+
+#define NPX_DTYPE_F32 0
+#define NPX_DTYPE_U16 1 /* uint16 camera counts, converted in-kernel exactly like astype(float32) */
+
+#define NPX_AVG 1u
+#define NPX_VAR 2u
+#define NPX_TAC2 4u
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int npx_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *npx_last_error(void);
+
+/* Writes "<device name>;<gcnArchName>;<CUs>;<clock kHz>" of the current device into buf. */
+int npx_device_info(char *buf, int len);
+
+/*
+ * Per-frame eSRRF maps: d_out (N, H*M, W*M) float32 with a zero border of int(float(R)*M) pixels.
+ * Replaces eSRRF._run_threaded / _run_opencl, src/nanopyx/core/transform/_le_esrrf.pyx:172-195 and :39-170
+ * (CR x M, Roberts cross, CR x 2M of both gradients, RGC x intensity), which the reference binds through
+ * Cython (eSRRF.run, _le_esrrf.pyx:30-33). `pad_edges` has no counterpart: the reference does not forward
+ * it on this path (:194). d_in is (N, H, W) of `dtype`.
+ */
+int npx_esrrf_frames(const void *d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                     float *d_out, void *stream);
+
+/* Bytes of device workspace npx_esrrf_temporal uses to split the frames of one call over more blocks. */
+size_t npx_esrrf_temporal_workspace(int N, int H, int W, int M, double R);
+
+/*
+ * Fused eSRRF + temporal correlation: for each bit of mode_mask writes one (H*M, W*M) map — AVG, VAR
+ * (ddof 0) and/or TAC2 — of the per-frame eSRRF maps without materialising them. Replaces the composition
+ * eSRRF_ST.run -> calculate_eSRRF_temporal_correlations of src/nanopyx/methods/esrrf/eSRRF_workflow.py:98-119
+ * (temporal step src/nanopyx/core/transform/sr_temporal_correlations.py:46-75,205-221). d_work may be NULL
+ * (work_bytes 0): the frames are then not split. Output pointers of unset mode bits may be NULL.
+ */
+int npx_esrrf_temporal(const void *d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                       unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2, void *d_work, size_t work_bytes,
+                       void *stream);
+
+/*
+ * RadialGradientConvergence on caller arrays: d_gx, d_gy (N, 2*HM, 2*WM), d_img (N, HM, WM), indexed exactly as
+ * _c_calculate_rgc does (src/include/_c_sr_radial_gradient_convergence.c:17-90). Replaces
+ * RadialGradientConvergence._run_threaded / _run_opencl, src/nanopyx/core/transform/_le_radial_gradient_convergence.pyx:79-115
+ * and :229-332 (pad_edges -> margin 0, :92-95).
+ */
+int npx_rgc(const float *d_gx, const float *d_gy, const float *d_img, int N, int HM, int WM, int M, double R,
+            double S, int iw, int pad_edges, float *d_out, void *stream);
+
+/*
+ * AVG / VAR / TAC2 over axis 0 of a device (N, P) float32 stack. Replaces calculate_eSRRF_temporal_correlations,
+ * src/nanopyx/core/transform/sr_temporal_correlations.py:46-75 (and calculate_tac2, :205-221).
+ */
+int npx_temporal(const float *d_stack, int N, long long P, unsigned mode_mask, float *d_avg, float *d_var,
+                 float *d_tac2, void *stream);
+
+/*
+ * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
+ * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
+ */
+int npx_macro_pixel(float *d_img, int S, int HM, int WM, int M, float *d_cmap, void *stream);
+
+/*
+ * Host-only introspection of the geometry tables the kernels use (no GPU needed). info[16] receives
+ * K, margin, HM, WM, nphase, Q, HG, HI, TIR, TIC, TGR, TGC, LR, LP, nwaves, ngroups; finfo[4] receives
+ * fwhm, sigma, tSS, tSO. No reference counterpart: it exists so the host can check launch geometry and tables.
+ */
+int npx_plan_info(int H, int W, int M, double R, int *info, float *finfo);
+
+/* Copies the plan tables to host arrays sized from npx_plan_info (ring nphase*(2K+1)^2*4 floats,
+ * wsum nphase, lattice taps 2H+1 / 2W+1 entries, intensity taps HM / WM entries; *_w arrays hold 4 floats
+ * per entry). No reference counterpart (introspection, as npx_plan_info). */
+int npx_plan_tables(int H, int W, int M, double R, float *ring, float *wsum, int *lrb, float *lrw, int *lcb,
+                    float *lcw, int *crb, float *crw, int *ccb, float *ccw);
+
+/* Turns per-call HIP-event timing of the eSRRF kernels on (1) or off (0). No reference counterpart: it lets a
+ * caller (the benchmark) read the fused kernel's own duration on the stream it was launched on. */
+int npx_timing(int on);
+
+/* Waits for and sums the events recorded since the last read: number of calls, total milliseconds of the fused
+ * eSRRF kernel and of the moment merge that follows it. No reference counterpart (see npx_timing). */
+int npx_timing_read(int *n_calls, double *kernel_ms, double *merge_ms);
+
+/* Frees the cached device tables of every device (optional; the process exit frees them too). */
+void npx_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPX_ESRRF_H */

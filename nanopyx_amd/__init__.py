@@ -1,0 +1,16 @@
+"""nanopyx_amd — NanoPyx's eSRRF hot path on AMD MI355X (gfx950).
+
+Drop-in entry points with the reference's signatures:
+  * ``nanopyx_amd.eSRRF(...)`` — the workflow function (src/nanopyx/methods/esrrf/eSRRF_workflow.py:12)
+  * ``nanopyx_amd.core.transform.eSRRF_ST`` / ``RadialGradientConvergence`` — Liquid Engine operators whose
+    run type ``HIP_gfx950`` calls hand-written HIP kernels through the ctypes C ABI in include/npx_esrrf.h
+  * ``calculate_eSRRF_temporal_correlations``, ``macro_pixel_corrector``
+"""
+from .methods import eSRRF, Workflow
+from .core.transform import (RadialGradientConvergence, calculate_eSRRF_temporal_correlations, eSRRF_ST,
+                             macro_pixel_corrector)
+
+__version__ = "0.1.0"
+
+__all__ = ["eSRRF", "Workflow", "eSRRF_ST", "RadialGradientConvergence", "calculate_eSRRF_temporal_correlations",
+           "macro_pixel_corrector"]

@@ -1,0 +1,58 @@
+"""Build of libnpx_hip.so (gfx950) in-tree: nanopyx_amd/lib/libnpx_hip.so.
+
+Device code is compiled with packed FP32 disabled (``-packed-fp32-ops``): every FP32 operation stays a
+single-lane VALU op. The host-side plan (npx_plan.cpp) is built with ``-ffp-contract=off`` so its float32
+roundings follow the reference's arithmetic operation by operation.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+OUT_DIR = os.path.join(PKG, "lib")
+BUILD_DIR = os.path.join(PKG, "_build")
+LIB = os.path.join(OUT_DIR, "libnpx_hip.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({' '.join(cmd[:3])} ...):\n{r.stdout}")
+    if verbose and out:
+        print(out, flush=True)
+
+
+def build(verbose: bool = False, extra: list[str] | None = None) -> str:
+    """Compile every source of the library; always rebuilds. Returns the library path."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    extra = list(extra or [])
+    inc = ["-I", CSRC, "-I", os.path.join(os.path.dirname(PKG), "include")]
+    plan_o = os.path.join(BUILD_DIR, "npx_plan.o")
+    kern_o = os.path.join(BUILD_DIR, "npx_kernels.o")
+    capi_o = os.path.join(BUILD_DIR, "npx_capi.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", *inc, "-c",
+          os.path.join(CSRC, "npx_plan.cpp"), "-o", plan_o], verbose)
+    _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc, *extra, "-c",
+          os.path.join(CSRC, "npx_kernels.hip"), "-o", kern_o], verbose)
+    _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc, "-x", "hip", "-c",
+          os.path.join(CSRC, "npx_capi.cpp"), "-o", capi_o], verbose)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, capi_o, plan_o, "-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,246 @@
+"""ctypes binding of libnpx_hip.so (include/npx_esrrf.h) with PyTorch-ROCm tensors as device memory.
+
+There is no CPU path behind these functions: if the library or a ROCm GPU is missing they raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnpx_hip.so")
+_lib = None
+_lock = threading.Lock()
+
+NPX_AVG, NPX_VAR, NPX_TAC2 = 1, 2, 4
+MODE_BITS = {"AVG": NPX_AVG, "VAR": NPX_VAR, "TAC2": NPX_TAC2}
+DTYPE_F32, DTYPE_U16 = 0, 1
+
+_c_int, _c_double, _c_void_p, _c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class NpxError(RuntimeError):
+    """A libnpx_hip call failed (HIP error, unsupported parameters, or bad workspace)."""
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Loads libnpx_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(_LIB_PATH):
+            raise NpxError(f"{_LIB_PATH} is missing: build it with `python -m nanopyx_amd._build` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.npx_version.restype = _c_int
+        L.npx_last_error.restype = ctypes.c_char_p
+        L.npx_device_info.argtypes = [ctypes.c_char_p, _c_int]
+        L.npx_esrrf_frames.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_double,
+                                       _c_int, _c_void_p, _c_void_p]
+        L.npx_esrrf_temporal_workspace.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double]
+        L.npx_esrrf_temporal_workspace.restype = _c_size_t
+        L.npx_esrrf_temporal.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_double,
+                                         _c_int, ctypes.c_uint, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                         _c_size_t, _c_void_p]
+        L.npx_rgc.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_double,
+                              _c_double, _c_int, _c_int, _c_void_p, _c_void_p]
+        L.npx_temporal.argtypes = [_c_void_p, _c_int, ctypes.c_longlong, ctypes.c_uint, _c_void_p, _c_void_p,
+                                   _c_void_p, _c_void_p]
+        L.npx_macro_pixel.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]
+        L.npx_plan_info.argtypes = [_c_int, _c_int, _c_int, _c_double, _ip, _fp]
+        L.npx_plan_tables.argtypes = [_c_int, _c_int, _c_int, _c_double, _fp, _fp, _ip, _fp, _ip, _fp, _ip, _fp,
+                                      _ip, _fp]
+        L.npx_timing.argtypes = [_c_int]
+        L.npx_timing_read.argtypes = [_ip, ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]
+        L.npx_release.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().npx_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise NpxError(f"libnpx_hip error {rc}: {msg}")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise NpxError("the HIP_gfx950 run type needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream(torch, stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _dev_stack(x):
+    """(N, H, W) float32 or uint16 device tensor from a NumPy array or tensor (uint16 stays uint16: the kernel
+    converts it exactly as ``astype(float32)`` would; other dtypes are cast to float32 on the host first)."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.dtype not in (torch.float32, torch.uint16):
+            t = t.to(torch.float32)
+        if not t.is_cuda:
+            t = t.cuda()
+    else:
+        a = np.asarray(x)
+        if a.dtype != np.uint16 and a.dtype != np.float32:
+            a = a.astype(np.float32, copy=False)
+        t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if t.dim() == 2:
+        t = t.unsqueeze(0)
+    if t.dim() != 3:
+        raise ValueError("Image must be 2D and 3D (sequence of 2D images)")
+    return t.contiguous()
+
+
+def _dtype_code(t) -> int:
+    import torch
+    return DTYPE_U16 if t.dtype == torch.uint16 else DTYPE_F32
+
+
+def esrrf_frames(stack, magnification, radius, sensitivity, iw, stream=None):
+    """Per-frame eSRRF maps as a device tensor (N, H*M, W*M)."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    M = int(magnification)
+    out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
+    _check(lib().npx_esrrf_frames(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
+                                  int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+    return out
+
+
+def temporal_workspace_bytes(N, H, W, M, R) -> int:
+    return int(lib().npx_esrrf_temporal_workspace(N, H, W, int(M), float(R)))
+
+
+def esrrf_temporal(stack, magnification, radius, sensitivity, iw, modes=("AVG",), stream=None, workspace=True):
+    """Fused eSRRF + temporal correlation; returns {mode: device tensor (H*M, W*M)}."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    M = int(magnification)
+    mask = 0
+    outs = {}
+    for m in modes:
+        if m not in MODE_BITS:
+            raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+        mask |= MODE_BITS[m]
+        outs[m] = torch.empty((H * M, W * M), dtype=torch.float32, device=x.device)
+    nbytes = temporal_workspace_bytes(N, H, W, M, radius) if workspace else 0
+    work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=x.device) if nbytes else None
+    ptr = lambda k: outs[k].data_ptr() if k in outs else None
+    _check(lib().npx_esrrf_temporal(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
+                                    int(bool(iw)), mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"),
+                                    work.data_ptr() if work is not None else None, nbytes, _stream(torch, stream)))
+    if work is not None:
+        work.record_stream(torch.cuda.current_stream() if stream is None else stream)
+    return outs
+
+
+def rgc(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=None):
+    torch = _torch()
+    gx, gy, img = (_dev_stack(a) for a in (gx, gy, img))
+    gx, gy, img = (a.to(torch.float32) for a in (gx, gy, img))
+    N, HM, WM = img.shape
+    if tuple(gx.shape) != (N, 2 * HM, 2 * WM) or tuple(gy.shape) != (N, 2 * HM, 2 * WM):
+        raise ValueError("gradient arrays must be (N, 2*rowsM, 2*colsM) for an (N, rowsM, colsM) image")
+    out = torch.empty_like(img)
+    _check(lib().npx_rgc(gx.data_ptr(), gy.data_ptr(), img.data_ptr(), N, HM, WM, int(magnification), float(radius),
+                         float(sensitivity), int(bool(iw)), int(bool(pad_edges)), out.data_ptr(),
+                         _stream(torch, stream)))
+    return out
+
+
+def temporal(stack, modes, stream=None):
+    """AVG / VAR / TAC2 over axis 0 of an (N, ...) stack; returns {mode: device tensor}."""
+    torch = _torch()
+    x = stack if isinstance(stack, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(stack, np.float32))
+    x = x.to(device="cuda", dtype=torch.float32).contiguous()
+    N = x.shape[0]
+    P = int(np.prod(x.shape[1:]))
+    outs, mask = {}, 0
+    for m in modes:
+        if m not in MODE_BITS:
+            raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+        mask |= MODE_BITS[m]
+        outs[m] = torch.empty(tuple(x.shape[1:]), dtype=torch.float32, device=x.device)
+    ptr = lambda k: outs[k].data_ptr() if k in outs else None
+    _check(lib().npx_temporal(x.data_ptr(), N, P, mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"), _stream(torch, stream)))
+    return outs
+
+
+def macro_pixel_(img, magnification, stream=None):
+    """In-place macro-pixel correction of a (S, HM, WM) float32 device tensor."""
+    torch = _torch()
+    S, HM, WM = img.shape
+    M = int(magnification)
+    cmap = torch.empty(S * M * M, dtype=torch.float32, device=img.device)
+    _check(lib().npx_macro_pixel(img.data_ptr(), S, HM, WM, M, cmap.data_ptr(), _stream(torch, stream)))
+    return img
+
+
+def plan_info(H, W, M, R):
+    """Host-only view of the kernel geometry and reference-derived constants."""
+    info = (ctypes.c_int * 16)()
+    finfo = (ctypes.c_float * 4)()
+    _check(lib().npx_plan_info(H, W, M, float(R), info, finfo))
+    keys = ("K", "margin", "HM", "WM", "nphase", "Q", "HG", "HI", "TIR", "TIC", "TGR", "TGC", "LR", "LP", "nwaves",
+            "ngroups")
+    d = dict(zip(keys, list(info)))
+    d.update(fwhm=finfo[0], sigma=finfo[1], tSS=finfo[2], tSO=finfo[3])
+    return d
+
+
+def plan_tables(H, W, M, R):
+    """Host copies of the kernel tables (ring, wsum, lattice and intensity taps)."""
+    d = plan_info(H, W, M, R)
+    K, P, NW = d["K"], d["nphase"], 2 * d["K"] + 1
+    ring = np.zeros((P, NW, NW, 4), np.float32)
+    wsum = np.zeros(P, np.float32)
+    arr = {}
+    for name, n in (("lr", 2 * H + 1), ("lc", 2 * W + 1), ("cr", d["HM"]), ("cc", d["WM"])):
+        arr[name + "b"] = np.zeros(n, np.int32)
+        arr[name + "w"] = np.zeros((n, 4), np.float32)
+    f = lambda a: a.ctypes.data_as(_fp)
+    i = lambda a: a.ctypes.data_as(_ip)
+    _check(lib().npx_plan_tables(H, W, M, float(R), f(ring), f(wsum), i(arr["lrb"]), f(arr["lrw"]), i(arr["lcb"]),
+                                 f(arr["lcw"]), i(arr["crb"]), f(arr["crw"]), i(arr["ccb"]), f(arr["ccw"])))
+    return dict(info=d, ring=ring, wsum=wsum, **arr)
+
+
+def timing(on: bool) -> None:
+    """Record HIP events around every fused-kernel launch from now on (True) or stop (False)."""
+    _check(lib().npx_timing(int(bool(on))))
+
+
+def timing_read():
+    """(calls, total fused-kernel ms, total merge ms) of the launches recorded since the last read."""
+    n = ctypes.c_int(0)
+    k = ctypes.c_double(0)
+    m = ctypes.c_double(0)
+    _check(lib().npx_timing_read(ctypes.byref(n), ctypes.byref(k), ctypes.byref(m)))
+    return n.value, k.value, m.value
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().npx_device_info(buf, 256))
+    return buf.value.decode()

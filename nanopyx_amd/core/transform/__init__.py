@@ -1,0 +1,8 @@
+from ._le_esrrf import eSRRF as eSRRF_ST
+from ._le_esrrf import eSRRF
+from ._le_radial_gradient_convergence import RadialGradientConvergence
+from .mpcorrector import macro_pixel_corrector
+from .sr_temporal_correlations import calculate_eSRRF_temporal_correlations
+
+__all__ = ["eSRRF_ST", "eSRRF", "RadialGradientConvergence", "macro_pixel_corrector",
+           "calculate_eSRRF_temporal_correlations"]

@@ -1,0 +1,469 @@
+// C ABI of libnpx_hip.so (declarations and reference citations: include/npx_esrrf.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/npx_esrrf.h"
+#include "npx_kernels.h"
+#include "npx_plan.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(NPX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------------------------------ plan cache
+
+struct DevCopy {
+    void* mem = nullptr;
+    npx::DevTables t{};
+};
+
+struct Plan {
+    npx::PlanHost host;
+    std::map<int, DevCopy> dev;  // per device id
+};
+
+using Key = std::tuple<int, int, int, uint64_t>;
+std::mutex g_mu;
+std::map<Key, std::unique_ptr<Plan>> g_plans;
+
+uint64_t bits(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+}
+
+// Returns the cached host plan (built on first use) or nullptr with g_err set.
+Plan* get_plan(int H, int W, int M, double R) {
+    const Key k{H, W, M, bits(R)};
+    auto it = g_plans.find(k);
+    if (it != g_plans.end()) return it->second.get();
+    auto p = std::make_unique<Plan>();
+    std::string err;
+    if (!npx::build_plan(H, W, M, R, p->host, &err)) {
+        g_err = err;
+        return nullptr;
+    }
+    Plan* raw = p.get();
+    g_plans.emplace(k, std::move(p));
+    return raw;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Uploads the plan's tables to the current device once (blocking; completes before returning).
+int device_tables(Plan* p, const npx::DevTables** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    auto it = p->dev.find(dev);
+    if (it != p->dev.end()) {
+        *out = &it->second.t;
+        return NPX_OK;
+    }
+    const npx::PlanHost& h = p->host;
+    auto taps = [](const std::vector<npx::CrTap>& v, std::vector<float>& w, std::vector<int>& b) {
+        w.resize(v.size() * 4);
+        b.resize(v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            b[i] = v[i].base;
+            for (int u = 0; u < 4; ++u) w[4 * i + u] = v[i].w[u];
+        }
+    };
+    std::vector<float> lrw, lcw, crw, ccw;
+    std::vector<int> lrb, lcb, crb, ccb;
+    taps(h.lat_row, lrw, lrb);
+    taps(h.lat_col, lcw, lcb);
+    taps(h.cr_row, crw, crb);
+    taps(h.cr_col, ccw, ccb);
+    struct Seg {
+        const void* src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<Seg> segs = {
+        {h.ring.data(), h.ring.size() * 4, 0}, {h.wsum.data(), h.wsum.size() * 4, 0},
+        {lrw.data(), lrw.size() * 4, 0},       {lrb.data(), lrb.size() * 4, 0},
+        {lcw.data(), lcw.size() * 4, 0},       {lcb.data(), lcb.size() * 4, 0},
+        {crw.data(), crw.size() * 4, 0},       {crb.data(), crb.size() * 4, 0},
+        {ccw.data(), ccw.size() * 4, 0},       {ccb.data(), ccb.size() * 4, 0},
+    };
+    size_t total = 0;
+    for (auto& s : segs) {
+        s.off = total;
+        total += align256(s.bytes);
+    }
+    std::vector<uint8_t> blob(total, 0);
+    for (auto& s : segs) std::memcpy(blob.data() + s.off, s.src, s.bytes);
+    void* mem = nullptr;
+    e = hipMalloc(&mem, total);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan tables)");
+    e = hipMemcpy(mem, blob.data(), total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(mem);
+        return hip_fail(e, "upload of plan tables");
+    }
+    auto* base = static_cast<uint8_t*>(mem);
+    DevCopy dc;
+    dc.mem = mem;
+    dc.t.ring = reinterpret_cast<const float4*>(base + segs[0].off);
+    dc.t.wsum = reinterpret_cast<const float*>(base + segs[1].off);
+    dc.t.lrw = reinterpret_cast<const float4*>(base + segs[2].off);
+    dc.t.lrb = reinterpret_cast<const int*>(base + segs[3].off);
+    dc.t.lcw = reinterpret_cast<const float4*>(base + segs[4].off);
+    dc.t.lcb = reinterpret_cast<const int*>(base + segs[5].off);
+    dc.t.crw = reinterpret_cast<const float4*>(base + segs[6].off);
+    dc.t.crb = reinterpret_cast<const int*>(base + segs[7].off);
+    dc.t.ccw = reinterpret_cast<const float4*>(base + segs[8].off);
+    dc.t.ccb = reinterpret_cast<const int*>(base + segs[9].off);
+    auto res = p->dev.emplace(dev, dc);
+    *out = &res.first->second.t;
+    return NPX_OK;
+}
+
+// ------------------------------------------------------------------------------------------ kernel timing
+
+// Optional per-call HIP events around the fused kernel and the merge kernel (npx_timing / npx_timing_read).
+struct CallEvents {
+    hipEvent_t e0, e1, e2;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<CallEvents> g_events;
+
+bool timing_begin(hipStream_t st, CallEvents* ev) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    if (!g_timing) return false;
+    if (hipEventCreate(&ev->e0) != hipSuccess) return false;
+    if (hipEventCreate(&ev->e1) != hipSuccess) return false;
+    if (hipEventCreate(&ev->e2) != hipSuccess) return false;
+    return hipEventRecord(ev->e0, st) == hipSuccess;
+}
+
+void timing_end(const CallEvents& ev) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_events.push_back(ev);
+}
+
+// ------------------------------------------------------------------------------------------ launch shape
+
+struct Shape {
+    int nwaves, ngroups, nregx, nregy;
+};
+
+Shape fused_shape(int H, int W, int M) {
+    Shape s;
+    const int nphase = M * M;
+    s.ngroups = (nphase + 15) / 16;
+    s.nwaves = (nphase + s.ngroups - 1) / s.ngroups;
+    s.nregx = (W + 63) / 64;
+    s.nregy = (H + npx::kQ - 1) / npx::kQ;
+    return s;
+}
+
+// Frame chunking of the temporal kernel: enough blocks to fill 256 CUs several times over, while each
+// chunk keeps >= 32 frames so the moment merge stays a small fraction of the work.
+void temporal_chunks(int N, const Shape& s, int* nchunk, int* fpc) {
+    const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
+    long long want = (2048 + spatial - 1) / spatial;
+    want = std::min<long long>(want, std::max(1, N / 32));
+    want = std::max<long long>(want, 1);
+    int f = static_cast<int>((N + want - 1) / want);
+    *fpc = std::max(f, 1);
+    *nchunk = (N + *fpc - 1) / *fpc;
+}
+
+int pow_mode_of(double S) {
+    const float s = static_cast<float>(S);
+    if (!(s > 1.0f)) return 0;
+    if (s == 2.0f) return 2;
+    if (s == 3.0f) return 3;
+    if (s == 4.0f) return 4;
+    return 1;
+}
+
+int check_common(const void* d_in, int dtype, int N, int H, int W, int M) {
+    if (!d_in) return fail(NPX_ERR_ARG, "input pointer is NULL");
+    if (dtype != NPX_DTYPE_F32 && dtype != NPX_DTYPE_U16) return fail(NPX_ERR_ARG, "dtype must be f32 or u16");
+    if (N < 1 || H < 1 || W < 1) return fail(NPX_ERR_ARG, "stack must have N, H, W >= 1");
+    if (M < 1 || M > 64) return fail(NPX_ERR_ARG, "magnification must be in [1, 64]");
+    return NPX_OK;
+}
+
+npx::FusedArgs base_args(const void* d_in, int dtype, int N, int H, int W, int M, const npx::PlanHost& h,
+                         const npx::DevTables* t, const Shape& s, double S, int iw) {
+    npx::FusedArgs a{};
+    a.in = d_in;
+    a.u16 = dtype == NPX_DTYPE_U16;
+    a.N = N;
+    a.H = H;
+    a.W = W;
+    a.M = M;
+    a.HM = h.HM;
+    a.WM = h.WM;
+    a.margin = h.margin;
+    a.nphase = h.nphase;
+    a.nwaves = s.nwaves;
+    a.ngroups = s.ngroups;
+    a.pow_mode = pow_mode_of(S);
+    a.S = static_cast<float>(S);
+    a.iw = iw ? 1 : 0;
+    a.t = *t;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int npx_version(void) { return 100; }
+
+const char* npx_last_error(void) { return g_err.c_str(); }
+
+int npx_device_info(char* buf, int len) {
+    if (!buf || len <= 0) return fail(NPX_ERR_ARG, "buffer is NULL");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    snprintf(buf, len, "%s;%s;%d;%d", p.name, p.gcnArchName, p.multiProcessorCount, p.clockRate);
+    return NPX_OK;
+}
+
+int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                     float* d_out, void* stream) {
+    int rc = check_common(d_in, dtype, N, H, W, M);
+    if (rc) return rc;
+    if (!d_out) return fail(NPX_ERR_ARG, "output pointer is NULL");
+    const npx::DevTables* t = nullptr;
+    Plan* p;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        p = get_plan(H, W, M, R);
+        if (!p) return NPX_ERR_UNSUPPORTED;
+        rc = device_tables(p, &t);
+        if (rc) return rc;
+    }
+    const Shape s = fused_shape(H, W, M);
+    npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
+    // one frame per block unless the grid's z extent would overflow
+    const long long zmax = 65535 / s.ngroups;
+    a.fpc = static_cast<int>((N + zmax - 1) / zmax);
+    a.nchunk = (N + a.fpc - 1) / a.fpc;
+    a.out = d_out;
+    dim3 grid(s.nregx, s.nregy, s.ngroups * a.nchunk);
+    dim3 block(64 * s.nwaves);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    CallEvents ev{};
+    const bool timed = timing_begin(st, &ev);
+    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, npx::kModeFrames, st);
+    if (e != hipSuccess) return hip_fail(e, "esrrf_fused(frames) launch");
+    if (timed && hipEventRecord(ev.e1, st) == hipSuccess && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
+    return NPX_OK;
+}
+
+size_t npx_esrrf_temporal_workspace(int N, int H, int W, int M, double R) {
+    (void)R;
+    if (N < 1 || H < 1 || W < 1 || M < 1) return 0;
+    const Shape s = fused_shape(H, W, M);
+    int nchunk, fpc;
+    temporal_chunks(N, s, &nchunk, &fpc);
+    if (nchunk <= 1) return 0;
+    return static_cast<size_t>(nchunk) * 5 * static_cast<size_t>(H) * M * static_cast<size_t>(W) * M * sizeof(float);
+}
+
+int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                       unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2, void* d_work, size_t work_bytes,
+                       void* stream) {
+    int rc = check_common(d_in, dtype, N, H, W, M);
+    if (rc) return rc;
+    if ((mode_mask & (NPX_AVG | NPX_VAR | NPX_TAC2)) == 0) return fail(NPX_ERR_ARG, "empty temporal mode mask");
+    if (((mode_mask & NPX_AVG) && !d_avg) || ((mode_mask & NPX_VAR) && !d_var) || ((mode_mask & NPX_TAC2) && !d_tac2))
+        return fail(NPX_ERR_ARG, "output pointer of a requested temporal mode is NULL");
+    const npx::DevTables* t = nullptr;
+    Plan* p;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        p = get_plan(H, W, M, R);
+        if (!p) return NPX_ERR_UNSUPPORTED;
+        rc = device_tables(p, &t);
+        if (rc) return rc;
+    }
+    const Shape s = fused_shape(H, W, M);
+    npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
+    int nchunk = 1, fpc = N;
+    const size_t need = npx_esrrf_temporal_workspace(N, H, W, M, R);
+    if (d_work && need > 0) {
+        if (work_bytes < need) return fail(NPX_ERR_WORKSPACE, "workspace smaller than npx_esrrf_temporal_workspace()");
+        temporal_chunks(N, s, &nchunk, &fpc);
+    }
+    a.fpc = fpc;
+    a.nchunk = nchunk;
+    a.avg = (mode_mask & NPX_AVG) ? d_avg : nullptr;
+    a.var = (mode_mask & NPX_VAR) ? d_var : nullptr;
+    a.tac2 = (mode_mask & NPX_TAC2) ? d_tac2 : nullptr;
+    a.part = nchunk > 1 ? static_cast<float*>(d_work) : nullptr;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    dim3 grid(s.nregx, s.nregy, s.ngroups * nchunk);
+    dim3 block(64 * s.nwaves);
+    CallEvents ev{};
+    const bool timed = timing_begin(st, &ev);
+    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, npx::kModeTemporal, st);
+    if (e != hipSuccess) return hip_fail(e, "esrrf_fused(temporal) launch");
+    if (timed) (void)hipEventRecord(ev.e1, st);
+    if (nchunk > 1) {
+        e = npx::launch_merge(a.part, nchunk, fpc, N, static_cast<size_t>(p->host.HM) * p->host.WM, a.avg, a.var,
+                              a.tac2, st);
+        if (e != hipSuccess) return hip_fail(e, "moments_merge launch");
+    }
+    if (timed && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
+    return NPX_OK;
+}
+
+int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int HM, int WM, int M, double R,
+            double S, int iw, int pad_edges, float* d_out, void* stream) {
+    if (!d_gx || !d_gy || !d_img || !d_out) return fail(NPX_ERR_ARG, "NULL array pointer");
+    if (N < 1 || HM < 1 || WM < 1 || M < 1) return fail(NPX_ERR_ARG, "bad shape");
+    const float sigma = static_cast<float>(R / 2.355);
+    const float fwhm = static_cast<float>(R);
+    const float tSS = static_cast<float>(2.0f * sigma) * sigma;
+    const float tSO = static_cast<float>(2.0f * sigma) + 1.0f;
+    const int margin = pad_edges ? 0 : static_cast<int>(static_cast<double>(fwhm) * M);
+    hipError_t e = npx::launch_rgc_arrays(d_gx, d_gy, d_img, N, HM, WM, M, fwhm, tSO, tSS, pow_mode_of(S),
+                                          static_cast<float>(S), iw ? 1 : 0, margin, d_out,
+                                          static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "rgc_arrays launch");
+    return NPX_OK;
+}
+
+int npx_temporal(const float* d_stack, int N, long long P, unsigned mode_mask, float* d_avg, float* d_var,
+                 float* d_tac2, void* stream) {
+    if (!d_stack || N < 1 || P < 1) return fail(NPX_ERR_ARG, "bad stack");
+    if ((mode_mask & (NPX_AVG | NPX_VAR | NPX_TAC2)) == 0) return fail(NPX_ERR_ARG, "empty temporal mode mask");
+    hipError_t e = npx::launch_moments_stack(d_stack, N, static_cast<size_t>(P), (mode_mask & NPX_AVG) ? d_avg : nullptr,
+                                             (mode_mask & NPX_VAR) ? d_var : nullptr,
+                                             (mode_mask & NPX_TAC2) ? d_tac2 : nullptr,
+                                             static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "moments_stack launch");
+    return NPX_OK;
+}
+
+int npx_macro_pixel(float* d_img, int S, int HM, int WM, int M, float* d_cmap, void* stream) {
+    if (!d_img || !d_cmap || S < 1 || HM < 1 || WM < 1 || M < 1) return fail(NPX_ERR_ARG, "bad arguments");
+    if (HM / M < 1 || WM / M < 1) return fail(NPX_ERR_ARG, "image smaller than one macro-pixel");
+    hipError_t e = npx::launch_macro_pixel(d_img, S, HM, WM, M, d_cmap, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "macro_pixel launch");
+    return NPX_OK;
+}
+
+int npx_plan_info(int H, int W, int M, double R, int* info, float* finfo) {
+    if (!info || !finfo) return fail(NPX_ERR_ARG, "NULL output");
+    std::lock_guard<std::mutex> lk(g_mu);
+    Plan* p = get_plan(H, W, M, R);
+    if (!p) return NPX_ERR_UNSUPPORTED;
+    const npx::PlanHost& h = p->host;
+    const Shape s = fused_shape(H, W, M);
+    const int K = h.K, Q = npx::kQ;
+    const int HG = K / 2 + 3, HI = HG + 1;
+    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, Q, HG, HI, Q + 2 * HI, 64 + 2 * HI, Q + 2 * HG,
+                       64 + 2 * HG, 2 * Q + 2 * K, 64 + K, s.nwaves, s.ngroups};
+    for (int i = 0; i < 16; ++i) info[i] = v[i];
+    finfo[0] = h.fwhm;
+    finfo[1] = h.sigma;
+    finfo[2] = h.tSS;
+    finfo[3] = h.tSO;
+    return NPX_OK;
+}
+
+int npx_plan_tables(int H, int W, int M, double R, float* ring, float* wsum, int* lrb, float* lrw, int* lcb, float* lcw,
+                    int* crb, float* crw, int* ccb, float* ccw) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Plan* p = get_plan(H, W, M, R);
+    if (!p) return NPX_ERR_UNSUPPORTED;
+    const npx::PlanHost& h = p->host;
+    if (ring) std::copy(h.ring.begin(), h.ring.end(), ring);
+    if (wsum) std::copy(h.wsum.begin(), h.wsum.end(), wsum);
+    auto put = [](const std::vector<npx::CrTap>& v, int* b, float* w) {
+        for (size_t i = 0; i < v.size(); ++i) {
+            if (b) b[i] = v[i].base;
+            if (w)
+                for (int u = 0; u < 4; ++u) w[4 * i + u] = v[i].w[u];
+        }
+    };
+    put(h.lat_row, lrb, lrw);
+    put(h.lat_col, lcb, lcw);
+    put(h.cr_row, crb, crw);
+    put(h.cr_col, ccb, ccw);
+    return NPX_OK;
+}
+
+int npx_timing(int on) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing = on != 0;
+    return NPX_OK;
+}
+
+int npx_timing_read(int* n_calls, double* kernel_ms, double* merge_ms) {
+    std::vector<CallEvents> evs;
+    {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        evs.swap(g_events);
+    }
+    double k = 0, m = 0;
+    int n = 0;
+    int rc = NPX_OK;
+    for (auto& ev : evs) {
+        float a = 0, b = 0;
+        hipError_t e = hipEventSynchronize(ev.e2);
+        if (e == hipSuccess) e = hipEventElapsedTime(&a, ev.e0, ev.e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev.e1, ev.e2);
+        if (e != hipSuccess) rc = hip_fail(e, "timing events");
+        else {
+            k += a;
+            m += b;
+            ++n;
+        }
+        (void)hipEventDestroy(ev.e0);
+        (void)hipEventDestroy(ev.e1);
+        (void)hipEventDestroy(ev.e2);
+    }
+    if (n_calls) *n_calls = n;
+    if (kernel_ms) *kernel_ms = k;
+    if (merge_ms) *merge_ms = m;
+    return rc;
+}
+
+void npx_release(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    for (auto& kv : g_plans) {
+        for (auto& d : kv.second->dev) {
+            if (hipSetDevice(d.first) == hipSuccess) (void)hipFree(d.second.mem);
+        }
+        kv.second->dev.clear();
+    }
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+
+}  // extern "C"

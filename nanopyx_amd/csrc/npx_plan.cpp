@@ -1,0 +1,138 @@
+// Host-side eSRRF plan tables (see npx_plan.h). Compiled for the host only; float64 is used here and
+// never on the GPU. The float32 roundings the reference performs are reproduced with explicit casts
+// (this file is built with -ffp-contract=off so no product is fused into a sum behind our back).
+#include <cmath>
+#include <string>
+
+#include "npx_plan.h"
+
+namespace npx {
+
+// Keys cubic, a = 0.5 (src/include/_c_interpolation_catmull_rom.c:6-18).
+static double cubic(double v) {
+    const double a = 0.5;
+    if (v < 0) v = -v;
+    if (v < 1) return v * v * (v * (-a + 2) + (a - 3)) + 1;
+    if (v < 2) return -a * v * v * v + 5 * a * v * v - 8 * a * v + 4 * a;
+    return 0.0;
+}
+
+CrTap cr_taps(int i, float mag, int n) {
+    CrTap t{};
+    // row = i / magnification (int / float in float), _le_interpolation_catmull_rom.pyx:165-168
+    const float r = static_cast<float>(i) / mag;
+    t.base = 0;
+    for (int u = 0; u < 4; ++u) t.w[u] = 0.0f;
+    if (r < 0 || r >= static_cast<float>(n)) return t;  // _c_interpolate returns 0 outside (:23-25)
+    const int r_int = static_cast<int>(std::floor(static_cast<float>(static_cast<double>(r) - 0.5)));
+    t.base = r_int - 1;
+    for (int u = 0; u < 4; ++u) {
+        const int rn = r_int - 1 + u;
+        if (rn < 0 || rn >= n) continue;  // neighbour skipped, no renormalisation (:37-45)
+        t.w[u] = static_cast<float>(cubic(static_cast<double>(r) - (rn + 0.5)));
+    }
+    return t;
+}
+
+bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
+    if (H < 1 || W < 1 || M < 1) {
+        if (err) *err = "image size and magnification must be >= 1";
+        return false;
+    }
+    p.H = H;
+    p.W = W;
+    p.M = M;
+    p.R = R;
+    p.HM = H * M;
+    p.WM = W * M;
+    // _le_radial_gradient_convergence.pyx:85-98: radius is a Python float, sigma = float(radius / 2.355)
+    p.sigma = static_cast<float>(R / 2.355);
+    p.fwhm = static_cast<float>(R);
+    p.tSS = static_cast<float>(2.0f * p.sigma) * p.sigma;
+    p.tSO = static_cast<float>(2.0f * p.sigma) + 1.0f;
+    p.margin = static_cast<int>(static_cast<double>(p.fwhm) * M);
+    // _c_sr_radial_gradient_convergence.c:27-28 with Gx_Gy_MAGNIFICATION = 2.0f
+    const int start = -static_cast<int>(2.0f * p.fwhm);
+    const int end = static_cast<int>(2.0f * p.fwhm + 1.0f);
+    p.K = -start;
+    if (end - 1 != p.K) {
+        if (err) *err = "asymmetric ring window (radius too large for float32)";
+        return false;
+    }
+    if (p.K > 6 || !(R > 0)) {
+        if (err) *err = "radius must be in (0, 3.5): ring windows wider than 13x13 lattice points are not built";
+        return false;
+    }
+    if (static_cast<long long>(2) * p.HM >= (1LL << 24) || static_cast<long long>(2) * p.WM >= (1LL << 24)) {
+        if (err) *err = "magnified image too large for exact float32 sampling coordinates";
+        return false;
+    }
+    const int K = p.K, NW = 2 * K + 1;
+    p.nphase = M * M;
+    p.ey.resize(M);
+    p.ex.resize(M);
+    for (int q = 0; q < M; ++q) {
+        p.ey[q] = (2 * q + 1) / M;
+        p.ex[q] = p.ey[q];
+    }
+    // Every output row rM = M*y + py must reach lattice row floor(2*yc) = 2*y + ey[py] with
+    // yc = float((rM + 0.5) / M) as the reference computes it (:21-22,31); check all rows and columns.
+    for (int dim = 0; dim < 2; ++dim) {
+        const int n = dim ? p.WM : p.HM;
+        for (int r = 0; r < n; ++r) {
+            const float yc = static_cast<float>((r + 0.5) / M);
+            const int a0 = static_cast<int>(2.0f * yc);
+            if (a0 != 2 * (r / M) + p.ey[r % M]) {
+                if (err) *err = "lattice base mismatch (image too large for float32 pixel centres)";
+                return false;
+            }
+        }
+    }
+    // Ring geometry per phase, in the exact phase-relative coordinates:
+    // dy = (ey + j)/2 - (py + 0.5)/M, dx = (ex + i)/2 - (px + 0.5)/M.
+    p.ring.assign(static_cast<size_t>(p.nphase) * NW * NW * 4, 0.0f);
+    p.wsum.assign(p.nphase, 0.0f);
+    const double tSS = p.tSS;
+    for (int ph = 0; ph < p.nphase; ++ph) {
+        const int py = ph / M, px = ph % M;
+        std::vector<float> wgrid(NW * NW, 0.0f);  // [j][i]
+        for (int j = -K; j <= K; ++j) {
+            for (int i = -K; i <= K; ++i) {
+                const double dye = (p.ey[py] + j) / 2.0 - (py + 0.5) / M;
+                const double dxe = (p.ex[px] + i) / 2.0 - (px + 0.5) / M;
+                const float dy = static_cast<float>(dye);
+                const float dx = static_cast<float>(dxe);
+                const float dd = static_cast<float>(dx * dx) + static_cast<float>(dy * dy);
+                const float d = static_cast<float>(std::sqrt(static_cast<double>(dd)));
+                if (!(d != 0.0f && d <= p.tSO)) continue;  // :44
+                // _c_calculate_dw (:4-6) in double, stored as float (:63)
+                const double dd64 = d;
+                const float w = static_cast<float>(std::pow(dd64 * std::exp((-dd64 * dd64) / tSS), 4));
+                const size_t e = ((static_cast<size_t>(ph) * NW + (i + K)) * NW + (j + K)) * 4;
+                p.ring[e + 0] = static_cast<float>(static_cast<double>(dx) * w / d);
+                p.ring[e + 1] = static_cast<float>(static_cast<double>(dy) * w / d);
+                p.ring[e + 2] = w;
+                wgrid[(j + K) * NW + (i + K)] = w;
+            }
+        }
+        float s = 0.0f;  // distanceWeightSum, float, j outer / i inner (:30-35,64)
+        for (int k = 0; k < NW * NW; ++k) s += wgrid[k];
+        p.wsum[ph] = s;
+    }
+    // Gradient lattice taps: lattice index a reads row pos = a*M of the x2M image, clamped to 2*H*M-1
+    // (:45-52); that row samples the gradient at pos / (2M) (_le_esrrf.pyx:192-193).
+    const float mag2 = static_cast<float>(2 * M);
+    p.lat_row.resize(2 * H + 1);
+    p.lat_col.resize(2 * W + 1);
+    for (int a = 0; a <= 2 * H; ++a) p.lat_row[a] = cr_taps(a < 2 * H ? a * M : 2 * H * M - 1, mag2, H);
+    for (int b = 0; b <= 2 * W; ++b) p.lat_col[b] = cr_taps(b < 2 * W ? b * M : 2 * W * M - 1, mag2, W);
+    // Intensity Catmull-Rom taps at x M (_le_esrrf.pyx:190).
+    const float mag = static_cast<float>(M);
+    p.cr_row.resize(p.HM);
+    p.cr_col.resize(p.WM);
+    for (int r = 0; r < p.HM; ++r) p.cr_row[r] = cr_taps(r, mag, H);
+    for (int c = 0; c < p.WM; ++c) p.cr_col[c] = cr_taps(c, mag, W);
+    return true;
+}
+
+}  // namespace npx

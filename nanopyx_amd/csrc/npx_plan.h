@@ -1,0 +1,44 @@
+// Host-side eSRRF plan: every quantity of the reference algorithm that depends only on the image size,
+// the magnification and the radius, computed once in float64 on the host exactly as the reference
+// derives it, then handed to the gfx950 kernels as small FP32 tables.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace npx {
+
+// One tap row of a separable 4-tap Catmull-Rom evaluation: first source index and four weights
+// (weights of taps outside the image are 0, the reference skips them).
+struct CrTap {
+    int base;
+    float w[4];
+};
+
+struct PlanHost {
+    int H = 0, W = 0, M = 0;
+    double R = 0;
+    // reference-derived constants (src/nanopyx/core/transform/_le_radial_gradient_convergence.pyx:85-98)
+    float fwhm = 0, sigma = 0, tSS = 0, tSO = 0;
+    int K = 0;            // window half-width int(2*fwhm) (_c_sr_radial_gradient_convergence.c:27-28)
+    int margin = 0;       // int(fwhm * M)
+    int HM = 0, WM = 0;
+    int nphase = 0;       // M * M output phases
+    // ring tables per phase, [phase][i + K][j + K] -> (A, B, w, 0) with A = dx*w/d, B = dy*w/d
+    std::vector<float> ring;
+    std::vector<float> wsum;        // [phase] float32 sum of w in the reference's j-outer/i-inner order
+    std::vector<int> ey, ex;        // [phase row], [phase col]: floor((2p+1)/M)
+    // Catmull-Rom taps for the x2M gradient lattice (index a in [0, 2H], 2H = the clamped last row)
+    std::vector<CrTap> lat_row, lat_col;
+    // Catmull-Rom taps for the xM intensity image (index rM in [0, HM), cM in [0, WM))
+    std::vector<CrTap> cr_row, cr_col;
+};
+
+// Builds the tables; returns false with a message on unsupported input.
+bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err);
+
+// Reference-exact Catmull-Rom taps of one output coordinate i sampled at float(i) / float(mag)
+// (src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:165-168, _c_interpolation_catmull_rom.c:21-52).
+CrTap cr_taps(int i, float mag, int n);
+
+}  // namespace npx

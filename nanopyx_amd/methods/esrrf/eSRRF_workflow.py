@@ -1,0 +1,51 @@
+"""``eSRRF()`` workflow function (src/nanopyx/methods/esrrf/eSRRF_workflow.py:12-121) on the gfx950 path.
+
+Same signature and output: frames are split into timepoints of ``frames_per_timepoint`` (0 = all), each
+timepoint becomes one (H*M, W*M) map of the requested temporal correlation, optionally macro-pixel corrected,
+and the (T, H*M, W*M) float32 result is squeezed. On this path each timepoint is one fused launch
+(eSRRF + AVG/VAR/TAC2, ``npx_esrrf_temporal``) followed by the macro-pixel kernel; the stack is uploaded once and
+the per-frame maps never leave the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ...__agent__ import Agent
+from ...core.transform import eSRRF_ST
+from ...core.transform.__interpolation_tools__ import check_image
+from ... import _hip
+
+
+def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float = 1, frames_per_timepoint: int = 0,
+          temporal_correlation: str = "AVG", use_fht: bool = False, doIntensityWeighting: bool = True,
+          macro_pixel_correction: bool = True, pad_edges: bool = False, _force_run_type=None):
+    """Perform eSRRF analysis on an (N, H, W) stack; returns the squeezed (T, H*M, W*M) float32 result."""
+    if use_fht:
+        raise ValueError("use_fht=True (Fourier interpolation) is not part of the HIP_gfx950 run type")
+    if temporal_correlation not in ("AVG", "VAR", "TAC2"):
+        raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+    import torch
+
+    is_tensor = isinstance(image, torch.Tensor)
+    N = image.shape[0]
+    if frames_per_timepoint == 0:
+        frames_per_timepoint = N
+    elif frames_per_timepoint > N:
+        frames_per_timepoint = N
+    T = N // frames_per_timepoint + (1 if N % frames_per_timepoint else 0)
+    M = int(magnification)
+    stack = _hip._dev_stack(check_image(image, keep_uint16=True))
+    out = torch.zeros((T, image.shape[1] * M, image.shape[2] * M), dtype=torch.float32, device=stack.device)
+    op = eSRRF_ST(verbose=False)
+    for i in range(T):
+        part = stack[frames_per_timepoint * i:frames_per_timepoint * (i + 1)]
+        run_type = _force_run_type or Agent.get_run_type(op, (part,), {})
+        m = op.run_temporal(part, magnification=M, radius=radius, sensitivity=sensitivity,
+                            temporal_correlation=temporal_correlation, doIntensityWeighting=doIntensityWeighting,
+                            run_type=run_type)
+        Agent._inform(op)
+        if macro_pixel_correction:
+            m = _hip.macro_pixel_(m.unsqueeze(0).contiguous(), M)[0]
+        out[i] = m
+    out = torch.squeeze(out)
+    return out if is_tensor else out.cpu().numpy().astype(np.float32)

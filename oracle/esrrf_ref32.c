@@ -1,0 +1,277 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * FP32 CPU restatement of NanoPyx's threaded Cython eSRRF chain (v2.1.2), used as the checker for the
+ * HIP path and as the timed CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product (nanopyx_amd) never does.
+ *
+ * Every function follows a reference file:line (paths relative to the reference repo root). Storage is
+ * float32 and the per-pixel helpers compute in double exactly where the reference C does, so that this
+ * file and the reference's own C kernels (built into oracle/_ref by oracle/Makefile) give bit-identical
+ * results under the same compiler flags; tests/test_oracle.py checks that.
+ *
+ * The per-pixel kernels can be swapped for the reference's own compiled ones (ora_use_kernels), so the
+ * same loop drivers run either "port" (this file) or "reference" (reference C + these loops).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+/* ---------------------------------------------------------------- per-pixel kernels (restated) */
+
+/* src/include/_c_interpolation_catmull_rom.c:6-18 — Keys cubic, a = 0.5, in double. */
+static double ora_cubic(double v) {
+    double a = 0.5;
+    double z = 0;
+    if (v < 0) v = -v;
+    if (v < 1) {
+        z = v * v * (v * (-a + 2) + (a - 3)) + 1;
+    } else if (v < 2) {
+        z = -a * v * v * v + 5 * a * v * v - 8 * a * v + 4 * a;
+    }
+    return z;
+}
+
+/* src/include/_c_interpolation_catmull_rom.c:21-52 — 4x4 taps, pixel centres at n+0.5, neighbours
+ * outside the image skipped (no renormalisation), columns outer / rows inner, double accumulators. */
+static float ora_interpolate(float *image, float r, float c, int rows, int cols) {
+    if (r < 0 || r >= rows || c < 0 || c >= cols) return 0;
+    const int r_int = (int)floor((float)(r - 0.5));
+    const int c_int = (int)floor((float)(c - 0.5));
+    double q = 0;
+    double p = 0;
+    for (int j = 0; j < 4; j++) {
+        int cn = c_int - 1 + j;
+        p = 0;
+        if (cn < 0 || cn >= cols) continue;
+        for (int i = 0; i < 4; i++) {
+            int rn = r_int - 1 + i;
+            if (rn < 0 || rn >= rows) continue;
+            p = p + image[rn * cols + cn] * ora_cubic(r - (rn + 0.5));
+        }
+        q = q + p * ora_cubic(c - (cn + 0.5));
+    }
+    return q;
+}
+
+/* src/include/_c_gradients.c:53-72 — Roberts cross with the previous row/column clamped at 0. */
+static void ora_gradient_roberts_cross(float *image, float *imGc, float *imGr, int rows, int cols) {
+    for (int r1 = 0; r1 < rows; r1++) {
+        for (int c1 = 0; c1 < cols; c1++) {
+            int c0 = c1 > 0 ? c1 - 1 : 0;
+            int r0 = r1 > 0 ? r1 - 1 : 0;
+            float a_r0c1 = image[r0 * cols + c1];
+            float a_r1c0 = image[r1 * cols + c0];
+            float a_r0c0 = image[r0 * cols + c0];
+            float a_r1c1 = image[r1 * cols + c1];
+            imGc[r1 * cols + c1] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0;
+            imGr[r1 * cols + c1] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0;
+        }
+    }
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:4-6 */
+static double ora_dw(double distance, double tSS) {
+    return pow((distance * exp((-distance * distance) / tSS)), 4);
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:8-15 */
+static double ora_dk(float Gx, float Gy, float dx, float dy, float distance) {
+    float Dk = fabs(Gy * dx - Gx * dy) / sqrt(Gx * Gx + Gy * Gy);
+    if (isnan(Dk)) Dk = distance;
+    Dk = 1 - Dk / distance;
+    return Dk;
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:17-90 — ring sum for one output pixel. */
+static float ora_calculate_rgc(int xM, int yM, float *imIntGx, float *imIntGy, int colsM, int rowsM,
+                               int magnification, float G2, float fwhm, float tSO, float tSS,
+                               float sensitivity) {
+    float vx, vy, Gx, Gy, dx, dy, distance, distanceWeight, GdotR, Dk, pos_x, pos_y;
+    float xc = (xM + 0.5) / magnification;
+    float yc = (yM + 0.5) / magnification;
+    float RGC = 0;
+    float distanceWeightSum = 0;
+    int _start = -(int)(G2 * fwhm);
+    int _end = (int)(G2 * fwhm + 1);
+    for (int j = _start; j < _end; j++) {
+        vy = (int)(G2 * yc) + j;
+        vy /= G2;
+        if (0 < vy && vy <= rowsM - 1) {
+            for (int i = _start; i < _end; i++) {
+                vx = (int)(G2 * xc) + i;
+                vx /= G2;
+                if (0 < vx && vx <= colsM - 1) {
+                    dx = vx - xc;
+                    dy = vy - yc;
+                    distance = sqrt(dx * dx + dy * dy);
+                    if (distance != 0 && distance <= tSO) {
+                        pos_y = vy * magnification * G2;
+                        pos_x = vx * magnification * G2;
+                        if (pos_y >= rowsM * G2) pos_y = rowsM * G2 - 1;
+                        if (pos_x >= colsM * G2) pos_x = colsM * G2 - 1;
+                        if (pos_y < 0) pos_y = 0;
+                        if (pos_x < 0) pos_x = 0;
+                        int grad_width = (int)(colsM * G2);
+                        Gx = imIntGx[(int)pos_y * grad_width + (int)pos_x];
+                        Gy = imIntGy[(int)pos_y * grad_width + (int)pos_x];
+                        distanceWeight = ora_dw(distance, tSS);
+                        distanceWeightSum += distanceWeight;
+                        GdotR = Gx * dx + Gy * dy;
+                        if (GdotR < 0) {
+                            Dk = ora_dk(Gx, Gy, dx, dy, distance);
+                            RGC += Dk * distanceWeight;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (distanceWeightSum > 0) RGC /= distanceWeightSum;
+    else RGC = 0;
+    if (RGC >= 0 && sensitivity > 1) RGC = pow(RGC, sensitivity);
+    else if (RGC < 0) RGC = 0;
+    return RGC;
+}
+
+/* ---------------------------------------------------------------- kernel selection */
+
+typedef float (*rgc_fn_t)(int, int, float *, float *, int, int, int, float, float, float, float, float);
+typedef float (*interp_fn_t)(float *, float, float, int, int);
+typedef void (*roberts_fn_t)(float *, float *, float *, int, int);
+
+static rgc_fn_t k_rgc = ora_calculate_rgc;
+static interp_fn_t k_interp = ora_interpolate;
+static roberts_fn_t k_roberts = ora_gradient_roberts_cross;
+
+/* Swap the per-pixel kernels (NULL restores this file's restatement). The pointers are the
+ * reference's own _c_calculate_rgc / _c_interpolate / _c_gradient_roberts_cross from oracle/_ref. */
+void ora_use_kernels(void *rgc, void *interp, void *roberts) {
+    k_rgc = rgc ? (rgc_fn_t)rgc : ora_calculate_rgc;
+    k_interp = interp ? (interp_fn_t)interp : ora_interpolate;
+    k_roberts = roberts ? (roberts_fn_t)roberts : ora_gradient_roberts_cross;
+}
+
+/* Direct entry points to one per-pixel kernel (for the bit-exactness tests). */
+float ora_px_interpolate(float *image, float r, float c, int rows, int cols) {
+    return k_interp(image, r, c, rows, cols);
+}
+float ora_px_rgc(int xM, int yM, float *gx, float *gy, int colsM, int rowsM, int M, float G2, float fwhm,
+                 float tSO, float tSS, float S) {
+    return k_rgc(xM, yM, gx, gy, colsM, rowsM, M, G2, fwhm, tSO, tSS, S);
+}
+void ora_px_roberts(float *image, float *gc, float *gr, int rows, int cols) {
+    k_roberts(image, gc, gr, rows, cols);
+}
+
+int ora_num_threads(void) { return omp_get_max_threads(); }
+
+/* ---------------------------------------------------------------- stage drivers (Cython loops) */
+
+/* ShiftAndMagnify._run_threaded: src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:143-170.
+ * Output (N, int(rows*mr), int(cols*mc)); prange over output columns; coordinates are float divisions. */
+void ora_shift_magnify(const float *in, int nFrames, int rows, int cols, float shift_row, float shift_col,
+                       float magnification_row, float magnification_col, float *out) {
+    int rowsM = (int)(rows * magnification_row);
+    int colsM = (int)(cols * magnification_col);
+    for (int f = 0; f < nFrames; f++) {
+        float *img = (float *)in + (size_t)f * rows * cols;
+        float *o = out + (size_t)f * rowsM * colsM;
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < colsM; j++) {
+            float col = j / magnification_col - shift_col;
+            for (int i = 0; i < rowsM; i++) {
+                float row = i / magnification_row - shift_row;
+                o[(size_t)i * colsM + j] = k_interp(img, row, col, rows, cols);
+            }
+        }
+    }
+}
+
+/* GradientRobertsCross._run_threaded: src/nanopyx/core/transform/_le_roberts_cross_gradients.pyx:67-83
+ * (prange over frames). */
+void ora_roberts(const float *in, int nFrames, int rows, int cols, float *gc, float *gr) {
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nFrames; n++) {
+        size_t off = (size_t)n * rows * cols;
+        k_roberts((float *)in + off, gc + off, gr + off, rows, cols);
+    }
+}
+
+/* Parameter derivation of RadialGradientConvergence._run_threaded
+ * (src/nanopyx/core/transform/_le_radial_gradient_convergence.pyx:85-98): `radius` arrives as a Python
+ * float, so sigma = float(radius / 2.355) with the division in double; tSS, tSO are float arithmetic;
+ * margin = int(float(fwhm) * magnification) computed by Python in double. */
+void ora_rgc_params(double radius, int magnification, int pad_edges, float *fwhm, float *sigma, float *tSS,
+                    float *tSO, int *margin) {
+    float s = (float)(radius / 2.355);
+    float fw = (float)radius;
+    *sigma = s;
+    *fwhm = fw;
+    *tSS = 2 * s * s;
+    *tSO = 2 * s + 1;
+    *margin = pad_edges ? 0 : (int)((double)fw * (double)magnification);
+}
+
+/* RadialGradientConvergence._run_threaded: _le_radial_gradient_convergence.pyx:79-115.
+ * Frames sequential, prange over rows in [margin, rowsM - margin), zeros elsewhere. */
+void ora_rgc(const float *gci, const float *gri, const float *img_i, int nFrames, int rowsM, int colsM,
+             int magnification, double radius, double sensitivity, int doIntensityWeighting, int pad_edges,
+             float *out) {
+    float fwhm, sigma, tSS, tSO;
+    int margin;
+    ora_rgc_params(radius, magnification, pad_edges, &fwhm, &sigma, &tSS, &tSO, &margin);
+    float G2 = 2.0f;
+    float S = (float)sensitivity;
+    size_t plane = (size_t)rowsM * colsM;
+    size_t gplane = (size_t)(2 * rowsM) * (2 * colsM);
+    memset(out, 0, sizeof(float) * plane * nFrames);
+    for (int f = 0; f < nFrames; f++) {
+        float *gx = (float *)gci + f * gplane;
+        float *gy = (float *)gri + f * gplane;
+        const float *im = img_i + f * plane;
+        float *o = out + f * plane;
+#pragma omp parallel for schedule(static)
+        for (int rM = margin; rM < rowsM - margin; rM++) {
+            for (int cM = margin; cM < colsM - margin; cM++) {
+                float v = k_rgc(cM, rM, gx, gy, colsM, rowsM, magnification, G2, fwhm, tSO, tSS, S);
+                if (doIntensityWeighting) v = v * im[(size_t)rM * colsM + cM];
+                o[(size_t)rM * colsM + cM] = v;
+            }
+        }
+    }
+}
+
+/* eSRRF._run_threaded (non-FHT branch): src/nanopyx/core/transform/_le_esrrf.pyx:172-195.
+ * CR xM, Roberts, CR x2M of both gradients, RGC; pad_edges is not forwarded (:194). Frames are processed in
+ * chunks of `chunk` so the full-size intermediates stay bounded; each chunk runs the reference's stages. */
+int ora_esrrf(const float *in, int nFrames, int rows, int cols, int magnification, double radius,
+              double sensitivity, int doIntensityWeighting, int chunk, float *out) {
+    if (chunk <= 0) chunk = nFrames;
+    int M = magnification;
+    size_t p = (size_t)rows * cols, pM = (size_t)(rows * M) * (cols * M), p2 = 4 * pM;
+    float *mag = (float *)malloc(sizeof(float) * pM * chunk);
+    float *gc = (float *)malloc(sizeof(float) * p * chunk);
+    float *gr = (float *)malloc(sizeof(float) * p * chunk);
+    float *gci = (float *)malloc(sizeof(float) * p2 * chunk);
+    float *gri = (float *)malloc(sizeof(float) * p2 * chunk);
+    if (!mag || !gc || !gr || !gci || !gri) {
+        free(mag); free(gc); free(gr); free(gci); free(gri);
+        return -1;
+    }
+    for (int f0 = 0; f0 < nFrames; f0 += chunk) {
+        int n = nFrames - f0 < chunk ? nFrames - f0 : chunk;
+        const float *x = in + (size_t)f0 * p;
+        ora_shift_magnify(x, n, rows, cols, 0, 0, (float)M, (float)M, mag);
+        ora_roberts(x, n, rows, cols, gc, gr);
+        ora_shift_magnify(gc, n, rows, cols, 0, 0, (float)(M * 2), (float)(M * 2), gci);
+        ora_shift_magnify(gr, n, rows, cols, 0, 0, (float)(M * 2), (float)(M * 2), gri);
+        ora_rgc(gci, gri, mag, n, rows * M, cols * M, M, radius, sensitivity, doIntensityWeighting, 0,
+                out + (size_t)f0 * pM);
+    }
+    free(mag); free(gc); free(gr); free(gci); free(gri);
+    return 0;
+}
+

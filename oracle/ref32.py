@@ -1,0 +1,162 @@
+"""ORACLE (test infrastructure only) — ctypes front end of the FP32 restatement of the reference CPU path.
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg may import this module.
+
+``libnpx_oracle.so`` (oracle/esrrf_ref32.c, oracle/post_ref32.c) restates the reference's threaded Cython
+eSRRF chain stage for stage (see the C file for the file:line each function follows). With
+``use_reference_kernels(True)`` its loop drivers call the reference's own per-pixel C kernels, compiled
+from the reference sources into ``oracle/_ref/libnpx_ref.so`` by ``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libnpx_oracle.so")
+_REF_PATH = os.path.join(_HERE, "_ref", "libnpx_ref.so")
+_lib = None
+_ref = None
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle (and oracle/_ref when the reference sources are present) with make."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.ora_shift_magnify.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+        L.ora_roberts.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp]
+        L.ora_rgc.argtypes = [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _fp]
+        L.ora_esrrf.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int, _fp]
+        L.ora_esrrf.restype = ctypes.c_int
+        L.ora_temporal.argtypes = [_fp, ctypes.c_int, ctypes.c_long, ctypes.c_int, _fp]
+        L.ora_macro_pixel.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ora_use_kernels.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.ora_px_interpolate.argtypes = [_fp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        L.ora_px_interpolate.restype = ctypes.c_float
+        L.ora_px_rgc.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.ora_px_rgc.restype = ctypes.c_float
+        L.ora_px_roberts.argtypes = [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
+        L.ora_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def reference_available() -> bool:
+    return os.path.exists(_REF_PATH)
+
+
+def use_reference_kernels(on: bool) -> None:
+    """Route the per-pixel kernels to the reference's compiled C (True) or to the restatement (False)."""
+    global _ref
+    L = lib()
+    if not on:
+        L.ora_use_kernels(None, None, None)
+        return
+    if _ref is None:
+        _ref = ctypes.CDLL(_REF_PATH)
+    addr = lambda name: ctypes.cast(getattr(_ref, name), ctypes.c_void_p).value
+    L.ora_use_kernels(addr("_c_calculate_rgc"), addr("_c_interpolate"), addr("_c_gradient_roberts_cross"))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _p(a):
+    return a.ctypes.data_as(_fp)
+
+
+def shift_magnify(img, mag_row, mag_col, shift_row=0.0, shift_col=0.0):
+    img = _f32(img if np.ndim(img) == 3 else np.asarray(img)[None])
+    N, H, W = img.shape
+    out = np.zeros((N, int(H * np.float32(mag_row)), int(W * np.float32(mag_col))), np.float32)
+    lib().ora_shift_magnify(_p(img), N, H, W, shift_row, shift_col, mag_row, mag_col, _p(out))
+    return out
+
+
+def roberts(img):
+    img = _f32(img if np.ndim(img) == 3 else np.asarray(img)[None])
+    gc = np.zeros_like(img)
+    gr = np.zeros_like(img)
+    lib().ora_roberts(_p(img), *img.shape, _p(gc), _p(gr))
+    return gc, gr
+
+
+def rgc(gci, gri, img_i, magnification=5, radius=1.5, sensitivity=1, doIntensityWeighting=True, pad_edges=False):
+    gci, gri, img_i = _f32(gci), _f32(gri), _f32(img_i)
+    N, HM, WM = img_i.shape
+    out = np.zeros_like(img_i)
+    lib().ora_rgc(_p(gci), _p(gri), _p(img_i), N, HM, WM, int(magnification), float(radius), float(sensitivity),
+                  int(bool(doIntensityWeighting)), int(bool(pad_edges)), _p(out))
+    return out
+
+
+def esrrf(stack, magnification=5, radius=1.5, sensitivity=1, doIntensityWeighting=True, chunk=0):
+    """eSRRF_ST.run on the CPU ("Threaded" chain), (N, HM, WM) float32."""
+    stack = _f32(stack if np.ndim(stack) == 3 else np.asarray(stack)[None])
+    N, H, W = stack.shape
+    M = int(magnification)
+    out = np.zeros((N, H * M, W * M), np.float32)
+    rc = lib().ora_esrrf(_p(stack), N, H, W, M, float(radius), float(sensitivity), int(bool(doIntensityWeighting)),
+                         int(chunk), _p(out))
+    if rc != 0:
+        raise MemoryError("oracle eSRRF could not allocate its intermediates")
+    return out
+
+
+_MODES = {"AVG": 0, "VAR": 1, "TAC2": 2}
+
+
+def temporal(stack, mode):
+    stack = _f32(stack)
+    N = stack.shape[0]
+    P = int(np.prod(stack.shape[1:]))
+    out = np.zeros(stack.shape[1:], np.float32)
+    lib().ora_temporal(_p(stack), N, P, _MODES[mode], _p(out))
+    return out
+
+
+def macro_pixel(img, magnification):
+    x = np.array(img, dtype=np.float32, copy=True)
+    if x.ndim == 2:
+        x = x[None]
+    lib().ora_macro_pixel(_p(x), x.shape[0], x.shape[1], x.shape[2], int(magnification))
+    return x
+
+
+def esrrf_workflow(image, magnification=5, radius=1.5, sensitivity=1, frames_per_timepoint=0,
+                   temporal_correlation="AVG", doIntensityWeighting=True, macro_pixel_correction=True, chunk=0):
+    """eSRRF() of src/nanopyx/methods/esrrf/eSRRF_workflow.py:12-121 over the FP32 chain."""
+    image = np.asarray(image)
+    N = image.shape[0]
+    fpt = N if frames_per_timepoint == 0 or frames_per_timepoint > N else frames_per_timepoint
+    T = N // fpt + (1 if N % fpt else 0)
+    out = np.zeros((T, image.shape[1] * magnification, image.shape[2] * magnification), np.float32)
+    for t in range(T):
+        rgc_maps = esrrf(image[fpt * t:fpt * (t + 1)], magnification, radius, sensitivity, doIntensityWeighting,
+                         chunk)
+        m = temporal(rgc_maps, temporal_correlation)[None]
+        out[t] = macro_pixel(m, magnification)[0] if macro_pixel_correction else m[0]
+    return np.squeeze(out.astype(np.float32))
+
+
+def num_threads() -> int:
+    return lib().ora_num_threads()

@@ -1,0 +1,186 @@
+"""CPU model of the fused gfx950 kernel (nanopyx_amd/csrc/npx_kernels.hip, esrrf_fused), block by block.
+
+It repeats the kernel's index arithmetic statement for statement — tiles, halos, the two column-parity lattice
+planes, the per-lane column sweep and the edge normaliser — on the library's own plan tables, in float32, and
+asserts that every LDS / table / global index the kernel forms stays inside its buffer and that every tap with a
+non-zero weight reads inside its tile (clamped taps must carry weight 0). Tests run it on small stacks against the
+oracle, so an indexing mistake shows up on the CPU before the kernel ever runs on a GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from nanopyx_amd import _hip
+
+f32 = np.float32
+
+
+def model_frames(stack, M, R, S, iw=True):
+    stack = np.asarray(stack)
+    N, H, W = stack.shape
+    t = _hip.plan_tables(H, W, M, R)
+    d = t["info"]
+    K, Q, HG, HI = d["K"], d["Q"], d["HG"], d["HI"]
+    TIR, TIC, TGR, TGC, LR, LP = d["TIR"], d["TIC"], d["TGR"], d["TGC"], d["LR"], d["LP"]
+    HM, WM, margin, nphase = d["HM"], d["WM"], d["margin"], d["nphase"]
+    nwaves, ngroups = d["nwaves"], d["ngroups"]
+    NW = 2 * K + 1
+    NR = 2 * Q + 2 * K - 1
+    LPL = LR * LP
+    ring = t["ring"].reshape(nphase * NW * NW, 4)
+    wsum = t["wsum"]
+    Sf = f32(S)
+    pow_mode = 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
+    out = np.zeros((N, HM, WM), np.float32)
+    nregx, nregy = (W + 63) // 64, (H + Q - 1) // Q
+    lanes = np.arange(64)
+    for f in range(N):
+        img = stack[f].astype(np.float32)
+        for by in range(nregy):
+            for bx in range(nregx):
+                x0, y0 = bx * 64, by * Q
+                # (1) input tile
+                s_in = np.zeros((TIR, TIC), np.float32)
+                gy = y0 - HI + np.arange(TIR)
+                gx = x0 - HI + np.arange(TIC)
+                oky, okx = (gy >= 0) & (gy < H), (gx >= 0) & (gx < W)
+                s_in[np.ix_(oky, okx)] = img[np.ix_(gy[oky], gx[okx])]
+                # (2) Roberts
+                s_gc = np.zeros((TGR, TGC), np.float32)
+                s_gr = np.zeros((TGR, TGC), np.float32)
+                for r in range(TGR):
+                    yy = y0 - HG + r
+                    if yy < 0 or yy >= H:
+                        continue
+                    cc = np.arange(TGC)
+                    xx = x0 - HG + cc
+                    ok = (xx >= 0) & (xx < W)
+                    r1 = yy - (y0 - HI)
+                    r0 = (yy - 1 if yy > 0 else 0) - (y0 - HI)
+                    c1 = xx - (x0 - HI)
+                    c0 = np.where(xx > 0, xx - 1, 0) - (x0 - HI)
+                    assert 0 <= r0 < TIR and 0 <= r1 < TIR
+                    assert (c0[ok] >= 0).all() and (c1[ok] < TIC).all()
+                    a_r0c1 = s_in[r0, c1[ok]]
+                    a_r1c0 = s_in[r1, c0[ok]]
+                    a_r0c0 = s_in[r0, c0[ok]]
+                    a_r1c1 = s_in[r1, c1[ok]]
+                    s_gc[r, ok] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0
+                    s_gr[r, ok] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0
+                # (3) lattice
+                s_lat = np.zeros((2 * LPL, 2), np.float32)
+                a_lo, b_lo = 2 * y0 - K, 2 * x0 - K
+                for idx in range(2 * LPL):
+                    plane, rem = divmod(idx, LPL)
+                    la, pos = divmod(rem, LP)
+                    aa, bb = a_lo + la, b_lo + 2 * pos + plane
+                    if not (1 <= aa <= 2 * HM - 2 and 1 <= bb <= 2 * WM - 2):
+                        continue
+                    ai, bi = min(aa, 2 * H), min(bb, 2 * W)
+                    wr, wc = t["lrw"][ai], t["lcw"][bi]
+                    rb = t["lrb"][ai] - (y0 - HG)
+                    cb = t["lcb"][bi] - (x0 - HG)
+                    gxv = f32(0)
+                    gyv = f32(0)
+                    for v in range(4):
+                        tc = min(max(cb + v, 0), TGC - 1)
+                        if wc[v] != 0:
+                            assert 0 <= cb + v < TGC, ("lattice column tap outside tile", aa, bb)
+                        sx = f32(0)
+                        sy = f32(0)
+                        for u in range(4):
+                            tr = min(max(rb + u, 0), TGR - 1)
+                            if wr[u] != 0 and wc[v] != 0:
+                                assert 0 <= rb + u < TGR, ("lattice row tap outside tile", aa, bb)
+                            sx = f32(sx + s_gc[tr, tc] * wr[u])
+                            sy = f32(sy + s_gr[tr, tc] * wr[u])
+                        gxv = f32(gxv + sx * wc[v])
+                        gyv = f32(gyv + sy * wc[v])
+                    n2 = f32(gxv * gxv + gyv * gyv)
+                    if n2 > 0:
+                        inv = f32(1.0 / np.sqrt(np.float64(n2)))
+                        s_lat[idx] = (gxv * inv, gyv * inv)
+                # (4)+(5) waves
+                for group in range(ngroups):
+                    for wave in range(nwaves):
+                        ph = group * nwaves + wave
+                        if ph >= nphase:
+                            continue
+                        py, px = divmod(ph, M)
+                        ey, ex = (2 * py + 1) // M, (2 * px + 1) // M
+                        xi = x0 + lanes
+                        cM = M * xi + px
+                        acc = np.zeros((Q, 64), np.float32)
+                        for i in range(-K, K + 1):
+                            c = ex + i + K
+                            base = (c & 1) * LPL + ey * LP + lanes + (c >> 1)
+                            rows = base[None, :] + np.arange(NR)[:, None] * LP
+                            assert rows.min() >= 0 and rows.max() < 2 * LPL
+                            g = s_lat[rows]                       # (NR, 64, 2)
+                            for jj in range(NW):
+                                e = ph * NW * NW + (i + K) * NW + jj
+                                A, B, w = ring[e, 0], ring[e, 1], ring[e, 2]
+                                for q in range(Q):
+                                    gg = g[2 * q + jj]
+                                    dot = gg[:, 0] * A + gg[:, 1] * B
+                                    crs = gg[:, 1] * A - gg[:, 0] * B
+                                    y = w - np.abs(crs)
+                                    contrib = np.median(np.stack([np.zeros(64, np.float32), -dot, y]), axis=0)
+                                    acc[q] = acc[q] + contrib.astype(np.float32)
+                        cMs = np.minimum(cM, WM - 1)
+                        wcol = t["ccw"][cMs]
+                        ccol = t["ccb"][cMs] - (x0 - HI)
+                        for q in range(Q):
+                            yi = y0 + q
+                            rM = M * yi + py
+                            inside = (xi < W) & (yi < H)
+                            valid = inside & (rM >= margin) & (rM < HM - margin) & (cM >= margin) & (cM < WM - margin)
+                            a0 = 2 * yi + ey
+                            b0 = 2 * xi + ex
+                            interior = (a0 - K >= 1) & (a0 + K <= 2 * HM - 2) & (b0 - K >= 1) & (b0 + K <= 2 * WM - 2)
+                            den = np.full(64, wsum[ph], np.float32)
+                            for L in np.nonzero(valid & ~interior)[0]:
+                                s = f32(0)
+                                for jj in range(NW):
+                                    aa = a0 + jj - K
+                                    if aa < 1 or aa > 2 * HM - 2:
+                                        continue
+                                    for ii in range(NW):
+                                        bb = b0[L] + ii - K
+                                        if bb < 1 or bb > 2 * WM - 2:
+                                            continue
+                                        s = f32(s + ring[ph * NW * NW + ii * NW + jj, 2])
+                                den[L] = s
+                            with np.errstate(invalid="ignore", divide="ignore"):
+                                rgc = np.where(den > 0, acc[q] / den, 0).astype(np.float32)
+                            if pow_mode == 2:
+                                rgc = rgc * rgc
+                            elif pow_mode == 3:
+                                rgc = rgc * rgc * rgc
+                            elif pow_mode == 4:
+                                rgc = (rgc * rgc) * (rgc * rgc)
+                            elif pow_mode == 1:
+                                rgc = np.power(rgc, Sf).astype(np.float32)
+                            if iw:
+                                rMs = min(rM, HM - 1)
+                                wrow = t["crw"][rMs]
+                                crow = t["crb"][rMs] - (y0 - HI)
+                                s = np.zeros(64, np.float32)
+                                for v in range(4):
+                                    tc = np.clip(ccol + v, 0, TIC - 1)
+                                    bad = (wcol[:, v] != 0) & ((ccol + v < 0) | (ccol + v >= TIC)) & valid
+                                    assert not bad.any(), "intensity column tap outside tile"
+                                    p = np.zeros(64, np.float32)
+                                    for u in range(4):
+                                        tr = min(max(crow + u, 0), TIR - 1)
+                                        if wrow[u] != 0 and valid.any():
+                                            assert 0 <= crow + u < TIR, "intensity row tap outside tile"
+                                        p = p + s_in[tr, tc] * wrow[u]
+                                    s = s + p * wcol[:, v]
+                                rgc = rgc * s
+                            v_out = np.where(valid, rgc, 0).astype(np.float32)
+                            sel = inside
+                            if sel.any():
+                                assert rM < HM and (cM[sel] < WM).all()
+                                out[f, rM, cM[sel]] = v_out[sel]
+    return out

@@ -1,0 +1,82 @@
+"""The C ABI library loads without a GPU, exports every symbol include/npx_esrrf.h declares, and its host-only
+plan reproduces the reference's parameter derivation; argument errors come back as status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from nanopyx_amd import _hip
+from oracle import fp64
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "npx_esrrf.h")).read()
+    return sorted(set(re.findall(r"\b(npx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_exports_every_declared_symbol():
+    lib = _hip.lib()
+    names = _declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_version_and_error_paths():
+    lib = _hip.lib()
+    assert lib.npx_version() >= 100
+    rc = lib.npx_esrrf_frames(None, 0, 1, 8, 8, 4, 1.5, 1.0, 1, None, None)
+    assert rc == -1 and b"NULL" in lib.npx_last_error()
+    rc = lib.npx_esrrf_temporal(ctypes.c_void_p(16), 0, 1, 8, 8, 4, 1.5, 1.0, 1, 0, None, None, None, None, 0, None)
+    assert rc == -1
+    with pytest.raises(_hip.NpxError, match="radius"):
+        _hip.plan_info(8, 8, 4, 5.0)  # window wider than the kernels are built for
+
+
+@pytest.mark.parametrize("R,M", [(1.0, 2), (1.5, 5), (2.0, 4), (2.5, 8), (1.2345, 3)])
+def test_plan_constants_match_reference_derivation(R, M):
+    d = _hip.plan_info(64, 48, M, R)
+    p = fp64.ref_params(R, M)
+    assert d["K"] == -p["start"] == p["end"] - 1
+    assert d["margin"] == p["margin"]
+    assert np.float32(d["tSS"]) == np.float32(p["tSS"]) and np.float32(d["tSO"]) == np.float32(p["tSO"])
+    assert (d["HM"], d["WM"]) == (64 * M, 48 * M)
+
+
+def test_plan_ring_tables_against_fp64_geometry():
+    H = W = 16
+    for M, R in [(4, 2.0), (5, 1.5), (3, 2.5)]:
+        t = _hip.plan_tables(H, W, M, R)
+        K = t["info"]["K"]
+        p = fp64.ref_params(R, M)
+        for ph in range(M * M):
+            py, px = divmod(ph, M)
+            ey, ex = (2 * py + 1) // M, (2 * px + 1) // M
+            for j in range(-K, K + 1):
+                for i in range(-K, K + 1):
+                    A, B, w, _ = t["ring"][ph, i + K, j + K]
+                    dy = (ey + j) / 2 - (py + 0.5) / M
+                    dx = (ex + i) / 2 - (px + 0.5) / M
+                    d = np.hypot(dx, dy)
+                    inring = d != 0 and d <= p["tSO"]
+                    assert (w > 0) == inring
+                    if inring:
+                        dw = (d * np.exp(-d * d / p["tSS"])) ** 4
+                        assert abs(w - dw) <= 2e-5 * dw  # w is built from the float32 distance, as the reference does
+                        assert abs(A - dx * dw / d) <= 2e-5 * dw and abs(B - dy * dw / d) <= 2e-5 * dw
+
+
+def test_lattice_taps_are_catmull_rom_at_half_integers():
+    t = _hip.plan_tables(9, 9, 3, 1.5)
+    for a in range(1, 18):
+        w = t["lrw"][a]
+        if a % 2 == 0:       # integer coordinate a/2: [-1, 9, 9, -1]/16 minus taps outside the image
+            full = np.array([-1, 9, 9, -1]) / 16
+            base = a // 2 - 2
+            exp = np.array([full[u] if 0 <= base + u < 9 else 0 for u in range(4)])
+            assert t["lrb"][a] == base and np.allclose(w, exp)
+        else:                # n + 1/2: the pixel itself
+            assert t["lrb"][a] == (a - 1) // 2 - 1 and np.allclose(w, [0, 1, 0, 0])

@@ -1,0 +1,205 @@
+"""Parity of the HIP_gfx950 path with the reference (golden vectors from the reference's C kernels + NumPy),
+with the FP32 restatement of the reference CPU chain and with the FP64 oracle. Bar: max |x - ref| / max|ref|
+<= 1e-4 per output map (the eSRRF tolerance); the measured errors are printed."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    assert _torch.cuda.is_available(), "GPU tests need a ROCm device"
+    from nanopyx_amd import _hip
+    print("device:", _hip.device_info(), flush=True)
+    return _torch
+
+
+def _case(golden, name):
+    M, R, S, iw = golden[f"{name}/params"]
+    return golden[f"{name}/input"], int(M), float(R), float(S), bool(iw)
+
+
+def _report(tag, x, ref):
+    mx, mean = rel_err(x, ref)
+    print(f"{tag}: max {mx:.3e} mean {mean:.3e}", flush=True)
+    return mx
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_frames_match_golden(torch, golden, name):
+    from nanopyx_amd.core.transform import eSRRF_ST
+    from oracle import fp64
+    x, M, R, S, iw = _case(golden, name)
+    print(f"case {name} shape {x.shape} dtype {x.dtype}", flush=True)
+    out = eSRRF_ST(verbose=False).run(x, magnification=M, radius=R, sensitivity=S, doIntensityWeighting=iw)
+    assert out.shape == golden[f"{name}/rgc_maps"].shape and out.dtype == np.float32
+    assert _report(f"{name} frames vs reference", out, golden[f"{name}/rgc_maps"]) <= TOL
+    assert _report(f"{name} frames vs fp64", out, fp64.esrrf_frames(x, M, R, S, iw)) <= TOL
+    margin = int(float(np.float32(R)) * M)
+    if margin:
+        assert np.all(out[:, :margin] == 0) and np.all(out[:, :, -margin:] == 0)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_fused_temporal_match_golden(torch, golden, name):
+    from nanopyx_amd.core.transform import eSRRF_ST
+    x, M, R, S, iw = _case(golden, name)
+    print(f"case {name} temporal", flush=True)
+    outs = eSRRF_ST(verbose=False).run_temporal(x, M, R, S, temporal_correlation=("AVG", "VAR", "TAC2"),
+                                                doIntensityWeighting=iw)
+    for mode in ("AVG", "VAR", "TAC2"):
+        assert _report(f"{name} {mode} vs reference", outs[mode].cpu().numpy(), golden[f"{name}/{mode}"]) <= TOL
+
+
+def test_rgc_class_match_golden(torch, golden):
+    from nanopyx_amd.core.transform import RadialGradientConvergence
+    name = "m2_r10_s3_noiw"
+    gx, gy, mag = (golden[f"{name}/{k}"] for k in ("grad_col_interp", "grad_row_interp", "magnified"))
+    rgc = RadialGradientConvergence(verbose=False)
+    out = rgc.run(gx, gy, mag, magnification=2, radius=1.0, sensitivity=3.0, doIntensityWeighting=False)
+    assert _report("RGC class vs reference", out, golden[f"{name}/rgc_maps"]) <= TOL
+    out = rgc.run(gx, gy, mag, magnification=2, radius=1.0, sensitivity=3.0, doIntensityWeighting=False,
+                  pad_edges=True)
+    assert _report("RGC class pad_edges vs reference", out, golden[f"{name}/rgc_pad_edges"]) <= TOL
+
+
+@pytest.mark.parametrize("shape,M,R,S", [
+    ((2, 3, 5), 4, 2.0, 2.0),
+    ((2, 9, 65), 2, 3.4, 1.0),
+    ((2, 17, 3), 5, 0.4, 1.0),
+    ((2, 1, 1), 3, 1.5, 1.0),
+    ((2, 11, 7), 7, 1.7, 2.5),
+    ((3, 70, 130), 3, 1.5, 4.0),
+])
+def test_edge_shapes(torch, oracle_built, shape, M, R, S):
+    from nanopyx_amd.core.transform import eSRRF_ST
+    print(f"edge case {shape} M{M} R{R} S{S}", flush=True)
+    rng = np.random.default_rng(2)
+    x = (rng.random(shape) * 100 + 1).astype(np.float32)
+    out = eSRRF_ST(verbose=False).run(x, magnification=M, radius=R, sensitivity=S)
+    assert _report(f"edge {shape} vs reference restatement", out, oracle_built.esrrf(x, M, R, S)) <= TOL
+
+
+def test_cpu_small_workload_full(torch, oracle_built):
+    """100 frames x 64x64, M5 R1.5 S1, AVG: the reference CPU workload, end to end through eSRRF()."""
+    import nanopyx_amd
+    from nanopyx_amd import synthetic
+    from oracle import fp64
+    sc = synthetic.CONFIGS["cpu_small"]
+    x = synthetic.generate(sc)
+    e = sc.esrrf
+    for mpc in (False, True):
+        out = nanopyx_amd.eSRRF(x, magnification=e["magnification"], radius=e["radius"],
+                                sensitivity=e["sensitivity"], temporal_correlation="AVG", macro_pixel_correction=mpc)
+        ref = oracle_built.esrrf_workflow(x, e["magnification"], e["radius"], e["sensitivity"],
+                                          temporal_correlation="AVG", macro_pixel_correction=mpc)
+        assert out.shape == (320, 320)
+        assert _report(f"cpu_small eSRRF() mpc={mpc} vs reference restatement", out, ref) <= TOL
+    maps = nanopyx_amd.eSRRF_ST(verbose=False).run(x[:20], 5, 1.5, 1)
+    assert _report("cpu_small frames vs fp64", maps, fp64.esrrf_frames(x[:20], 5, 1.5, 1)) <= TOL
+
+
+def test_plain_256_subset(torch, oracle_built):
+    """500 x 256x256, M5 R1.5 S2, AVG/VAR/TAC2 — on its first 24 frames against the reference restatement."""
+    from nanopyx_amd.core.transform import eSRRF_ST
+    from nanopyx_amd import synthetic
+    sc = synthetic.CONFIGS["plain_256"]
+    x = synthetic.generate(sc, frames=24)
+    outs = eSRRF_ST(verbose=False).run_temporal(x, 5, 1.5, 2, temporal_correlation=("AVG", "VAR", "TAC2"))
+    maps = oracle_built.esrrf(x, 5, 1.5, 2)
+    for mode in ("AVG", "VAR", "TAC2"):
+        ref = oracle_built.temporal(maps, mode)
+        assert _report(f"plain_256[:24] {mode} vs reference restatement", outs[mode].cpu().numpy(), ref) <= TOL
+
+
+def test_large_u16_properties(torch):
+    """1000 x 512x512 uint16, M4 R2 S2, AVG + TAC2 at full size, through size-independent properties:
+    exact power-of-two scale equivariance, chunked vs unchunked frame streaming, fused AVG vs the mean of the
+    per-frame maps, transpose equivariance."""
+    from nanopyx_amd import _hip, synthetic
+    sc = synthetic.CONFIGS["large_512_u16"]
+    x = torch.from_numpy(synthetic.generate(sc)).cuda()
+    print("large_512_u16 generated", tuple(x.shape), flush=True)
+    M, R, S = 4, 2.0, 2.0
+    a = _hip.esrrf_temporal(x, M, R, S, True, ("AVG", "TAC2"))
+    torch.cuda.synchronize()
+    xf = x.to(torch.float32)
+    b = _hip.esrrf_temporal(2.0 * xf, M, R, S, True, ("AVG", "TAC2"))
+    assert torch.equal(b["AVG"], 2.0 * a["AVG"]), "scale equivariance (AVG) must be exact"
+    assert torch.equal(b["TAC2"], 4.0 * a["TAC2"]), "scale equivariance (TAC2) must be exact"
+    c = _hip.esrrf_temporal(x, M, R, S, True, ("AVG", "TAC2"), workspace=False)
+    for m in ("AVG", "TAC2"):
+        assert _report(f"large {m} chunked vs single pass", a[m].cpu().numpy(), c[m].cpu().numpy()) <= 1e-5
+    frames = _hip.esrrf_frames(x[:64], M, R, S, True)
+    avg64 = _hip.esrrf_temporal(x[:64], M, R, S, True, ("AVG",))["AVG"]
+    assert _report("large AVG(64) vs mean of frames", avg64.cpu().numpy(),
+                   frames.double().mean(0).cpu().numpy()) <= 1e-5
+    t = _hip.esrrf_temporal(x[:64].transpose(1, 2).contiguous(), M, R, S, True, ("AVG",))["AVG"]
+    assert _report("large AVG(64) transpose equivariance", t.T.cpu().numpy(), avg64.cpu().numpy()) <= 1e-5
+
+
+def test_large_u16_vs_reference_subset(torch, oracle_built):
+    from nanopyx_amd import _hip, synthetic
+    sc = synthetic.CONFIGS["large_512_u16"]
+    x = synthetic.generate(sc, frames=6)
+    outs = _hip.esrrf_temporal(x, 4, 2.0, 2.0, True, ("AVG", "TAC2"))
+    maps = oracle_built.esrrf(x, 4, 2.0, 2.0)
+    for m in ("AVG", "TAC2"):
+        assert _report(f"large[:6] {m} vs reference restatement", outs[m].cpu().numpy(),
+                       oracle_built.temporal(maps, m)) <= TOL
+
+
+def test_uint16_equals_float32(torch):
+    from nanopyx_amd import _hip
+    rng = np.random.default_rng(4)
+    x = rng.integers(90, 3000, size=(5, 40, 70)).astype(np.uint16)
+    a = _hip.esrrf_frames(x, 3, 1.5, 2.0, True)
+    b = _hip.esrrf_frames(x.astype(np.float32), 3, 1.5, 2.0, True)
+    assert torch.equal(a, b)
+
+
+def test_temporal_and_macro_pixel_kernels(torch, oracle_built):
+    from nanopyx_amd import calculate_eSRRF_temporal_correlations, macro_pixel_corrector
+    from oracle import fp64
+    rng = np.random.default_rng(9)
+    st = (rng.random((50, 30, 40)) * 20 + 5).astype(np.float32)
+    for m in ("AVG", "VAR", "TAC2"):
+        assert _report(f"temporal {m} vs fp64", calculate_eSRRF_temporal_correlations(st, m),
+                       fp64.temporal(st, m)) <= TOL
+    with pytest.raises(ValueError):
+        calculate_eSRRF_temporal_correlations(st, "MEAN")
+    img = (rng.random((2, 30, 40)) * 10).astype(np.float32)
+    assert _report("macro pixel vs reference restatement", macro_pixel_corrector(img, 5),
+                   oracle_built.macro_pixel(img, 5)) <= TOL
+
+
+def test_workflow_timepoints(torch, oracle_built):
+    import nanopyx_amd
+    rng = np.random.default_rng(6)
+    x = (rng.random((20, 24, 28)) * 100 + 50).astype(np.float32)
+    for mode in ("AVG", "VAR", "TAC2"):
+        out = nanopyx_amd.eSRRF(x, magnification=2, radius=1.5, sensitivity=1, frames_per_timepoint=7,
+                                temporal_correlation=mode)
+        ref = oracle_built.esrrf_workflow(x, 2, 1.5, 1, frames_per_timepoint=7, temporal_correlation=mode)
+        assert out.shape == (3, 48, 56)
+        assert _report(f"eSRRF() T=3 {mode}", out, ref) <= TOL
+
+
+def test_errors(torch):
+    import nanopyx_amd
+    from nanopyx_amd.core.transform import eSRRF_ST
+    x = np.ones((2, 8, 8), np.float32)
+    with pytest.raises(ValueError):
+        eSRRF_ST(verbose=False).run(x, use_fht=True)
+    with pytest.raises(ValueError):
+        nanopyx_amd.eSRRF(x, temporal_correlation="XYZ")
+    with pytest.raises(ValueError):
+        eSRRF_ST(verbose=False).run(np.ones((1, 2, 8, 8), np.float32))
+    with pytest.raises(ValueError):
+        eSRRF_ST(verbose=False).run(x, run_type="Threaded")

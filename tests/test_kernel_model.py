@@ -1,0 +1,31 @@
+"""CPU checks of the fused kernel's launch geometry and arithmetic before it runs on a GPU: the block-by-block
+model in kernel_model.py repeats the kernel's index math on the library's own tables (asserting every index
+stays in its buffer) and must reproduce the oracle."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, rel_err
+from kernel_model import model_frames
+
+
+@pytest.mark.parametrize("name", ["m5_r15_s1", "m2_r10_s3_noiw", "m3_r25_s17", "m8_r12_s2", "m1_r15_s1"])
+def test_model_matches_golden(golden, name):
+    M, R, S, iw = golden[f"{name}/params"]
+    x = golden[f"{name}/input"][:1]
+    out = model_frames(x, int(M), float(R), float(S), bool(iw))
+    assert rel_err(out, golden[f"{name}/rgc_maps"][:1])[0] <= 2e-6
+
+
+@pytest.mark.parametrize("shape,M,R,S", [
+    ((1, 3, 5), 4, 2.0, 2.0),        # image smaller than the ring window
+    ((1, 9, 65), 2, 3.4, 1.0),       # widest window (K = 6), two column regions
+    ((1, 17, 3), 5, 0.4, 1.0),       # K = 0
+    ((1, 1, 1), 3, 1.5, 1.0),        # one pixel
+    ((1, 11, 7), 7, 1.7, 2.5),       # two phase groups (49 phases), non-integer sensitivity
+])
+def test_model_edge_shapes(oracle_built, shape, M, R, S):
+    rng = np.random.default_rng(1)
+    x = (rng.random(shape) * 100 + 1).astype(np.float32)
+    out = model_frames(x, M, R, S)
+    ref = oracle_built.esrrf(x, M, R, S)
+    assert rel_err(out, ref)[0] <= 1e-5
