@@ -88,12 +88,13 @@ int npx_macro_pixel(float *d_img, int S, int HM, int WM, int M, float *d_cmap, v
 
 /*
  * Host-only introspection of the geometry tables the kernels use (no GPU needed). info[16] receives
- * K, margin, HM, WM, nphase, Q, HG, HI, TIR, TIC, TGR, TGC, LR, LP, nwaves, ngroups; finfo[4] receives
+ * K, margin, HM, WM, nphase, Q (rows per lane) of the per-frame kernel, Q of the temporal kernel, floats per
+ * phase ring block, HG, HI (tile halos), LP, waves per block, phase groups, prefetch slots, 0, 0; finfo[4] receives
  * fwhm, sigma, tSS, tSO. No reference counterpart: it exists so the host can check launch geometry and tables.
  */
 int npx_plan_info(int H, int W, int M, double R, int *info, float *finfo);
 
-/* Copies the plan tables to host arrays sized from npx_plan_info (ring nphase*(2K+1)^2*4 floats,
+/* Copies the plan tables to host arrays sized from npx_plan_info (ring nphase*ring_block floats,
  * wsum nphase, lattice taps 2H+1 / 2W+1 entries, intensity taps HM / WM entries; *_w arrays hold 4 floats
  * per entry). No reference counterpart (introspection, as npx_plan_info). */
 int npx_plan_tables(int H, int W, int M, double R, float *ring, float *wsum, int *lrb, float *lrw, int *lcb,

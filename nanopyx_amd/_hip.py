@@ -202,8 +202,8 @@ def plan_info(H, W, M, R):
     info = (ctypes.c_int * 16)()
     finfo = (ctypes.c_float * 4)()
     _check(lib().npx_plan_info(H, W, M, float(R), info, finfo))
-    keys = ("K", "margin", "HM", "WM", "nphase", "Q", "HG", "HI", "TIR", "TIC", "TGR", "TGC", "LR", "LP", "nwaves",
-            "ngroups")
+    keys = ("K", "margin", "HM", "WM", "nphase", "Q_frames", "Q_temporal", "ring_stride", "HG", "HI", "LP", "nwaves",
+            "ngroups", "prefetch", "_r0", "_r1")
     d = dict(zip(keys, list(info)))
     d.update(fwhm=finfo[0], sigma=finfo[1], tSS=finfo[2], tSO=finfo[3])
     return d
@@ -213,7 +213,7 @@ def plan_tables(H, W, M, R):
     """Host copies of the kernel tables (ring, wsum, lattice and intensity taps)."""
     d = plan_info(H, W, M, R)
     K, P, NW = d["K"], d["nphase"], 2 * d["K"] + 1
-    ring = np.zeros((P, NW, NW, 4), np.float32)
+    ring = np.zeros((P, d["ring_stride"]), np.float32)
     wsum = np.zeros(P, np.float32)
     arr = {}
     for name, n in (("lr", 2 * H + 1), ("lc", 2 * W + 1), ("cr", d["HM"]), ("cc", d["WM"])):
@@ -223,7 +223,12 @@ def plan_tables(H, W, M, R):
     i = lambda a: a.ctypes.data_as(_ip)
     _check(lib().npx_plan_tables(H, W, M, float(R), f(ring), f(wsum), i(arr["lrb"]), f(arr["lrw"]), i(arr["lcb"]),
                                  f(arr["lcw"]), i(arr["crb"]), f(arr["crw"]), i(arr["ccb"]), f(arr["ccw"])))
-    return dict(info=d, ring=ring, wsum=wsum, **arr)
+    # per-phase ring block: dy[j], dx[i], (d, t)[i][j], w[i][j]
+    views = dict(dy=ring[:, :NW], dx=ring[:, NW:2 * NW],
+                 d=ring[:, 2 * NW:2 * NW + 2 * NW * NW].reshape(P, NW, NW, 2)[..., 0],
+                 t=ring[:, 2 * NW:2 * NW + 2 * NW * NW].reshape(P, NW, NW, 2)[..., 1],
+                 w=ring[:, 2 * NW + 2 * NW * NW:2 * NW + 3 * NW * NW].reshape(P, NW, NW))
+    return dict(info=d, ring=ring, wsum=wsum, **views, **arr)
 
 
 def timing(on: bool) -> None:

@@ -124,7 +124,7 @@ int device_tables(Plan* p, const npx::DevTables** out) {
     auto* base = static_cast<uint8_t*>(mem);
     DevCopy dc;
     dc.mem = mem;
-    dc.t.ring = reinterpret_cast<const float4*>(base + segs[0].off);
+    dc.t.ring = reinterpret_cast<const float*>(base + segs[0].off);
     dc.t.wsum = reinterpret_cast<const float*>(base + segs[1].off);
     dc.t.lrw = reinterpret_cast<const float4*>(base + segs[2].off);
     dc.t.lrb = reinterpret_cast<const int*>(base + segs[3].off);
@@ -175,7 +175,7 @@ Shape fused_shape(int H, int W, int M) {
     s.ngroups = (nphase + 15) / 16;
     s.nwaves = (nphase + s.ngroups - 1) / s.ngroups;
     s.nregx = (W + 63) / 64;
-    s.nregy = (H + npx::kQ - 1) / npx::kQ;
+    s.nregy = (H + npx::kQTemporal - 1) / npx::kQTemporal;
     return s;
 }
 
@@ -264,7 +264,8 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
         rc = device_tables(p, &t);
         if (rc) return rc;
     }
-    const Shape s = fused_shape(H, W, M);
+    Shape s = fused_shape(H, W, M);
+    s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
     // one frame per block unless the grid's z extent would overflow
     const long long zmax = 65535 / s.ngroups;
@@ -383,10 +384,10 @@ int npx_plan_info(int H, int W, int M, double R, int* info, float* finfo) {
     if (!p) return NPX_ERR_UNSUPPORTED;
     const npx::PlanHost& h = p->host;
     const Shape s = fused_shape(H, W, M);
-    const int K = h.K, Q = npx::kQ;
+    const int K = h.K;
     const int HG = K / 2 + 3, HI = HG + 1;
-    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, Q, HG, HI, Q + 2 * HI, 64 + 2 * HI, Q + 2 * HG,
-                       64 + 2 * HG, 2 * Q + 2 * K, 64 + K, s.nwaves, s.ngroups};
+    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::kQFrames, npx::kQTemporal, h.ring_stride, HG, HI,
+                       64 + K, s.nwaves, s.ngroups, npx::kPrefetch, 0, 0};
     for (int i = 0; i < 16; ++i) info[i] = v[i];
     finfo[0] = h.fwhm;
     finfo[1] = h.sigma;

@@ -57,18 +57,28 @@ __device__ __forceinline__ void moments_push(float v, int n, float inv_np1, floa
 
 // ------------------------------------------------------------------------------------------ fused kernel
 
+__device__ __forceinline__ float load_px(const FusedArgs& a, int f, int gy, int gx) {
+    if (gy < 0 || gy >= a.H || gx < 0 || gx >= a.W) return 0.f;
+    const size_t o = (static_cast<size_t>(f) * a.H + gy) * a.W + gx;
+    return a.u16 ? static_cast<float>(static_cast<const uint16_t*>(a.in)[o]) : static_cast<const float*>(a.in)[o];
+}
+
 template <int K, int Q, int MODE>
 __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
-    constexpr int NW = 2 * K + 1;
-    constexpr int HG = Geometry<K, Q>::HG, HI = Geometry<K, Q>::HI;
-    constexpr int TIR = Geometry<K, Q>::TIR, TIC = Geometry<K, Q>::TIC;
-    constexpr int TGR = Geometry<K, Q>::TGR, TGC = Geometry<K, Q>::TGC;
-    constexpr int LR = Geometry<K, Q>::LR, LP = Geometry<K, Q>::LP, LPL = LR * LP;
+    using G = Geometry<K, Q>;
+    constexpr int NW = G::NW, RS = G::RS;
+    constexpr int HG = G::HG, HI = G::HI;
+    constexpr int TIR = G::TIR, TIC = G::TIC;
+    constexpr int TGR = G::TGR, TGC = G::TGC;
+    constexpr int LR = G::LR, LP = G::LP, LPL = LR * LP;
     constexpr int NR = 2 * Q + 2 * K - 1;
+    constexpr int RDX = NW, RDT = 2 * NW, RW = 2 * NW + 2 * NW * NW;  // ring block offsets
 
     __shared__ float s_in[TIR * TIC];
     __shared__ float s_gc[TGR * TGC];
     __shared__ float s_gr[TGR * TGC];
+    __shared__ float s_rx[LR * TGC];   // gradients filtered along rows (one row per lattice row)
+    __shared__ float s_ry[LR * TGC];
     __shared__ float2 s_lat[2 * LPL];
 
     const int tid = threadIdx.x;
@@ -92,26 +102,47 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     const int cM = M * xi + px;
     const int a_lo = 2 * y0 - K;
     const int b_lo = 2 * x0 - K;
-    const float4* __restrict__ ring_ph = a.t.ring + static_cast<size_t>(active ? ph : 0) * NW * NW;
+    const float* __restrict__ rb_ph = a.t.ring + static_cast<size_t>(active ? ph : 0) * RS;
 
     float mu[Q], m2[Q], cc[Q], fst[Q], prv[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) mu[q] = m2[q] = cc[q] = fst[q] = prv[q] = 0.f;
 
-    for (int f = f_begin; f < f_end; ++f) {
-        // (1) input tile + halo, zero outside the image (u16 camera counts converted as a plain cast)
-        for (int idx = tid; idx < TIR * TIC; idx += nthr) {
+    // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
+    float pf[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+        const int idx = tid + k * nthr;
+        pf[k] = 0.f;
+        if (idx < TIR * TIC && f_begin < f_end) {
             const int r = idx / TIC, c = idx - r * TIC;
-            const int gy = y0 - HI + r, gx = x0 - HI + c;
-            float v = 0.f;
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-                const size_t o = (static_cast<size_t>(f) * a.H + gy) * a.W + gx;
-                v = a.u16 ? static_cast<float>(static_cast<const uint16_t*>(a.in)[o])
-                          : static_cast<const float*>(a.in)[o];
-            }
-            s_in[idx] = v;
+            pf[k] = load_px(a, f_begin, y0 - HI + r, x0 - HI + c);
+        }
+    }
+
+    for (int f = f_begin; f < f_end; ++f) {
+        // (1) input tile + halo of frame f (zero outside the image; uint16 converted as a plain cast)
+#pragma unroll
+        for (int k = 0; k < kPrefetch; ++k) {
+            const int idx = tid + k * nthr;
+            if (idx < TIR * TIC) s_in[idx] = pf[k];
+        }
+        for (int idx = tid + kPrefetch * nthr; idx < TIR * TIC; idx += nthr) {
+            const int r = idx / TIC, c = idx - r * TIC;
+            s_in[idx] = load_px(a, f, y0 - HI + r, x0 - HI + c);
         }
         __syncthreads();
+        // prefetch frame f + 1 while this frame is processed (consumed at the top of the next iteration)
+        if (f + 1 < f_end) {
+#pragma unroll
+            for (int k = 0; k < kPrefetch; ++k) {
+                const int idx = tid + k * nthr;
+                if (idx < TIR * TIC) {
+                    const int r = idx / TIC, c = idx - r * TIC;
+                    pf[k] = load_px(a, f + 1, y0 - HI + r, x0 - HI + c);
+                }
+            }
+        }
         // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
         for (int idx = tid; idx < TGR * TGC; idx += nthr) {
             const int r = idx / TGC, c = idx - r * TGC;
@@ -132,37 +163,48 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
             s_gr[idx] = grv;
         }
         __syncthreads();
-        // (3) normalised gradient lattice: point (a, b) holds G/|G| of the x2M Catmull-Rom gradients at
-        // (a/2, b/2); rows a >= 2H read the clamped last row; points the ring test excludes hold 0.
+        // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
+        // (rows a >= 2H read the clamped last row, :45-52); lattice rows the ring test excludes stay 0
+        for (int idx = tid; idx < LR * TGC; idx += nthr) {
+            const int la = idx / TGC, c = idx - la * TGC;
+            const int aa = a_lo + la;
+            float sx = 0.f, sy = 0.f;
+            if (aa >= 1 && aa <= 2 * a.HM - 2) {
+                const int ai = min(aa, 2 * a.H);
+                const float4 wr = a.t.lrw[ai];
+                const int rb = a.t.lrb[ai] - (y0 - HG);
+                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int tr = min(max(rb + u, 0), TGR - 1);
+                    sx += s_gc[tr * TGC + c] * wrv[u];
+                    sy += s_gr[tr * TGC + c] * wrv[u];
+                }
+            }
+            s_rx[idx] = sx;
+            s_ry[idx] = sy;
+        }
+        __syncthreads();
+        // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
+        // column-parity planes so the 64 lanes of one read touch 64 consecutive float2
         for (int idx = tid; idx < 2 * LPL; idx += nthr) {
             const int plane = idx / LPL;
             const int rem = idx - plane * LPL;
             const int la = rem / LP;
             const int pos = rem - la * LP;
-            const int aa = a_lo + la;
             const int bb = b_lo + 2 * pos + plane;
             float2 g = make_float2(0.f, 0.f);
-            if (aa >= 1 && aa <= 2 * a.HM - 2 && bb >= 1 && bb <= 2 * a.WM - 2) {
-                const int ai = min(aa, 2 * a.H), bi = min(bb, 2 * a.W);
-                const float4 wr = a.t.lrw[ai];
+            if (bb >= 1 && bb <= 2 * a.WM - 2) {
+                const int bi = min(bb, 2 * a.W);
                 const float4 wc = a.t.lcw[bi];
-                const int rb = a.t.lrb[ai] - (y0 - HG);
                 const int cb = a.t.lcb[bi] - (x0 - HG);
-                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
                 const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
                 float gx = 0.f, gy = 0.f;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int tc = min(max(cb + v, 0), TGC - 1);
-                    float sx = 0.f, sy = 0.f;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int tr = min(max(rb + u, 0), TGR - 1);
-                        sx += s_gc[tr * TGC + tc] * wrv[u];
-                        sy += s_gr[tr * TGC + tc] * wrv[u];
-                    }
-                    gx += sx * wcv[v];
-                    gy += sy * wcv[v];
+                    gx += s_rx[la * TGC + tc] * wcv[v];
+                    gy += s_ry[la * TGC + tc] * wcv[v];
                 }
                 const float n2 = gx * gx + gy * gy;
                 if (n2 > 0.f) {
@@ -174,30 +216,43 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
         }
         __syncthreads();
 
-        // (4) ring sums: sample (j, i) of pixel q reads lattice row 2q + ey + j + K, column 2*lane + ex + i + K.
-        // Per sample: dot = g.(A,B) = w cos, crs = g x (A,B) = w sin, contribution = med3(0, -dot, w - |crs|),
-        // which is w * (1 - |sin|) when the gradient points against the offset (reference GdotR < 0) else 0.
+        // (4) ring sums. Sample (j, i) of pixel q reads lattice row 2q + ey + j + K, column 2*lane + ex + i + K.
+        // With u = g.(dx, dy) and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's
+        // contribution [G.d < 0] * w * (1 - |G x d| / (|G| d)) is t * med3(0, -u, d - |v|). g.x*dx and g.y*dx are
+        // formed once per lattice point of the column and shared by the samples that read it.
         if (active) {
             float acc[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) acc[q] = 0.f;
-#pragma unroll 1
-            for (int i = -K; i <= K; ++i) {
-                const int c = ex + i + K;
-                const float2* col = s_lat + (c & 1) * LPL + ey * LP + lane + (c >> 1);
-                float2 g[NR];
+            float dyv[NW];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) g[r] = col[r * LP];
-                const float4* __restrict__ rt = ring_ph + (i + K) * NW;
+            for (int jj = 0; jj < NW; ++jj) dyv[jj] = rb_ph[jj];
+#pragma unroll 1
+            for (int i = 0; i < NW; ++i) {
+                const int c = ex + i;
+                const float2* col = s_lat + (c & 1) * LPL + ey * LP + lane + (c >> 1);
+                const float dxi = rb_ph[RDX + i];
+                const float* __restrict__ dt = rb_ph + RDT + 2 * i * NW;
+                float dv[NW], tv[NW];
 #pragma unroll
                 for (int jj = 0; jj < NW; ++jj) {
-                    const float4 t = rt[jj];
+                    dv[jj] = dt[2 * jj];
+                    tv[jj] = dt[2 * jj + 1];
+                }
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) {
-                        const float2 gg = g[2 * q + jj];
-                        const float dot = gg.x * t.x + gg.y * t.y;
-                        const float crs = gg.y * t.x - gg.x * t.y;
-                        acc[q] += __builtin_amdgcn_fmed3f(0.f, -dot, t.z - fabsf(crs));
+                for (int r = 0; r < NR; ++r) {
+                    const float2 g = col[r * LP];
+                    const float px_ = g.x * dxi;
+                    const float py_ = g.y * dxi;
+#pragma unroll
+                    for (int jj = 0; jj < NW; ++jj) {
+                        const int rq = r - jj;
+                        if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
+                        const int q = rq >> 1;
+                        const float u = fmaf(g.y, dyv[jj], px_);
+                        const float v = fmaf(-g.x, dyv[jj], py_);
+                        const float m = __builtin_amdgcn_fmed3f(0.f, -u, dv[jj] - fabsf(v));
+                        acc[q] = fmaf(tv[jj], m, acc[q]);
                     }
                 }
             }
@@ -214,14 +269,14 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                 const bool inside = xi < a.W && yi < a.H;
                 const bool valid = inside && rM >= a.margin && rM < a.HM - a.margin && cM >= a.margin &&
                                    cM < a.WM - a.margin;
-                float v = 0.f;
+                float val = 0.f;
                 if (valid) {
                     const int a0 = 2 * yi + ey, b0 = 2 * xi + ex;
                     float den;
                     if (a0 - K >= 1 && a0 + K <= 2 * a.HM - 2 && b0 - K >= 1 && b0 + K <= 2 * a.WM - 2) {
                         den = a.t.wsum[ph];
                     } else {
-                        // edge pixel: sum the weights of in-bounds samples in the reference order
+                        // edge pixel: sum the weights of in-bounds samples in the reference order (j outer)
                         den = 0.f;
                         for (int jj = 0; jj < NW; ++jj) {
                             const int aa = a0 + jj - K;
@@ -229,7 +284,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                             for (int ii = 0; ii < NW; ++ii) {
                                 const int bb = b0 + ii - K;
                                 if (bb < 1 || bb > 2 * a.WM - 2) continue;
-                                den += ring_ph[ii * NW + jj].z;
+                                den += rb_ph[RW + ii * NW + jj];
                             }
                         }
                     }
@@ -240,7 +295,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                         const float4 wrow = a.t.crw[rMs];
                         const int crow = a.t.crb[rMs] - (y0 - HI);
                         const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
-                        float s = 0.f;
+                        float sacc = 0.f;
 #pragma unroll
                         for (int v2 = 0; v2 < 4; ++v2) {
                             const int tc = min(max(ccol + v2, 0), TIC - 1);
@@ -250,16 +305,16 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                                 const int tr = min(max(crow + u, 0), TIR - 1);
                                 p += s_in[tr * TIC + tc] * wrv[u];
                             }
-                            s += p * wcv[v2];
+                            sacc += p * wcv[v2];
                         }
-                        rgc = rgc * s;
+                        rgc = rgc * sacc;
                     }
-                    v = rgc;
+                    val = rgc;
                 }
                 if (MODE == kModeFrames) {
-                    if (inside) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = v;
+                    if (inside) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
                 } else {
-                    moments_push(v, f - f_begin, inv_np1, mu[q], m2[q], cc[q], fst[q], prv[q]);
+                    moments_push(val, f - f_begin, inv_np1, mu[q], m2[q], cc[q], fst[q], prv[q]);
                 }
             }
         }
@@ -449,24 +504,24 @@ __global__ __launch_bounds__(256) void macro_pixel_divide(float* __restrict__ im
 
 // ------------------------------------------------------------------------------------------ launchers
 
-template <int K, int Q>
-static hipError_t launch_fused_kq(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
+template <int K>
+static hipError_t launch_fused_k(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
     if (mode == kModeFrames)
-        hipLaunchKernelGGL((esrrf_fused<K, Q, kModeFrames>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((esrrf_fused<K, Q, kModeTemporal>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporal>), grid, block, 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_fused(const FusedArgs& a, int K, dim3 grid, dim3 block, int mode, hipStream_t st) {
     switch (K) {
-        case 0: return launch_fused_kq<0, kQ>(a, grid, block, mode, st);
-        case 1: return launch_fused_kq<1, kQ>(a, grid, block, mode, st);
-        case 2: return launch_fused_kq<2, kQ>(a, grid, block, mode, st);
-        case 3: return launch_fused_kq<3, kQ>(a, grid, block, mode, st);
-        case 4: return launch_fused_kq<4, kQ>(a, grid, block, mode, st);
-        case 5: return launch_fused_kq<5, kQ>(a, grid, block, mode, st);
-        case 6: return launch_fused_kq<6, kQ>(a, grid, block, mode, st);
+        case 0: return launch_fused_k<0>(a, grid, block, mode, st);
+        case 1: return launch_fused_k<1>(a, grid, block, mode, st);
+        case 2: return launch_fused_k<2>(a, grid, block, mode, st);
+        case 3: return launch_fused_k<3>(a, grid, block, mode, st);
+        case 4: return launch_fused_k<4>(a, grid, block, mode, st);
+        case 5: return launch_fused_k<5>(a, grid, block, mode, st);
+        case 6: return launch_fused_k<6>(a, grid, block, mode, st);
         default: return hipErrorInvalidValue;
     }
 }

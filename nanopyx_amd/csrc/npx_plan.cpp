@@ -90,28 +90,31 @@ bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
     }
     // Ring geometry per phase, in the exact phase-relative coordinates:
     // dy = (ey + j)/2 - (py + 0.5)/M, dx = (ex + i)/2 - (px + 0.5)/M.
-    p.ring.assign(static_cast<size_t>(p.nphase) * NW * NW * 4, 0.0f);
+    const RingLayout L(K);
+    p.ring_stride = L.stride;
+    p.ring.assign(static_cast<size_t>(p.nphase) * L.stride, 0.0f);
     p.wsum.assign(p.nphase, 0.0f);
     const double tSS = p.tSS;
     for (int ph = 0; ph < p.nphase; ++ph) {
         const int py = ph / M, px = ph % M;
+        float* blk = p.ring.data() + static_cast<size_t>(ph) * L.stride;
+        for (int j = -K; j <= K; ++j) blk[L.dy + j + K] = static_cast<float>((p.ey[py] + j) / 2.0 - (py + 0.5) / M);
+        for (int i = -K; i <= K; ++i) blk[L.dx + i + K] = static_cast<float>((p.ex[px] + i) / 2.0 - (px + 0.5) / M);
         std::vector<float> wgrid(NW * NW, 0.0f);  // [j][i]
         for (int j = -K; j <= K; ++j) {
             for (int i = -K; i <= K; ++i) {
-                const double dye = (p.ey[py] + j) / 2.0 - (py + 0.5) / M;
-                const double dxe = (p.ex[px] + i) / 2.0 - (px + 0.5) / M;
-                const float dy = static_cast<float>(dye);
-                const float dx = static_cast<float>(dxe);
+                const float dy = blk[L.dy + j + K];
+                const float dx = blk[L.dx + i + K];
                 const float dd = static_cast<float>(dx * dx) + static_cast<float>(dy * dy);
                 const float d = static_cast<float>(std::sqrt(static_cast<double>(dd)));
+                const int e = (i + K) * NW + (j + K);
+                blk[L.dt + 2 * e] = d;
                 if (!(d != 0.0f && d <= p.tSO)) continue;  // :44
                 // _c_calculate_dw (:4-6) in double, stored as float (:63)
                 const double dd64 = d;
                 const float w = static_cast<float>(std::pow(dd64 * std::exp((-dd64 * dd64) / tSS), 4));
-                const size_t e = ((static_cast<size_t>(ph) * NW + (i + K)) * NW + (j + K)) * 4;
-                p.ring[e + 0] = static_cast<float>(static_cast<double>(dx) * w / d);
-                p.ring[e + 1] = static_cast<float>(static_cast<double>(dy) * w / d);
-                p.ring[e + 2] = w;
+                blk[L.dt + 2 * e + 1] = static_cast<float>(static_cast<double>(w) / dd64);
+                blk[L.w + e] = w;
                 wgrid[(j + K) * NW + (i + K)] = w;
             }
         }

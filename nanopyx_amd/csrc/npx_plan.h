@@ -24,14 +24,27 @@ struct PlanHost {
     int margin = 0;       // int(fwhm * M)
     int HM = 0, WM = 0;
     int nphase = 0;       // M * M output phases
-    // ring tables per phase, [phase][i + K][j + K] -> (A, B, w, 0) with A = dx*w/d, B = dy*w/d
+    // ring tables, one block of ring_stride floats per phase (layout in ring_layout below)
     std::vector<float> ring;
+    int ring_stride = 0;
     std::vector<float> wsum;        // [phase] float32 sum of w in the reference's j-outer/i-inner order
     std::vector<int> ey, ex;        // [phase row], [phase col]: floor((2p+1)/M)
     // Catmull-Rom taps for the x2M gradient lattice (index a in [0, 2H], 2H = the clamped last row)
     std::vector<CrTap> lat_row, lat_col;
     // Catmull-Rom taps for the xM intensity image (index rM in [0, HM), cM in [0, WM))
     std::vector<CrTap> cr_row, cr_col;
+};
+
+// Per-phase ring block (float offsets from the block start), NW = 2K+1, window offsets i (columns), j (rows):
+//   dy[j]          at 0            offset of lattice row j from the pixel centre, float32
+//   dx[i]          at NW           offset of lattice column i
+//   (d, t)[i][j]   at 2*NW         distance and t = w / d (t = 0 outside the ring)
+//   w[i][j]        at 2*NW+2*NW^2  distance weight (0 outside the ring)
+struct RingLayout {
+    int NW, dy, dx, dt, w, stride;
+    explicit RingLayout(int K)
+        : NW(2 * K + 1), dy(0), dx(NW), dt(2 * NW), w(2 * NW + 2 * NW * NW),
+          stride(((2 * NW + 3 * NW * NW) + 15) / 16 * 16) {}
 };
 
 // Builds the tables; returns false with a message on unsupported input.

@@ -1,7 +1,8 @@
 """CPU model of the fused gfx950 kernel (nanopyx_amd/csrc/npx_kernels.hip, esrrf_fused), block by block.
 
-It repeats the kernel's index arithmetic statement for statement — tiles, halos, the two column-parity lattice
-planes, the per-lane column sweep and the edge normaliser — on the library's own plan tables, in float32, and
+It repeats the kernel's index arithmetic statement for statement — tiles, halos, the row then column Catmull-Rom
+passes of the lattice, the two column-parity lattice planes, the per-lane column sweep with its per-lattice-point
+products, and the edge normaliser — on the library's own plan tables, in float32, and
 asserts that every LDS / table / global index the kernel forms stays inside its buffer and that every tap with a
 non-zero weight reads inside its tile (clamped taps must carry weight 0). Tests run it on small stacks against the
 oracle, so an indexing mistake shows up on the CPU before the kernel ever runs on a GPU.
@@ -15,19 +16,23 @@ from nanopyx_amd import _hip
 f32 = np.float32
 
 
-def model_frames(stack, M, R, S, iw=True):
+def model_frames(stack, M, R, S, iw=True, Q=None):
+    """Per-frame maps as the kernel computes them with Q input rows per lane (default: the per-frame kernel's)."""
     stack = np.asarray(stack)
     N, H, W = stack.shape
     t = _hip.plan_tables(H, W, M, R)
     d = t["info"]
-    K, Q, HG, HI = d["K"], d["Q"], d["HG"], d["HI"]
-    TIR, TIC, TGR, TGC, LR, LP = d["TIR"], d["TIC"], d["TGR"], d["TGC"], d["LR"], d["LP"]
+    K, HG, HI, LP = d["K"], d["HG"], d["HI"], d["LP"]
+    Q = d["Q_frames"] if Q is None else Q
+    TIR, TIC = Q + 2 * HI, 64 + 2 * HI
+    TGR, TGC = Q + 2 * HG, 64 + 2 * HG
+    LR = 2 * Q + 2 * K
     HM, WM, margin, nphase = d["HM"], d["WM"], d["margin"], d["nphase"]
     nwaves, ngroups = d["nwaves"], d["ngroups"]
     NW = 2 * K + 1
     NR = 2 * Q + 2 * K - 1
     LPL = LR * LP
-    ring = t["ring"].reshape(nphase * NW * NW, 4)
+    dyt, dxt, dtab, ttab, wtab = t["dy"], t["dx"], t["d"], t["t"], t["w"]
     wsum = t["wsum"]
     Sf = f32(S)
     pow_mode = 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
@@ -67,35 +72,45 @@ def model_frames(stack, M, R, S, iw=True):
                     a_r1c1 = s_in[r1, c1[ok]]
                     s_gc[r, ok] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0
                     s_gr[r, ok] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0
-                # (3) lattice
-                s_lat = np.zeros((2 * LPL, 2), np.float32)
+                # (3a) rows: Catmull-Rom of the gradient tile along rows for every lattice row
                 a_lo, b_lo = 2 * y0 - K, 2 * x0 - K
+                s_rx = np.zeros((LR, TGC), np.float32)
+                s_ry = np.zeros((LR, TGC), np.float32)
+                for la in range(LR):
+                    aa = a_lo + la
+                    if not (1 <= aa <= 2 * HM - 2):
+                        continue
+                    ai = min(aa, 2 * H)
+                    wr = t["lrw"][ai]
+                    rb = t["lrb"][ai] - (y0 - HG)
+                    sx = np.zeros(TGC, np.float32)
+                    sy = np.zeros(TGC, np.float32)
+                    for u in range(4):
+                        tr = min(max(rb + u, 0), TGR - 1)
+                        if wr[u] != 0:
+                            assert 0 <= rb + u < TGR, ("lattice row tap outside tile", aa)
+                        sx = (sx + s_gc[tr] * wr[u]).astype(np.float32)
+                        sy = (sy + s_gr[tr] * wr[u]).astype(np.float32)
+                    s_rx[la], s_ry[la] = sx, sy
+                # (3b) columns + normalisation into the two column-parity planes
+                s_lat = np.zeros((2 * LPL, 2), np.float32)
                 for idx in range(2 * LPL):
                     plane, rem = divmod(idx, LPL)
                     la, pos = divmod(rem, LP)
-                    aa, bb = a_lo + la, b_lo + 2 * pos + plane
-                    if not (1 <= aa <= 2 * HM - 2 and 1 <= bb <= 2 * WM - 2):
+                    bb = b_lo + 2 * pos + plane
+                    if not (1 <= bb <= 2 * WM - 2):
                         continue
-                    ai, bi = min(aa, 2 * H), min(bb, 2 * W)
-                    wr, wc = t["lrw"][ai], t["lcw"][bi]
-                    rb = t["lrb"][ai] - (y0 - HG)
+                    bi = min(bb, 2 * W)
+                    wc = t["lcw"][bi]
                     cb = t["lcb"][bi] - (x0 - HG)
                     gxv = f32(0)
                     gyv = f32(0)
                     for v in range(4):
                         tc = min(max(cb + v, 0), TGC - 1)
                         if wc[v] != 0:
-                            assert 0 <= cb + v < TGC, ("lattice column tap outside tile", aa, bb)
-                        sx = f32(0)
-                        sy = f32(0)
-                        for u in range(4):
-                            tr = min(max(rb + u, 0), TGR - 1)
-                            if wr[u] != 0 and wc[v] != 0:
-                                assert 0 <= rb + u < TGR, ("lattice row tap outside tile", aa, bb)
-                            sx = f32(sx + s_gc[tr, tc] * wr[u])
-                            sy = f32(sy + s_gr[tr, tc] * wr[u])
-                        gxv = f32(gxv + sx * wc[v])
-                        gyv = f32(gyv + sy * wc[v])
+                            assert 0 <= cb + v < TGC, ("lattice column tap outside tile", bb)
+                        gxv = f32(gxv + s_rx[la, tc] * wc[v])
+                        gyv = f32(gyv + s_ry[la, tc] * wc[v])
                     n2 = f32(gxv * gxv + gyv * gyv)
                     if n2 > 0:
                         inv = f32(1.0 / np.sqrt(np.float64(n2)))
@@ -111,22 +126,27 @@ def model_frames(stack, M, R, S, iw=True):
                         xi = x0 + lanes
                         cM = M * xi + px
                         acc = np.zeros((Q, 64), np.float32)
-                        for i in range(-K, K + 1):
-                            c = ex + i + K
+                        for i in range(NW):
+                            c = ex + i
                             base = (c & 1) * LPL + ey * LP + lanes + (c >> 1)
                             rows = base[None, :] + np.arange(NR)[:, None] * LP
                             assert rows.min() >= 0 and rows.max() < 2 * LPL
                             g = s_lat[rows]                       # (NR, 64, 2)
-                            for jj in range(NW):
-                                e = ph * NW * NW + (i + K) * NW + jj
-                                A, B, w = ring[e, 0], ring[e, 1], ring[e, 2]
-                                for q in range(Q):
-                                    gg = g[2 * q + jj]
-                                    dot = gg[:, 0] * A + gg[:, 1] * B
-                                    crs = gg[:, 1] * A - gg[:, 0] * B
-                                    y = w - np.abs(crs)
-                                    contrib = np.median(np.stack([np.zeros(64, np.float32), -dot, y]), axis=0)
-                                    acc[q] = acc[q] + contrib.astype(np.float32)
+                            dxi = dxt[ph, i]
+                            for r in range(NR):
+                                px_ = (g[r, :, 0] * dxi).astype(np.float32)
+                                py_ = (g[r, :, 1] * dxi).astype(np.float32)
+                                for jj in range(NW):
+                                    rq = r - jj
+                                    if rq < 0 or rq % 2 or rq // 2 >= Q:
+                                        continue
+                                    q = rq // 2
+                                    dyj = dyt[ph, jj]
+                                    u = (g[r, :, 1] * dyj + px_).astype(np.float32)
+                                    v = (-g[r, :, 0] * dyj + py_).astype(np.float32)
+                                    y = (dtab[ph, i, jj] - np.abs(v)).astype(np.float32)
+                                    m = np.median(np.stack([np.zeros(64, np.float32), -u, y]), axis=0)
+                                    acc[q] = (acc[q] + ttab[ph, i, jj] * m).astype(np.float32)
                         cMs = np.minimum(cM, WM - 1)
                         wcol = t["ccw"][cMs]
                         ccol = t["ccb"][cMs] - (x0 - HI)
@@ -149,7 +169,7 @@ def model_frames(stack, M, R, S, iw=True):
                                         bb = b0[L] + ii - K
                                         if bb < 1 or bb > 2 * WM - 2:
                                             continue
-                                        s = f32(s + ring[ph * NW * NW + ii * NW + jj, 2])
+                                        s = f32(s + wtab[ph, ii, jj])
                                 den[L] = s
                             with np.errstate(invalid="ignore", divide="ignore"):
                                 rgc = np.where(den > 0, acc[q] / den, 0).astype(np.float32)

@@ -56,17 +56,20 @@ def test_plan_ring_tables_against_fp64_geometry():
             py, px = divmod(ph, M)
             ey, ex = (2 * py + 1) // M, (2 * px + 1) // M
             for j in range(-K, K + 1):
+                dy = (ey + j) / 2 - (py + 0.5) / M
+                assert abs(t["dy"][ph, j + K] - dy) <= 1e-7
                 for i in range(-K, K + 1):
-                    A, B, w, _ = t["ring"][ph, i + K, j + K]
-                    dy = (ey + j) / 2 - (py + 0.5) / M
                     dx = (ex + i) / 2 - (px + 0.5) / M
+                    assert abs(t["dx"][ph, i + K] - dx) <= 1e-7
+                    w, tt, dd = t["w"][ph, i + K, j + K], t["t"][ph, i + K, j + K], t["d"][ph, i + K, j + K]
                     d = np.hypot(dx, dy)
                     inring = d != 0 and d <= p["tSO"]
-                    assert (w > 0) == inring
+                    assert (w > 0) == inring and (tt > 0) == inring
+                    assert abs(dd - d) <= 1e-6
                     if inring:
                         dw = (d * np.exp(-d * d / p["tSS"])) ** 4
                         assert abs(w - dw) <= 2e-5 * dw  # w is built from the float32 distance, as the reference does
-                        assert abs(A - dx * dw / d) <= 2e-5 * dw and abs(B - dy * dw / d) <= 2e-5 * dw
+                        assert abs(tt - dw / d) <= 2e-5 * dw / d
 
 
 def test_lattice_taps_are_catmull_rom_at_half_integers():

@@ -64,7 +64,7 @@ def test_rgc_class_match_golden(torch, golden):
     rgc = RadialGradientConvergence(verbose=False)
     out = rgc.run(gx, gy, mag, magnification=2, radius=1.0, sensitivity=3.0, doIntensityWeighting=False)
     assert _report("RGC class vs reference", out, golden[f"{name}/rgc_maps"]) <= TOL
-    out = rgc.run(gx, gy, mag, magnification=2, radius=1.0, sensitivity=3.0, doIntensityWeighting=False,
+    out = rgc.run(gx, gy, mag, magnification=2, radius=1.0, sensitivity=3.0, doIntensityWeighting=True,
                   pad_edges=True)
     assert _report("RGC class pad_edges vs reference", out, golden[f"{name}/rgc_pad_edges"]) <= TOL
 
@@ -140,7 +140,7 @@ def test_large_u16_properties(torch):
     avg64 = _hip.esrrf_temporal(x[:64], M, R, S, True, ("AVG",))["AVG"]
     assert _report("large AVG(64) vs mean of frames", avg64.cpu().numpy(),
                    frames.double().mean(0).cpu().numpy()) <= 1e-5
-    t = _hip.esrrf_temporal(x[:64].transpose(1, 2).contiguous(), M, R, S, True, ("AVG",))["AVG"]
+    t = _hip.esrrf_temporal(x[:64].float().transpose(1, 2).contiguous(), M, R, S, True, ("AVG",))["AVG"]
     assert _report("large AVG(64) transpose equivariance", t.T.cpu().numpy(), avg64.cpu().numpy()) <= 1e-5
 
 

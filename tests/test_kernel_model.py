@@ -23,9 +23,11 @@ def test_model_matches_golden(golden, name):
     ((1, 1, 1), 3, 1.5, 1.0),        # one pixel
     ((1, 11, 7), 7, 1.7, 2.5),       # two phase groups (49 phases), non-integer sensitivity
 ])
-def test_model_edge_shapes(oracle_built, shape, M, R, S):
+@pytest.mark.parametrize("Q", [None, 8])
+def test_model_edge_shapes(oracle_built, shape, M, R, S, Q):
+    """Q=None: the per-frame kernel's rows per lane; Q=8: the temporal kernel's."""
     rng = np.random.default_rng(1)
     x = (rng.random(shape) * 100 + 1).astype(np.float32)
-    out = model_frames(x, M, R, S)
+    out = model_frames(x, M, R, S, Q=Q)
     ref = oracle_built.esrrf(x, M, R, S)
     assert rel_err(out, ref)[0] <= 1e-5
