@@ -35,7 +35,7 @@ def ring_samples_mean(H, W, M, R):
     """Mean in-disc ring-sample count over the M*M output phases (interior pixels)."""
     from nanopyx_amd import _hip
     t = _hip.plan_tables(H, W, M, R)
-    return float((t["ring"][..., 2] > 0).reshape(M * M, -1).sum(1).mean())
+    return float((t["w"] > 0).reshape(M * M, -1).sum(1).mean())
 
 
 def traffic_from_profile(workload):

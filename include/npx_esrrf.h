@@ -100,6 +100,12 @@ int npx_plan_info(int H, int W, int M, double R, int *info, float *finfo);
 int npx_plan_tables(int H, int W, int M, double R, float *ring, float *wsum, int *lrb, float *lrw, int *lcb,
                     float *lcw, int *crb, float *crw, int *ccb, float *ccw);
 
+/* Launch shape of the fused kernels for subsequent calls: at most max_waves_per_block output phases (one
+ * wavefront each) per block, and the block count the temporal kernel aims for when it splits the frames into
+ * chunks (0 = defaults 16 / 2048). No reference counterpart: it is what the autotuner varies. Changing it
+ * between calls changes the frame chunking and therefore the last bits of the temporal moments. */
+int npx_set_launch(int max_waves_per_block, int target_blocks);
+
 /* Turns per-call HIP-event timing of the eSRRF kernels on (1) or off (0). No reference counterpart: it lets a
  * caller (the benchmark) read the fused kernel's own duration on the stream it was launched on. */
 int npx_timing(int on);

@@ -61,6 +61,7 @@ def lib():
         L.npx_plan_info.argtypes = [_c_int, _c_int, _c_int, _c_double, _ip, _fp]
         L.npx_plan_tables.argtypes = [_c_int, _c_int, _c_int, _c_double, _fp, _fp, _ip, _fp, _ip, _fp, _ip, _fp,
                                       _ip, _fp]
+        L.npx_set_launch.argtypes = [_c_int, _c_int]
         L.npx_timing.argtypes = [_c_int]
         L.npx_timing_read.argtypes = [_ip, ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]
         L.npx_release.restype = None
@@ -229,6 +230,11 @@ def plan_tables(H, W, M, R):
                  t=ring[:, 2 * NW:2 * NW + 2 * NW * NW].reshape(P, NW, NW, 2)[..., 1],
                  w=ring[:, 2 * NW + 2 * NW * NW:2 * NW + 3 * NW * NW].reshape(P, NW, NW))
     return dict(info=d, ring=ring, wsum=wsum, **views, **arr)
+
+
+def set_launch(max_waves_per_block: int = 0, target_blocks: int = 0) -> None:
+    """Launch shape of the fused kernels for subsequent calls (0 = library defaults)."""
+    _check(lib().npx_set_launch(int(max_waves_per_block), int(target_blocks)))
 
 
 def timing(on: bool) -> None:

@@ -169,10 +169,15 @@ struct Shape {
     int nwaves, ngroups, nregx, nregy;
 };
 
+// Launch-shape knobs (npx_set_launch): wavefronts per block (one output phase each) and the number of
+// blocks the temporal kernel aims for when it splits the frames into chunks.
+int g_max_waves = 16;
+int g_target_blocks = 2048;
+
 Shape fused_shape(int H, int W, int M) {
     Shape s;
     const int nphase = M * M;
-    s.ngroups = (nphase + 15) / 16;
+    s.ngroups = (nphase + g_max_waves - 1) / g_max_waves;
     s.nwaves = (nphase + s.ngroups - 1) / s.ngroups;
     s.nregx = (W + 63) / 64;
     s.nregy = (H + npx::kQTemporal - 1) / npx::kQTemporal;
@@ -183,7 +188,7 @@ Shape fused_shape(int H, int W, int M) {
 // chunk keeps >= 32 frames so the moment merge stays a small fraction of the work.
 void temporal_chunks(int N, const Shape& s, int* nchunk, int* fpc) {
     const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
-    long long want = (2048 + spatial - 1) / spatial;
+    long long want = (g_target_blocks + spatial - 1) / spatial;
     want = std::min<long long>(want, std::max(1, N / 32));
     want = std::max<long long>(want, 1);
     int f = static_cast<int>((N + want - 1) / want);
@@ -415,6 +420,15 @@ int npx_plan_tables(int H, int W, int M, double R, float* ring, float* wsum, int
     put(h.lat_col, lcb, lcw);
     put(h.cr_row, crb, crw);
     put(h.cr_col, ccb, ccw);
+    return NPX_OK;
+}
+
+int npx_set_launch(int max_waves_per_block, int target_blocks) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (max_waves_per_block < 0 || max_waves_per_block > 16 || target_blocks < 0)
+        return fail(NPX_ERR_ARG, "max_waves_per_block must be in [1, 16] (0 = default), target_blocks >= 0");
+    g_max_waves = max_waves_per_block ? max_waves_per_block : 16;
+    g_target_blocks = target_blocks ? target_blocks : 2048;
     return NPX_OK;
 }
 

@@ -9,7 +9,7 @@ constexpr int kModeFrames = 0;    // per-frame RGC maps (eSRRF_ST.run contract)
 constexpr int kModeTemporal = 1;  // AVG / VAR / TAC2 moments streamed over the frames
 constexpr int kQFrames = 16;      // input rows per lane, per-frame-map kernel
 constexpr int kQTemporal = 8;     // input rows per lane, temporal kernel (5 moments per pixel in registers)
-constexpr int kPrefetch = 8;      // input-tile elements per thread prefetched for the next frame
+constexpr int kPrefetch = 4;      // input-tile elements per thread prefetched for the next frame
 
 // LDS tile geometry of the fused kernel for window half-width K and Q rows per lane.
 template <int K, int Q>
@@ -21,7 +21,7 @@ struct Geometry {
     static constexpr int LR = 2 * Q + 2 * K;      // lattice rows per block
     static constexpr int LP = 64 + K;             // lattice float2 per row in one column-parity plane
     static constexpr int NW = 2 * K + 1;
-    static constexpr int RS = ((2 * NW + 3 * NW * NW) + 15) / 16 * 16;  // ring block floats per phase
+    static constexpr int RS = ((2 * NW + 3 * NW * NW + 1) + 15) / 16 * 16;  // ring block floats per phase
 };
 
 // Device copies of the plan tables (npx_plan.h).

@@ -71,15 +71,29 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     constexpr int TIR = G::TIR, TIC = G::TIC;
     constexpr int TGR = G::TGR, TGC = G::TGC;
     constexpr int LR = G::LR, LP = G::LP, LPL = LR * LP;
+    constexpr int LB = 128 + 2 * K;  // lattice columns per block
     constexpr int NR = 2 * Q + 2 * K - 1;
-    constexpr int RDX = NW, RDT = 2 * NW, RW = 2 * NW + 2 * NW * NW;  // ring block offsets
+    constexpr int RDX = NW, RDT = 2 * NW, RW = 2 * NW + 2 * NW * NW, RSUM = 2 * NW + 3 * NW * NW;
 
     __shared__ float s_in[TIR * TIC];
-    __shared__ float s_gc[TGR * TGC];
-    __shared__ float s_gr[TGR * TGC];
     __shared__ float s_rx[LR * TGC];   // gradients filtered along rows (one row per lattice row)
     __shared__ float s_ry[LR * TGC];
     __shared__ float2 s_lat[2 * LPL];
+    // the Roberts gradients are dead once the row pass has read them, before the lattice is written: they
+    // share the lattice's storage
+    static_assert(2 * TGR * TGC <= 4 * LPL, "gradient tile must fit in the lattice storage");
+    float* const s_gc = reinterpret_cast<float*>(s_lat);
+    float* const s_gr = s_gc + TGR * TGC;
+    // frame-invariant tables of this block, staged once: lattice row / column taps (tile-relative bases,
+    // zero weights where the ring test excludes the lattice line), intensity row taps per (wave, q)
+    __shared__ float4 s_lrw[LR];
+    __shared__ int s_lrb[LR];
+    __shared__ float4 s_lcw[LB];
+    __shared__ int s_lcb[LB];
+    __shared__ float4 s_crw[16 * Q];
+    __shared__ int s_crb[16 * Q];
+    // dynamic: nwaves ring blocks of RS floats, then (temporal mode) the first frame's value of every pixel
+    extern __shared__ float s_ring[];
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -102,11 +116,50 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     const int cM = M * xi + px;
     const int a_lo = 2 * y0 - K;
     const int b_lo = 2 * x0 - K;
-    const float* __restrict__ rb_ph = a.t.ring + static_cast<size_t>(active ? ph : 0) * RS;
 
-    float mu[Q], m2[Q], cc[Q], fst[Q], prv[Q];
+    // ---- stage the frame-invariant tables
+    for (int k = tid; k < a.nwaves * RS; k += nthr) {
+        const int w = k / RS, phw = group * a.nwaves + w;
+        s_ring[k] = phw < a.nphase ? a.t.ring[static_cast<size_t>(phw) * RS + (k - w * RS)] : 0.f;
+    }
+    for (int la = tid; la < LR; la += nthr) {
+        const int aa = a_lo + la;
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        int b = 0;
+        if (aa >= 1 && aa <= 2 * a.HM - 2) {  // rows a >= 2H read the clamped last row (:45-52)
+            const int ai = min(aa, 2 * a.H);
+            w = a.t.lrw[ai];
+            b = a.t.lrb[ai] - (y0 - HG);
+        }
+        s_lrw[la] = w;
+        s_lrb[la] = b;
+    }
+    for (int lb = tid; lb < LB; lb += nthr) {
+        const int bb = b_lo + lb;
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        int b = 0;
+        if (bb >= 1 && bb <= 2 * a.WM - 2) {
+            const int bi = min(bb, 2 * a.W);
+            w = a.t.lcw[bi];
+            b = a.t.lcb[bi] - (x0 - HG);
+        }
+        s_lcw[lb] = w;
+        s_lcb[lb] = b;
+    }
+    for (int k = tid; k < a.nwaves * Q; k += nthr) {
+        const int w = k / Q, q = k - w * Q, phw = group * a.nwaves + w;
+        const int rM = min(M * (y0 + q) + (phw < a.nphase ? phw / M : 0), a.HM - 1);
+        s_crw[k] = a.t.crw[rM];
+        s_crb[k] = a.t.crb[rM] - (y0 - HI);
+    }
+    const int cMs = min(cM, a.WM - 1);
+    const float4 wcol = a.t.ccw[cMs];
+    const int ccol = a.t.ccb[cMs] - (x0 - HI);
+
+    float mu[Q], m2[Q], cc[Q], prv[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) mu[q] = m2[q] = cc[q] = fst[q] = prv[q] = 0.f;
+    for (int q = 0; q < Q; ++q) mu[q] = m2[q] = cc[q] = prv[q] = 0.f;
+    float* s_first = s_ring + a.nwaves * RS + (wave * Q) * 64 + lane;  // [wave][q][lane], stride 64
 
     // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
     float pf[kPrefetch];
@@ -119,6 +172,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
             pf[k] = load_px(a, f_begin, y0 - HI + r, x0 - HI + c);
         }
     }
+    const float* ringL = s_ring + wave * RS;
 
     for (int f = f_begin; f < f_end; ++f) {
         // (1) input tile + halo of frame f (zero outside the image; uint16 converted as a plain cast)
@@ -164,22 +218,17 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
         }
         __syncthreads();
         // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
-        // (rows a >= 2H read the clamped last row, :45-52); lattice rows the ring test excludes stay 0
         for (int idx = tid; idx < LR * TGC; idx += nthr) {
             const int la = idx / TGC, c = idx - la * TGC;
-            const int aa = a_lo + la;
+            const float4 wr = s_lrw[la];
+            const int rb = s_lrb[la];
+            const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
             float sx = 0.f, sy = 0.f;
-            if (aa >= 1 && aa <= 2 * a.HM - 2) {
-                const int ai = min(aa, 2 * a.H);
-                const float4 wr = a.t.lrw[ai];
-                const int rb = a.t.lrb[ai] - (y0 - HG);
-                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int tr = min(max(rb + u, 0), TGR - 1);
-                    sx += s_gc[tr * TGC + c] * wrv[u];
-                    sy += s_gr[tr * TGC + c] * wrv[u];
-                }
+            for (int u = 0; u < 4; ++u) {
+                const int tr = min(max(rb + u, 0), TGR - 1);
+                sx += s_gc[tr * TGC + c] * wrv[u];
+                sy += s_gr[tr * TGC + c] * wrv[u];
             }
             s_rx[idx] = sx;
             s_ry[idx] = sy;
@@ -192,25 +241,22 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
             const int rem = idx - plane * LPL;
             const int la = rem / LP;
             const int pos = rem - la * LP;
-            const int bb = b_lo + 2 * pos + plane;
-            float2 g = make_float2(0.f, 0.f);
-            if (bb >= 1 && bb <= 2 * a.WM - 2) {
-                const int bi = min(bb, 2 * a.W);
-                const float4 wc = a.t.lcw[bi];
-                const int cb = a.t.lcb[bi] - (x0 - HG);
-                const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
-                float gx = 0.f, gy = 0.f;
+            const int lb = 2 * pos + plane;
+            const float4 wc = s_lcw[lb];
+            const int cb = s_lcb[lb];
+            const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
+            float gx = 0.f, gy = 0.f;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int tc = min(max(cb + v, 0), TGC - 1);
-                    gx += s_rx[la * TGC + tc] * wcv[v];
-                    gy += s_ry[la * TGC + tc] * wcv[v];
-                }
-                const float n2 = gx * gx + gy * gy;
-                if (n2 > 0.f) {
-                    const float inv = rsqrtf(n2);
-                    g = make_float2(gx * inv, gy * inv);
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int tc = min(max(cb + v, 0), TGC - 1);
+                gx += s_rx[la * TGC + tc] * wcv[v];
+                gy += s_ry[la * TGC + tc] * wcv[v];
+            }
+            const float n2 = gx * gx + gy * gy;
+            float2 g = make_float2(0.f, 0.f);
+            if (n2 > 0.f) {
+                const float inv = rsqrtf(n2);
+                g = make_float2(gx * inv, gy * inv);
             }
             s_lat[idx] = g;
         }
@@ -219,25 +265,27 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
         // (4) ring sums. Sample (j, i) of pixel q reads lattice row 2q + ey + j + K, column 2*lane + ex + i + K.
         // With u = g.(dx, dy) and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's
         // contribution [G.d < 0] * w * (1 - |G x d| / (|G| d)) is t * med3(0, -u, d - |v|). g.x*dx and g.y*dx are
-        // formed once per lattice point of the column and shared by the samples that read it.
+        // formed once per lattice point of the column and shared by the samples that read it. The ring
+        // constants come from LDS (the same address for all lanes: one broadcast read).
         if (active) {
             float acc[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) acc[q] = 0.f;
             float dyv[NW];
 #pragma unroll
-            for (int jj = 0; jj < NW; ++jj) dyv[jj] = rb_ph[jj];
+            for (int jj = 0; jj < NW; ++jj) dyv[jj] = ringL[jj];
 #pragma unroll 1
             for (int i = 0; i < NW; ++i) {
                 const int c = ex + i;
                 const float2* col = s_lat + (c & 1) * LPL + ey * LP + lane + (c >> 1);
-                const float dxi = rb_ph[RDX + i];
-                const float* __restrict__ dt = rb_ph + RDT + 2 * i * NW;
+                const float dxi = ringL[RDX + i];
+                const float2* dt = reinterpret_cast<const float2*>(ringL + RDT) + i * NW;
                 float dv[NW], tv[NW];
 #pragma unroll
                 for (int jj = 0; jj < NW; ++jj) {
-                    dv[jj] = dt[2 * jj];
-                    tv[jj] = dt[2 * jj + 1];
+                    const float2 e = dt[jj];
+                    dv[jj] = e.x;
+                    tv[jj] = e.y;
                 }
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
@@ -257,11 +305,9 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                 }
             }
             // (5) normalise, sensitivity, intensity weight; then store or fold into the moments
-            const int cMs = min(cM, a.WM - 1);
-            const float4 wcol = a.t.ccw[cMs];
-            const int ccol = a.t.ccb[cMs] - (x0 - HI);
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
             const float inv_np1 = 1.0f / static_cast<float>(f - f_begin + 1);
+            const float wsum = ringL[RSUM];
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int yi = y0 + q;
@@ -274,7 +320,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                     const int a0 = 2 * yi + ey, b0 = 2 * xi + ex;
                     float den;
                     if (a0 - K >= 1 && a0 + K <= 2 * a.HM - 2 && b0 - K >= 1 && b0 + K <= 2 * a.WM - 2) {
-                        den = a.t.wsum[ph];
+                        den = wsum;
                     } else {
                         // edge pixel: sum the weights of in-bounds samples in the reference order (j outer)
                         den = 0.f;
@@ -284,16 +330,15 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                             for (int ii = 0; ii < NW; ++ii) {
                                 const int bb = b0 + ii - K;
                                 if (bb < 1 || bb > 2 * a.WM - 2) continue;
-                                den += rb_ph[RW + ii * NW + jj];
+                                den += ringL[RW + ii * NW + jj];
                             }
                         }
                     }
                     float rgc = den > 0.f ? acc[q] / den : 0.f;
                     rgc = apply_pow(rgc, a.pow_mode, a.S);
                     if (a.iw) {
-                        const int rMs = min(rM, a.HM - 1);
-                        const float4 wrow = a.t.crw[rMs];
-                        const int crow = a.t.crb[rMs] - (y0 - HI);
+                        const float4 wrow = s_crw[wave * Q + q];
+                        const int crow = s_crb[wave * Q + q];
                         const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
                         float sacc = 0.f;
 #pragma unroll
@@ -314,7 +359,10 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                 if (MODE == kModeFrames) {
                     if (inside) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
                 } else {
-                    moments_push(val, f - f_begin, inv_np1, mu[q], m2[q], cc[q], fst[q], prv[q]);
+                    float fq = val;
+                    if (f == f_begin) s_first[q * 64] = val;
+                    else fq = s_first[q * 64];
+                    moments_push(val, f - f_begin, inv_np1, mu[q], m2[q], cc[q], fq, prv[q]);
                 }
             }
         }
@@ -338,7 +386,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                 base[0] = mu[q];
                 base[P] = m2[q];
                 base[2 * P] = cc[q];
-                base[3 * P] = fst[q];
+                base[3 * P] = s_first[q * 64];
                 base[4 * P] = prv[q];
             }
         }
@@ -506,10 +554,12 @@ __global__ __launch_bounds__(256) void macro_pixel_divide(float* __restrict__ im
 
 template <int K>
 static hipError_t launch_fused_k(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
+    const size_t dyn = static_cast<size_t>(a.nwaves) * Geometry<K, kQTemporal>::RS * sizeof(float) +
+                       (mode == kModeTemporal ? static_cast<size_t>(a.nwaves) * kQTemporal * 64 * sizeof(float) : 0);
     if (mode == kModeFrames)
-        hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, dyn, st, a);
     else
-        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporal>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporal>), grid, block, dyn, st, a);
     return hipGetLastError();
 }
 

@@ -121,6 +121,7 @@ bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
         float s = 0.0f;  // distanceWeightSum, float, j outer / i inner (:30-35,64)
         for (int k = 0; k < NW * NW; ++k) s += wgrid[k];
         p.wsum[ph] = s;
+        blk[L.wsum] = s;
     }
     // Gradient lattice taps: lattice index a reads row pos = a*M of the x2M image, clamped to 2*H*M-1
     // (:45-52); that row samples the gradient at pos / (2M) (_le_esrrf.pyx:192-193).

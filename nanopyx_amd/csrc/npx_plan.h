@@ -40,11 +40,12 @@ struct PlanHost {
 //   dx[i]          at NW           offset of lattice column i
 //   (d, t)[i][j]   at 2*NW         distance and t = w / d (t = 0 outside the ring)
 //   w[i][j]        at 2*NW+2*NW^2  distance weight (0 outside the ring)
+//   wsum           at 2*NW+3*NW^2  sum of w (the interior normaliser)
 struct RingLayout {
-    int NW, dy, dx, dt, w, stride;
+    int NW, dy, dx, dt, w, wsum, stride;
     explicit RingLayout(int K)
-        : NW(2 * K + 1), dy(0), dx(NW), dt(2 * NW), w(2 * NW + 2 * NW * NW),
-          stride(((2 * NW + 3 * NW * NW) + 15) / 16 * 16) {}
+        : NW(2 * K + 1), dy(0), dx(NW), dt(2 * NW), w(2 * NW + 2 * NW * NW), wsum(2 * NW + 3 * NW * NW),
+          stride(((2 * NW + 3 * NW * NW + 1) + 15) / 16 * 16) {}
 };
 
 // Builds the tables; returns false with a message on unsupported input.
