@@ -334,7 +334,8 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
     dim3 block(64 * s.nwaves);
     CallEvents ev{};
     const bool timed = timing_begin(st, &ev);
-    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, npx::kModeTemporal, st);
+    const int mode = a.var ? npx::kModeTemporalVar : npx::kModeTemporal;
+    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, mode, st);
     if (e != hipSuccess) return hip_fail(e, "esrrf_fused(temporal) launch");
     if (timed) (void)hipEventRecord(ev.e1, st);
     if (nchunk > 1) {

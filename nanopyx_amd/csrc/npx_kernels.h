@@ -6,7 +6,8 @@
 namespace npx {
 
 constexpr int kModeFrames = 0;    // per-frame RGC maps (eSRRF_ST.run contract)
-constexpr int kModeTemporal = 1;  // AVG / VAR / TAC2 moments streamed over the frames
+constexpr int kModeTemporal = 1;  // AVG / TAC2 moments streamed over the frames
+constexpr int kModeTemporalVar = 2;  // ... plus the sum of squared deviations for VAR
 constexpr int kQFrames = 16;      // input rows per lane, per-frame-map kernel
 constexpr int kQTemporal = 8;     // input rows per lane, temporal kernel (5 moments per pixel in registers)
 constexpr int kPrefetch = 4;      // input-tile elements per thread prefetched for the next frame

@@ -36,11 +36,12 @@ __device__ __forceinline__ float apply_pow(float v, int pow_mode, float S) {
 
 // One step of the streaming moments. State: mu (mean), m2 (sum of squared deviations), c (lag-1
 // co-moment sum_t (x_t - mu)(x_{t+1} - mu)), first and prev values; n = values already absorbed.
+template <bool WITH_M2 = true>
 __device__ __forceinline__ void moments_push(float v, int n, float inv_np1, float& mu, float& m2, float& c,
                                              float& first, float& prev) {
     if (n == 0) {
         mu = v;
-        m2 = 0.f;
+        if (WITH_M2) m2 = 0.f;
         c = 0.f;
         first = v;
         prev = v;
@@ -50,7 +51,7 @@ __device__ __forceinline__ void moments_push(float v, int n, float inv_np1, floa
     const float mun = mu + delta * inv_np1;
     const float dmu = mun - mu;
     c = c + dmu * ((prev - mu) + (first - mu)) + static_cast<float>(n - 1) * dmu * dmu + (prev - mun) * (v - mun);
-    m2 = m2 + delta * (v - mun);
+    if (WITH_M2) m2 = m2 + delta * (v - mun);
     mu = mun;
     prev = v;
 }
@@ -156,9 +157,12 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     const float4 wcol = a.t.ccw[cMs];
     const int ccol = a.t.ccb[cMs] - (x0 - HI);
 
-    float mu[Q], m2[Q], cc[Q], prv[Q];
+    constexpr bool WITH_M2 = MODE == kModeTemporalVar;
+    float mu[Q], m2[WITH_M2 ? Q : 1], cc[Q], prv[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) mu[q] = m2[q] = cc[q] = prv[q] = 0.f;
+    for (int q = 0; q < Q; ++q) mu[q] = cc[q] = prv[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < (WITH_M2 ? Q : 1); ++q) m2[q] = 0.f;
     float* s_first = s_ring + a.nwaves * RS + (wave * Q) * 64 + lane;  // [wave][q][lane], stride 64
 
     // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
@@ -287,9 +291,14 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                     dv[jj] = e.x;
                     tv[jj] = e.y;
                 }
+                // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
+                float2 g1 = col[0];
+                float2 g2 = col[LP];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    const float2 g = col[r * LP];
+                    const float2 g = g1;
+                    g1 = g2;
+                    if (r + 2 < NR) g2 = col[(r + 2) * LP];
                     const float px_ = g.x * dxi;
                     const float py_ = g.y * dxi;
 #pragma unroll
@@ -362,14 +371,14 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                     float fq = val;
                     if (f == f_begin) s_first[q * 64] = val;
                     else fq = s_first[q * 64];
-                    moments_push(val, f - f_begin, inv_np1, mu[q], m2[q], cc[q], fq, prv[q]);
+                    moments_push<WITH_M2>(val, f - f_begin, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, prv[q]);
                 }
             }
         }
         __syncthreads();
     }
 
-    if (MODE == kModeTemporal && active && f_end > f_begin) {
+    if (MODE != kModeFrames && active && f_end > f_begin) {
         const size_t P = static_cast<size_t>(a.HM) * a.WM;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -379,12 +388,12 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
             if (a.nchunk == 1) {
                 const float n = static_cast<float>(f_end - f_begin);
                 if (a.avg) a.avg[pix] = mu[q];
-                if (a.var) a.var[pix] = m2[q] / n;
+                if (a.var) a.var[pix] = WITH_M2 ? m2[WITH_M2 ? q : 0] / n : 0.f;
                 if (a.tac2) a.tac2[pix] = cc[q] / (n - 1.0f);
             } else {
                 float* base = a.part + static_cast<size_t>(chunk) * 5 * P + pix;
                 base[0] = mu[q];
-                base[P] = m2[q];
+                base[P] = WITH_M2 ? m2[WITH_M2 ? q : 0] : 0.f;
                 base[2 * P] = cc[q];
                 base[3 * P] = s_first[q * 64];
                 base[4 * P] = prv[q];
@@ -555,11 +564,13 @@ __global__ __launch_bounds__(256) void macro_pixel_divide(float* __restrict__ im
 template <int K>
 static hipError_t launch_fused_k(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
     const size_t dyn = static_cast<size_t>(a.nwaves) * Geometry<K, kQTemporal>::RS * sizeof(float) +
-                       (mode == kModeTemporal ? static_cast<size_t>(a.nwaves) * kQTemporal * 64 * sizeof(float) : 0);
+                       (mode != kModeFrames ? static_cast<size_t>(a.nwaves) * kQTemporal * 64 * sizeof(float) : 0);
     if (mode == kModeFrames)
         hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, dyn, st, a);
-    else
+    else if (mode == kModeTemporal)
         hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporal>), grid, block, dyn, st, a);
+    else
+        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporalVar>), grid, block, dyn, st, a);
     return hipGetLastError();
 }
 
