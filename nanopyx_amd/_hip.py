@@ -228,7 +228,9 @@ def plan_tables(H, W, M, R):
     views = dict(dy=ring[:, :NW], dx=ring[:, NW:2 * NW],
                  d=ring[:, 2 * NW:2 * NW + 2 * NW * NW].reshape(P, NW, NW, 2)[..., 0],
                  t=ring[:, 2 * NW:2 * NW + 2 * NW * NW].reshape(P, NW, NW, 2)[..., 1],
-                 w=ring[:, 2 * NW + 2 * NW * NW:2 * NW + 3 * NW * NW].reshape(P, NW, NW))
+                 w=ring[:, 2 * NW + 2 * NW * NW:2 * NW + 3 * NW * NW].reshape(P, NW, NW),
+                 prefix=ring[:, 2 * NW + 3 * NW * NW + 1:2 * NW + 3 * NW * NW + 1 + (NW + 1) ** 2].reshape(P, NW + 1,
+                                                                                                     NW + 1))
     return dict(info=d, ring=ring, wsum=wsum, **views, **arr)
 
 

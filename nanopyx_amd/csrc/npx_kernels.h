@@ -22,7 +22,7 @@ struct Geometry {
     static constexpr int LR = 2 * Q + 2 * K;      // lattice rows per block
     static constexpr int LP = 64 + K;             // lattice float2 per row in one column-parity plane
     static constexpr int NW = 2 * K + 1;
-    static constexpr int RS = ((2 * NW + 3 * NW * NW + 1) + 15) / 16 * 16;  // ring block floats per phase
+    static constexpr int RS = ((2 * NW + 3 * NW * NW + 1 + (NW + 1) * (NW + 1)) + 15) / 16 * 16;  // ring block floats
 };
 
 // Device copies of the plan tables (npx_plan.h).

@@ -23,14 +23,15 @@ namespace npx {
 // ------------------------------------------------------------------------------------------ helpers
 
 __device__ __forceinline__ float apply_pow(float v, int pow_mode, float S) {
-    // reference: if RGC >= 0 && S > 1: RGC = pow(RGC, S) (:83-87); RGC here is never negative
+    // reference: if RGC >= 0 && S > 1: RGC = pow(RGC, S) (:83-87); RGC here is never negative. Integer
+    // sensitivities are exact products; others use 2^(S log2 v) on the hardware log2/exp2 (a few ulp).
     if (pow_mode == 2) return v * v;
     if (pow_mode == 3) return v * v * v;
     if (pow_mode == 4) {
         const float s = v * v;
         return s * s;
     }
-    if (pow_mode == 1) return powf(v, S);
+    if (pow_mode == 1) return v > 0.f ? __builtin_amdgcn_exp2f(S * __builtin_amdgcn_logf(v)) : 0.f;
     return v;
 }
 
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     constexpr int LR = G::LR, LP = G::LP, LPL = LR * LP;
     constexpr int LB = 128 + 2 * K;  // lattice columns per block
     constexpr int NR = 2 * Q + 2 * K - 1;
-    constexpr int RDX = NW, RDT = 2 * NW, RW = 2 * NW + 2 * NW * NW, RSUM = 2 * NW + 3 * NW * NW;
+    constexpr int RDX = NW, RDT = 2 * NW, RSUM = 2 * NW + 3 * NW * NW, RPRE = RSUM + 1;
 
     __shared__ float s_in[TIR * TIC];
     __shared__ float s_rx[LR * TGC];   // gradients filtered along rows (one row per lattice row)
@@ -158,12 +159,14 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
     const int ccol = a.t.ccb[cMs] - (x0 - HI);
 
     constexpr bool WITH_M2 = MODE == kModeTemporalVar;
-    float mu[Q], m2[WITH_M2 ? Q : 1], cc[Q], prv[Q];
+    float mu[Q], m2[WITH_M2 ? Q : 1], cc[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) mu[q] = cc[q] = prv[q] = 0.f;
+    for (int q = 0; q < Q; ++q) mu[q] = cc[q] = 0.f;
 #pragma unroll
     for (int q = 0; q < (WITH_M2 ? Q : 1); ++q) m2[q] = 0.f;
-    float* s_first = s_ring + a.nwaves * RS + (wave * Q) * 64 + lane;  // [wave][q][lane], stride 64
+    // first and previous frame's value of every pixel of the wave: [wave][q][lane] and [nwaves + wave][q][lane]
+    float* s_first = s_ring + a.nwaves * RS + (wave * Q) * 64 + lane;
+    float* s_prev = s_first + a.nwaves * Q * 64;
 
     // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
     float pf[kPrefetch];
@@ -275,21 +278,23 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
             float acc[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) acc[q] = 0.f;
+            // ring constants through scalar loads (wave-uniform, SGPRs): keeps the VGPRs for pixels and lattice rows
+            using cfloat = const __attribute__((address_space(4))) float;  // constant address space: s_load
+            cfloat* rg = (cfloat*)(a.t.ring + static_cast<size_t>(ph) * RS);
             float dyv[NW];
 #pragma unroll
-            for (int jj = 0; jj < NW; ++jj) dyv[jj] = ringL[jj];
+            for (int jj = 0; jj < NW; ++jj) dyv[jj] = rg[jj];
 #pragma unroll 1
             for (int i = 0; i < NW; ++i) {
                 const int c = ex + i;
                 const float2* col = s_lat + (c & 1) * LPL + ey * LP + lane + (c >> 1);
-                const float dxi = ringL[RDX + i];
-                const float2* dt = reinterpret_cast<const float2*>(ringL + RDT) + i * NW;
+                const float dxi = rg[RDX + i];
+                cfloat* dt = rg + RDT + 2 * i * NW;
                 float dv[NW], tv[NW];
 #pragma unroll
                 for (int jj = 0; jj < NW; ++jj) {
-                    const float2 e = dt[jj];
-                    dv[jj] = e.x;
-                    tv[jj] = e.y;
+                    dv[jj] = dt[2 * jj];
+                    tv[jj] = dt[2 * jj + 1];
                 }
                 // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
                 float2 g1 = col[0];
@@ -313,66 +318,65 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                     }
                 }
             }
-            // (5) normalise, sensitivity, intensity weight; then store or fold into the moments
+            // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Branch-free per
+            // pixel: pixels whose window crosses the border take their normaliser from the prefix-sum table.
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
             const float inv_np1 = 1.0f / static_cast<float>(f - f_begin + 1);
             const float wsum = ringL[RSUM];
+            const int b0 = 2 * xi + ex;
+            const int i0 = max(0, 1 - (b0 - K));
+            const int i1 = max(0, min(NW, 2 * a.WM - 2 - (b0 - K) + 1));  // lanes past the image: empty range
+            const bool col_ok = xi < a.W && cM >= a.margin && cM < a.WM - a.margin;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int yi = y0 + q;
                 const int rM = M * yi + py;
-                const bool inside = xi < a.W && yi < a.H;
-                const bool valid = inside && rM >= a.margin && rM < a.HM - a.margin && cM >= a.margin &&
-                                   cM < a.WM - a.margin;
-                float val = 0.f;
-                if (valid) {
-                    const int a0 = 2 * yi + ey, b0 = 2 * xi + ex;
-                    float den;
-                    if (a0 - K >= 1 && a0 + K <= 2 * a.HM - 2 && b0 - K >= 1 && b0 + K <= 2 * a.WM - 2) {
-                        den = wsum;
-                    } else {
-                        // edge pixel: sum the weights of in-bounds samples in the reference order (j outer)
-                        den = 0.f;
-                        for (int jj = 0; jj < NW; ++jj) {
-                            const int aa = a0 + jj - K;
-                            if (aa < 1 || aa > 2 * a.HM - 2) continue;
-                            for (int ii = 0; ii < NW; ++ii) {
-                                const int bb = b0 + ii - K;
-                                if (bb < 1 || bb > 2 * a.WM - 2) continue;
-                                den += ringL[RW + ii * NW + jj];
-                            }
-                        }
-                    }
-                    float rgc = den > 0.f ? acc[q] / den : 0.f;
-                    rgc = apply_pow(rgc, a.pow_mode, a.S);
-                    if (a.iw) {
-                        const float4 wrow = s_crw[wave * Q + q];
-                        const int crow = s_crb[wave * Q + q];
-                        const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
-                        float sacc = 0.f;
-#pragma unroll
-                        for (int v2 = 0; v2 < 4; ++v2) {
-                            const int tc = min(max(ccol + v2, 0), TIC - 1);
-                            float p = 0.f;
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                const int tr = min(max(crow + u, 0), TIR - 1);
-                                p += s_in[tr * TIC + tc] * wrv[u];
-                            }
-                            sacc += p * wcv[v2];
-                        }
-                        rgc = rgc * sacc;
-                    }
-                    val = rgc;
+                const int a0 = 2 * yi + ey;
+                const int j0 = max(0, 1 - (a0 - K));
+                const int j1 = max(0, min(NW, 2 * a.HM - 2 - (a0 - K) + 1));
+                const bool row_ok = yi < a.H && rM >= a.margin && rM < a.HM - a.margin;  // uniform
+                float den = wsum;
+                if (j0 > 0 || j1 < NW || i0 > 0 || i1 < NW) {
+                    const float* P = ringL + RPRE;
+                    const int jl = min(j0, j1), il = min(i0, i1);
+                    den = (P[j1 * (NW + 1) + i1] - P[jl * (NW + 1) + i1]) - (P[j1 * (NW + 1) + il] - P[jl * (NW + 1) + il]);
                 }
+                float rgc = den > 0.f ? acc[q] / den : 0.f;
+                rgc = apply_pow(rgc, a.pow_mode, a.S);
+                if (a.iw) {
+                    const float4 wrow = s_crw[wave * Q + q];
+                    const int crow = s_crb[wave * Q + q];
+                    const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
+                    float sacc = 0.f;
+#pragma unroll
+                    for (int v2 = 0; v2 < 4; ++v2) {
+                        const int tc = min(max(ccol + v2, 0), TIC - 1);
+                        float pp = 0.f;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int tr = min(max(crow + u, 0), TIR - 1);
+                            pp += s_in[tr * TIC + tc] * wrv[u];
+                        }
+                        sacc += pp * wcv[v2];
+                    }
+                    rgc = rgc * sacc;
+                }
+                const float val = (row_ok && col_ok) ? rgc : 0.f;
                 if (MODE == kModeFrames) {
-                    if (inside) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
+                    if (xi < a.W && yi < a.H) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
                 } else {
-                    float fq = val;
-                    if (f == f_begin) s_first[q * 64] = val;
-                    else fq = s_first[q * 64];
-                    moments_push<WITH_M2>(val, f - f_begin, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, prv[q]);
+                    float fq = val, pq = val;
+                    if (f == f_begin) {
+                        s_first[q * 64] = val;
+                    } else {
+                        fq = s_first[q * 64];
+                        pq = s_prev[q * 64];
+                    }
+                    moments_push<WITH_M2>(val, f - f_begin, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
+                    s_prev[q * 64] = val;
                 }
+                // one pixel at a time: keeps the finalise's live registers low (no interleaving across q)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(1024) void esrrf_fused(FusedArgs a) {
                 base[P] = WITH_M2 ? m2[WITH_M2 ? q : 0] : 0.f;
                 base[2 * P] = cc[q];
                 base[3 * P] = s_first[q * 64];
-                base[4 * P] = prv[q];
+                base[4 * P] = s_prev[q * 64];
             }
         }
     }
@@ -564,7 +568,7 @@ __global__ __launch_bounds__(256) void macro_pixel_divide(float* __restrict__ im
 template <int K>
 static hipError_t launch_fused_k(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
     const size_t dyn = static_cast<size_t>(a.nwaves) * Geometry<K, kQTemporal>::RS * sizeof(float) +
-                       (mode != kModeFrames ? static_cast<size_t>(a.nwaves) * kQTemporal * 64 * sizeof(float) : 0);
+                       (mode != kModeFrames ? 2 * static_cast<size_t>(a.nwaves) * kQTemporal * 64 * sizeof(float) : 0);
     if (mode == kModeFrames)
         hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, dyn, st, a);
     else if (mode == kModeTemporal)

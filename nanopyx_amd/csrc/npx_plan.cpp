@@ -122,6 +122,15 @@ bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
         for (int k = 0; k < NW * NW; ++k) s += wgrid[k];
         p.wsum[ph] = s;
         blk[L.wsum] = s;
+        // prefix sums for pixels whose window is cut by the image border (rows / columns outside the valid
+        // lattice range form a prefix or a suffix of the window)
+        for (int j = 0; j <= NW; ++j)
+            for (int i = 0; i <= NW; ++i) {
+                double acc = 0.0;
+                for (int jj = 0; jj < j; ++jj)
+                    for (int ii = 0; ii < i; ++ii) acc += wgrid[jj * NW + ii];
+                blk[L.prefix + j * (NW + 1) + i] = static_cast<float>(acc);
+            }
     }
     // Gradient lattice taps: lattice index a reads row pos = a*M of the x2M image, clamped to 2*H*M-1
     // (:45-52); that row samples the gradient at pos / (2M) (_le_esrrf.pyx:192-193).

@@ -41,11 +41,13 @@ struct PlanHost {
 //   (d, t)[i][j]   at 2*NW         distance and t = w / d (t = 0 outside the ring)
 //   w[i][j]        at 2*NW+2*NW^2  distance weight (0 outside the ring)
 //   wsum           at 2*NW+3*NW^2  sum of w (the interior normaliser)
+//   P[j][i]        at 2*NW+3*NW^2+1  (NW+1)^2 prefix sums of w over rows < j, columns < i (edge normalisers)
 struct RingLayout {
-    int NW, dy, dx, dt, w, wsum, stride;
+    int NW, dy, dx, dt, w, wsum, prefix, stride;
     explicit RingLayout(int K)
         : NW(2 * K + 1), dy(0), dx(NW), dt(2 * NW), w(2 * NW + 2 * NW * NW), wsum(2 * NW + 3 * NW * NW),
-          stride(((2 * NW + 3 * NW * NW + 1) + 15) / 16 * 16) {}
+          prefix(2 * NW + 3 * NW * NW + 1),
+          stride(((2 * NW + 3 * NW * NW + 1 + (NW + 1) * (NW + 1)) + 15) / 16 * 16) {}
 };
 
 // Builds the tables; returns false with a message on unsupported input.

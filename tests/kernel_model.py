@@ -159,18 +159,18 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                             b0 = 2 * xi + ex
                             interior = (a0 - K >= 1) & (a0 + K <= 2 * HM - 2) & (b0 - K >= 1) & (b0 + K <= 2 * WM - 2)
                             den = np.full(64, wsum[ph], np.float32)
-                            for L in np.nonzero(valid & ~interior)[0]:
-                                s = f32(0)
-                                for jj in range(NW):
-                                    aa = a0 + jj - K
-                                    if aa < 1 or aa > 2 * HM - 2:
-                                        continue
-                                    for ii in range(NW):
-                                        bb = b0[L] + ii - K
-                                        if bb < 1 or bb > 2 * WM - 2:
-                                            continue
-                                        s = f32(s + wtab[ph, ii, jj])
-                                den[L] = s
+                            # border pixels: normaliser from the 2D prefix sums of w over the valid window part
+                            P = t["prefix"][ph]
+                            j0 = max(0, 1 - (a0 - K))
+                            j1 = max(0, min(NW, 2 * HM - 2 - (a0 - K) + 1))
+                            i0 = np.maximum(0, 1 - (b0 - K))
+                            i1 = np.maximum(0, np.minimum(NW, 2 * WM - 2 - (b0 - K) + 1))
+                            assert 0 <= j0 <= NW and 0 <= j1 <= NW and (i0 >= 0).all() and (i0 <= NW).all()
+                            edge = (j0 > 0) | (j1 < NW) | (i0 > 0) | (i1 < NW)
+                            jl = min(j0, j1)
+                            il = np.minimum(i0, i1)
+                            pd = ((P[j1, i1] - P[jl, i1]) - (P[j1, il] - P[jl, il])).astype(np.float32)
+                            den = np.where(edge, pd, den).astype(np.float32)
                             with np.errstate(invalid="ignore", divide="ignore"):
                                 rgc = np.where(den > 0, acc[q] / den, 0).astype(np.float32)
                             if pow_mode == 2:
@@ -180,7 +180,7 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                             elif pow_mode == 4:
                                 rgc = (rgc * rgc) * (rgc * rgc)
                             elif pow_mode == 1:
-                                rgc = np.power(rgc, Sf).astype(np.float32)
+                                rgc = np.where(rgc > 0, np.exp2(Sf * np.log2(np.maximum(rgc, 1e-38))), 0).astype(np.float32)
                             if iw:
                                 rMs = min(rM, HM - 1)
                                 wrow = t["crw"][rMs]
