@@ -47,6 +47,58 @@ def traffic_from_profile(workload):
     return float(m.group(1)) if m else None
 
 
+def sweep_bench(reps):
+    """The 32-setting parameter sweep (200 x 256 x 256 float32, radius {1, 1.5, 2, 2.5} x sensitivity {1, 2, 3, 4} x
+    magnification {2, 4}, AVG) as one batched launch, as its captured hipGraph, and as 32 sequential single-setting
+    fused launches; times from the host clock around `reps` synchronised repetitions of each."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    sc = synthetic.CONFIGS["sweep_256"]
+    e = sc.esrrf
+    mags, radii, sens = list(e["magnification"]), list(e["radius"]), list(e["sensitivity"])
+    x = torch.from_numpy(synthetic.generate(sc)).cuda()
+    N, H, W = x.shape
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    outs = _hip.SweepOutputs(torch, x, mags, radii, sens, workspace=True)
+    t_sweep = timed(lambda: _hip.esrrf_sweep(x, mags, radii, sens, outputs=outs))
+    _hip.timing(True)
+    _hip.esrrf_sweep(x, mags, radii, sens, outputs=outs)
+    torch.cuda.synchronize()
+    _hip.timing(False)
+    _, k_ms, m_ms = _hip.timing_read()
+    with _hip.Graph.capture() as g:
+        _hip.esrrf_sweep(x, mags, radii, sens, stream=g.stream, outputs=outs)
+    t_graph = timed(g.launch)
+    g.close()
+    settings = [(m, r, s) for m in mags for r in radii for s in sens]
+    t_seq = timed(lambda: [_hip.esrrf_temporal(x, m, r, s, True, ("AVG",)) for (m, r, s) in settings])
+    # deduplicated algorithmic work: ring sums once per (R, M), the final power/weight/mean per setting
+    flops = 0.0
+    for m in mags:
+        U = N * H * m * W * m
+        for r in radii:
+            flops += U * (9.0 * ring_samples_mean(H, W, m, r) + 32.0)
+        flops += U * 8.0 * len(radii) * len(sens)
+    k_s = k_ms / 1e3
+    return {"workload": f"sweep_256: {N} frames x {H}x{W} float32, M{mags} x R{radii} x S{sens}, AVG "
+                        f"({len(settings)} settings)",
+            "batched_ms": round(t_sweep * 1e3, 3), "graph_ms": round(t_graph * 1e3, 3),
+            "sequential_32_ms": round(t_seq * 1e3, 3),
+            "speedup_vs_sequential": round(t_seq / min(t_sweep, t_graph), 2),
+            "kernel_ms": round(k_ms, 3), "merge_ms": round(m_ms, 3),
+            "flops_dedup": flops, "tflops_kernel": round(flops / k_s / 1e12, 2),
+            "frac_fp32_peak": round(flops / k_s / 1e12 / FP32_PEAK_TFLOPS, 4)}
+
+
 def cpu_baseline(x, e, seconds):
     """The reference CPU chain (CR xM, Roberts, CR x2M twice, RGC, temporal) on the first frames of the same
     stack, chunk after chunk until `seconds` of wall clock have passed."""
@@ -94,6 +146,7 @@ def main():
     ap.add_argument("--workload", default="large_512_u16")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the eSRRF path is built and measured on a single MI355X "
@@ -168,7 +221,12 @@ def main():
                      "algorithmic_bytes": alg_bytes,
                      "hbm_gbs_algorithmic": round(alg_bytes / k_s / 1e9, 1)},
         "cpu_baseline": None,
+        "sweep": None,
     }
+    if not args.no_sweep:
+        log("parameter sweep ...")
+        result["sweep"] = sweep_bench(max(3, args.steps // 2))
+        log(f"sweep: {result['sweep']}")
     if not args.no_cpu:
         log("timing the reference CPU path ...")
         result["cpu_baseline"] = cpu_baseline(x_host, e, args.cpu_seconds)

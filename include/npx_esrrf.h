@@ -80,6 +80,37 @@ int npx_rgc(const float *d_gx, const float *d_gy, const float *d_img, int N, int
 int npx_temporal(const float *d_stack, int N, long long P, unsigned mode_mask, float *d_avg, float *d_var,
                  float *d_tac2, void *stream);
 
+/* Bytes of device workspace npx_esrrf_sweep uses to split the frames of one sweep over more blocks. */
+size_t npx_esrrf_sweep_workspace(int N, int H, int W, int n_mag, const int *mags, int n_rad, const double *radii,
+                                 int n_sens);
+
+/*
+ * Parameter sweep: the temporal AVG map of eSRRF for every (magnification, radius, sensitivity) of the three lists,
+ * from one device stack, in one batched launch (plus one merge launch per magnification when d_work is given).
+ * Sensitivities share the ring sums (only the final power differs) and the radii of one magnification share the
+ * gradient lattice. d_avg is a host array of n_mag * n_rad * n_sens device pointers; map (m, r, s) is
+ * d_avg[(m * n_rad + r) * n_sens + s], (H*mags[m], W*mags[m]) float32. Replaces the loop of
+ * ParameterSweep.run, src/nanopyx/core/analysis/parameter_sweep.py:33-42 (one eSRRF(...).run per (sensitivity,
+ * radius) at fixed magnification, :35-37, then calculate_eSRRF_temporal_correlations(rgc_map, "AVG"), :40).
+ * d_work may be NULL (work_bytes 0): the frames are then not split.
+ */
+int npx_esrrf_sweep(const void *d_in, int dtype, int N, int H, int W, int n_mag, const int *mags, int n_rad,
+                    const double *radii, int n_sens, const double *sens, int iw, float *const *d_avg, void *d_work,
+                    size_t work_bytes, void *stream);
+
+/*
+ * HIP graph capture of a launch sequence. npx_capture_begin creates a private stream in capture mode; every npx_*
+ * call made on it until npx_capture_end is recorded (not executed), and npx_capture_end returns an instantiated
+ * graph that npx_graph_launch replays on any stream. Calls made under capture need their geometry tables on the
+ * device already (make one ordinary call with the same parameters first); they record no memset nodes. No reference
+ * counterpart: it replaces the reference's per-call launch sequence (.wait() after each OpenCL kernel,
+ * src/nanopyx/core/transform/_le_esrrf.pyx:106-163) with one graph launch.
+ */
+int npx_capture_begin(void **capture_stream);
+int npx_capture_end(void *capture_stream, void **graph);
+int npx_graph_launch(void *graph, void *stream);
+int npx_graph_destroy(void *graph);
+
 /*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.

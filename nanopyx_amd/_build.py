@@ -6,6 +6,7 @@ roundings follow the reference's arithmetic operation by operation.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -41,15 +42,22 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     inc = ["-I", CSRC, "-I", os.path.join(os.path.dirname(PKG), "include")]
     plan_o = os.path.join(BUILD_DIR, "npx_plan.o")
     kern_o = os.path.join(BUILD_DIR, "npx_kernels.o")
+    batch_o = os.path.join(BUILD_DIR, "npx_batched.o")
     capi_o = os.path.join(BUILD_DIR, "npx_capi.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", *inc, "-c",
-          os.path.join(CSRC, "npx_plan.cpp"), "-o", plan_o], verbose)
-    _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc, *extra, "-c",
-          os.path.join(CSRC, "npx_kernels.hip"), "-o", kern_o], verbose)
-    _run([HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc, "-x", "hip", "-c",
-          os.path.join(CSRC, "npx_capi.cpp"), "-o", capi_o], verbose)
+    hip = [HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc]
+    steps = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", *inc, "-c",
+         os.path.join(CSRC, "npx_plan.cpp"), "-o", plan_o],
+        [*hip, *extra, "-c", os.path.join(CSRC, "npx_kernels.hip"), "-o", kern_o],
+        [*hip, *extra, "-c", os.path.join(CSRC, "npx_batched.hip"), "-o", batch_o],
+        [*hip, "-x", "hip", "-c", os.path.join(CSRC, "npx_capi.cpp"), "-o", capi_o],
+    ]
+    # the translation units are independent: compile them concurrently
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(steps)) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd in steps]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, capi_o, plan_o, "-o", tmp], verbose)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, batch_o, capi_o, plan_o, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -64,6 +64,16 @@ def lib():
         L.npx_set_launch.argtypes = [_c_int, _c_int]
         L.npx_timing.argtypes = [_c_int]
         L.npx_timing_read.argtypes = [_ip, ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]
+        L.npx_esrrf_sweep_workspace.argtypes = [_c_int, _c_int, _c_int, _c_int, _ip, _c_int,
+                                                ctypes.POINTER(_c_double), _c_int]
+        L.npx_esrrf_sweep_workspace.restype = _c_size_t
+        L.npx_esrrf_sweep.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _ip, _c_int,
+                                      ctypes.POINTER(_c_double), _c_int, ctypes.POINTER(_c_double), _c_int,
+                                      ctypes.POINTER(_c_void_p), _c_void_p, _c_size_t, _c_void_p]
+        L.npx_capture_begin.argtypes = [ctypes.POINTER(_c_void_p)]
+        L.npx_capture_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
+        L.npx_graph_launch.argtypes = [_c_void_p, _c_void_p]
+        L.npx_graph_destroy.argtypes = [_c_void_p]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -154,6 +164,99 @@ def esrrf_temporal(stack, magnification, radius, sensitivity, iw, modes=("AVG",)
     if work is not None:
         work.record_stream(torch.cuda.current_stream() if stream is None else stream)
     return outs
+
+
+def _sweep_lists(magnifications, radii, sensitivities):
+    mags = (ctypes.c_int * len(magnifications))(*[int(m) for m in magnifications])
+    rads = (ctypes.c_double * len(radii))(*[float(r) for r in radii])
+    sens = (ctypes.c_double * len(sensitivities))(*[float(s) for s in sensitivities])
+    return mags, rads, sens
+
+
+def sweep_workspace_bytes(N, H, W, magnifications, radii, sensitivities) -> int:
+    mags, rads, _ = _sweep_lists(magnifications, radii, sensitivities)
+    return int(lib().npx_esrrf_sweep_workspace(N, H, W, len(mags), mags, len(rads), rads, len(sensitivities)))
+
+
+class SweepOutputs:
+    """Output maps of a parameter sweep: ``maps[(M, R, S)]`` is the (H*M, W*M) AVG device tensor."""
+
+    def __init__(self, torch, x, magnifications, radii, sensitivities, workspace):
+        N, H, W = x.shape
+        self.key = [(int(m), float(r), float(s)) for m in magnifications for r in radii for s in sensitivities]
+        self.maps = {}
+        for (m, r, s) in self.key:
+            self.maps[(m, r, s)] = torch.empty((H * m, W * m), dtype=torch.float32, device=x.device)
+        self.ptrs = (_c_void_p * len(self.key))(*[self.maps[k].data_ptr() for k in self.key])
+        nbytes = sweep_workspace_bytes(N, H, W, magnifications, radii, sensitivities) if workspace else 0
+        self.nbytes = nbytes
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=x.device) if nbytes else None
+
+
+def esrrf_sweep(stack, magnifications, radii, sensitivities, iw=True, stream=None, workspace=True, outputs=None):
+    """eSRRF temporal AVG for every (magnification, radius, sensitivity) in one batched launch.
+
+    Returns a SweepOutputs whose ``maps[(M, R, S)]`` are device tensors. Passing the SweepOutputs of an earlier call
+    with the same stack shape and lists reuses its buffers."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    mags, rads, sens = _sweep_lists(magnifications, radii, sensitivities)
+    out = outputs if outputs is not None else SweepOutputs(torch, x, magnifications, radii, sensitivities, workspace)
+    _check(lib().npx_esrrf_sweep(x.data_ptr(), _dtype_code(x), N, H, W, len(mags), mags, len(rads), rads, len(sens),
+                                 sens, int(bool(iw)), out.ptrs,
+                                 out.work.data_ptr() if out.work is not None else None, out.nbytes,
+                                 _stream(torch, stream)))
+    return out
+
+
+class Graph:
+    """An instantiated HIP graph of npx_* launches (npx_capture_begin / npx_capture_end).
+
+    ``with Graph.capture() as g: fn(stream=g.stream)`` records the calls made with ``stream=g.stream``;
+    ``g.launch()`` replays them on the current stream."""
+
+    def __init__(self):
+        self._exec = None
+        self.stream = None
+
+    @classmethod
+    def capture(cls):
+        return cls()
+
+    def __enter__(self):
+        torch = _torch()
+        s = _c_void_p()
+        _check(lib().npx_capture_begin(ctypes.byref(s)))
+        self._raw = s.value
+        self.stream = torch.cuda.ExternalStream(self._raw)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        g = _c_void_p()
+        rc = lib().npx_capture_end(self._raw, ctypes.byref(g))
+        self.stream = None
+        if exc_type is None:
+            _check(rc)
+            self._exec = g.value
+        elif rc == 0:
+            lib().npx_graph_destroy(g.value)
+        return False
+
+    def launch(self, stream=None):
+        torch = _torch()
+        _check(lib().npx_graph_launch(self._exec, _stream(torch, stream)))
+
+    def close(self):
+        if self._exec is not None:
+            lib().npx_graph_destroy(self._exec)
+            self._exec = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rgc(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=None):

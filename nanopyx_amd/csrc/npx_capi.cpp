@@ -149,9 +149,11 @@ std::mutex g_tmu;
 bool g_timing = false;
 std::vector<CallEvents> g_events;
 
+bool capturing(hipStream_t st);
+
 bool timing_begin(hipStream_t st, CallEvents* ev) {
     std::lock_guard<std::mutex> lk(g_tmu);
-    if (!g_timing) return false;
+    if (!g_timing || capturing(st)) return false;
     if (hipEventCreate(&ev->e0) != hipSuccess) return false;
     if (hipEventCreate(&ev->e1) != hipSuccess) return false;
     if (hipEventCreate(&ev->e2) != hipSuccess) return false;
@@ -213,8 +215,9 @@ int check_common(const void* d_in, int dtype, int N, int H, int W, int M) {
     return NPX_OK;
 }
 
-npx::FusedArgs base_args(const void* d_in, int dtype, int N, int H, int W, int M, const npx::PlanHost& h,
-                         const npx::DevTables* t, const Shape& s, double S, int iw) {
+// Per-job fields shared by every entry point: stack, tables of one (H, W, M), launch shape.
+npx::FusedArgs job_args(const void* d_in, int dtype, int N, int H, int W, int M, const npx::PlanHost& h,
+                        const npx::DevTables* t, const Shape& s, int iw) {
     npx::FusedArgs a{};
     a.in = d_in;
     a.u16 = dtype == NPX_DTYPE_U16;
@@ -224,15 +227,141 @@ npx::FusedArgs base_args(const void* d_in, int dtype, int N, int H, int W, int M
     a.M = M;
     a.HM = h.HM;
     a.WM = h.WM;
-    a.margin = h.margin;
+    a.f0 = 0;
     a.nphase = h.nphase;
     a.nwaves = s.nwaves;
     a.ngroups = s.ngroups;
-    a.pow_mode = pow_mode_of(S);
-    a.S = static_cast<float>(S);
+    a.chunk0 = 0;
+    a.direct = 1;
     a.iw = iw ? 1 : 0;
-    a.t = *t;
+    a.lrw = t->lrw;
+    a.lrb = t->lrb;
+    a.lcw = t->lcw;
+    a.lcb = t->lcb;
+    a.crw = t->crw;
+    a.crb = t->crb;
+    a.ccw = t->ccw;
+    a.ccb = t->ccb;
+    a.nrad = 0;
+    a.ns = 0;
     return a;
+}
+
+void add_radius(npx::FusedArgs& a, const npx::PlanHost& h, const npx::DevTables* t) {
+    const int r = a.nrad++;
+    a.ring[r] = t->ring;
+    a.K[r] = h.K;
+    a.rs[r] = h.ring_stride;
+    a.margin[r] = h.margin;
+    a.KL = std::max(a.KL, h.K);
+}
+
+void add_sensitivity(npx::FusedArgs& a, double S) {
+    const int s = a.ns++;
+    a.sens[s] = static_cast<float>(S);
+    a.sens_pow[s] = pow_mode_of(S);
+}
+
+npx::FusedArgs base_args(const void* d_in, int dtype, int N, int H, int W, int M, const npx::PlanHost& h,
+                         const npx::DevTables* t, const Shape& s, double S, int iw) {
+    npx::FusedArgs a = job_args(d_in, dtype, N, H, W, M, h, t, s, iw);
+    a.KL = h.K;
+    add_radius(a, h, t);
+    add_sensitivity(a, S);
+    return a;
+}
+
+// Plan + device tables under the cache lock; NPX_OK or an error code with g_err set.
+// Under stream capture the tables must already be on the device (their upload blocks).
+int plan_and_tables(int H, int W, int M, double R, Plan** p, const npx::DevTables** t, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    *p = get_plan(H, W, M, R);
+    if (!*p) return NPX_ERR_UNSUPPORTED;
+    int dev = 0;
+    if (capturing(st) && (hipGetDevice(&dev) != hipSuccess || (*p)->dev.find(dev) == (*p)->dev.end()))
+        return fail(NPX_ERR_ARG, "geometry tables are uploaded by the first call: make one call with these "
+                                 "parameters outside stream capture first");
+    return device_tables(*p, t);
+}
+
+bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// ------------------------------------------------------------------------------------------ parameter sweep
+
+// One job of a sweep launch: magnification mi with radii [r0, r0 + nr) and sensitivities [s0, s0 + nsn).
+struct SweepJob {
+    int mi, r0, nr, s0, nsn;
+    int M, nregx, nregy, ngroups, kmax;
+    size_t P;
+    size_t ws_off;
+};
+
+struct SweepLayout {
+    std::vector<SweepJob> jobs;  // heaviest first
+    int nchunk = 1, fpc = 1;
+    size_t ws_bytes = 0;
+};
+
+// Job list, frame chunking and workspace of a sweep; deterministic in its arguments (the workspace query and
+// the sweep itself agree). chunks = false: one chunk, no workspace.
+int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, const double* radii, int n_sens,
+                 bool chunks, SweepLayout* L) {
+    if (n_mag < 1 || n_rad < 1 || n_sens < 1 || !mags || !radii) return fail(NPX_ERR_ARG, "empty sweep");
+    if (N < 1 || H < 1 || W < 1) return fail(NPX_ERR_ARG, "stack must have N, H, W >= 1");
+    L->jobs.clear();
+    const int nw = npx::kBatchThreads / 64;
+    for (int mi = 0; mi < n_mag; ++mi) {
+        const int M = mags[mi];
+        if (M < 1 || M > 64) return fail(NPX_ERR_ARG, "magnification must be in [1, 64]");
+        for (int r0 = 0; r0 < n_rad; r0 += npx::kMaxRad) {
+            const int nr = std::min(npx::kMaxRad, n_rad - r0);
+            int kmax = 0;
+            for (int r = r0; r < r0 + nr; ++r) {
+                std::lock_guard<std::mutex> lk(g_mu);
+                Plan* p = get_plan(H, W, M, radii[r]);
+                if (!p) return NPX_ERR_UNSUPPORTED;
+                kmax = std::max(kmax, p->host.K);
+            }
+            for (int s0 = 0; s0 < n_sens; s0 += npx::kMaxSens) {
+                SweepJob j{};
+                j.mi = mi;
+                j.r0 = r0;
+                j.nr = nr;
+                j.s0 = s0;
+                j.nsn = std::min(npx::kMaxSens, n_sens - s0);
+                j.M = M;
+                j.nregx = (W + 63) / 64;
+                j.nregy = (H + npx::kQTemporal - 1) / npx::kQTemporal;
+                j.ngroups = (M * M * nr + nw - 1) / nw;
+                j.kmax = kmax;
+                j.P = static_cast<size_t>(H) * M * static_cast<size_t>(W) * M;
+                L->jobs.push_back(j);
+            }
+        }
+    }
+    std::stable_sort(L->jobs.begin(), L->jobs.end(), [](const SweepJob& a, const SweepJob& b) {
+        return static_cast<long long>(a.M) * a.M * a.nr > static_cast<long long>(b.M) * b.M * b.nr;
+    });
+    long long spatial = 0;
+    for (const auto& j : L->jobs) spatial += static_cast<long long>(j.nregx) * j.nregy * j.ngroups;
+    long long want = 1;
+    if (chunks) {
+        want = (g_target_blocks + spatial - 1) / spatial;
+        want = std::max<long long>(1, std::min<long long>(want, N / 32));
+    }
+    L->fpc = static_cast<int>((N + want - 1) / want);
+    L->nchunk = (N + L->fpc - 1) / L->fpc;
+    L->ws_bytes = 0;
+    if (L->nchunk > 1) {
+        for (auto& j : L->jobs) {
+            j.ws_off = L->ws_bytes;
+            L->ws_bytes += align256(static_cast<size_t>(L->nchunk) * j.nr * j.nsn * j.P * sizeof(float));
+        }
+    }
+    return NPX_OK;
 }
 
 }  // namespace
@@ -261,14 +390,9 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     if (rc) return rc;
     if (!d_out) return fail(NPX_ERR_ARG, "output pointer is NULL");
     const npx::DevTables* t = nullptr;
-    Plan* p;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        p = get_plan(H, W, M, R);
-        if (!p) return NPX_ERR_UNSUPPORTED;
-        rc = device_tables(p, &t);
-        if (rc) return rc;
-    }
+    Plan* p = nullptr;
+    rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
     Shape s = fused_shape(H, W, M);
     s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
@@ -282,7 +406,7 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     const hipStream_t st = static_cast<hipStream_t>(stream);
     CallEvents ev{};
     const bool timed = timing_begin(st, &ev);
-    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, npx::kModeFrames, st);
+    hipError_t e = npx::launch_fused(a, grid, block, npx::kModeFrames, st);
     if (e != hipSuccess) return hip_fail(e, "esrrf_fused(frames) launch");
     if (timed && hipEventRecord(ev.e1, st) == hipSuccess && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
     return NPX_OK;
@@ -307,14 +431,9 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
     if (((mode_mask & NPX_AVG) && !d_avg) || ((mode_mask & NPX_VAR) && !d_var) || ((mode_mask & NPX_TAC2) && !d_tac2))
         return fail(NPX_ERR_ARG, "output pointer of a requested temporal mode is NULL");
     const npx::DevTables* t = nullptr;
-    Plan* p;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        p = get_plan(H, W, M, R);
-        if (!p) return NPX_ERR_UNSUPPORTED;
-        rc = device_tables(p, &t);
-        if (rc) return rc;
-    }
+    Plan* p = nullptr;
+    rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
     const Shape s = fused_shape(H, W, M);
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
     int nchunk = 1, fpc = N;
@@ -325,6 +444,7 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
     }
     a.fpc = fpc;
     a.nchunk = nchunk;
+    a.direct = nchunk == 1;
     a.avg = (mode_mask & NPX_AVG) ? d_avg : nullptr;
     a.var = (mode_mask & NPX_VAR) ? d_var : nullptr;
     a.tac2 = (mode_mask & NPX_TAC2) ? d_tac2 : nullptr;
@@ -335,7 +455,7 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
     CallEvents ev{};
     const bool timed = timing_begin(st, &ev);
     const int mode = a.var ? npx::kModeTemporalVar : npx::kModeTemporal;
-    hipError_t e = npx::launch_fused(a, p->host.K, grid, block, mode, st);
+    hipError_t e = npx::launch_fused(a, grid, block, mode, st);
     if (e != hipSuccess) return hip_fail(e, "esrrf_fused(temporal) launch");
     if (timed) (void)hipEventRecord(ev.e1, st);
     if (nchunk > 1) {
@@ -344,6 +464,136 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
         if (e != hipSuccess) return hip_fail(e, "moments_merge launch");
     }
     if (timed && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
+    return NPX_OK;
+}
+
+size_t npx_esrrf_sweep_workspace(int N, int H, int W, int n_mag, const int* mags, int n_rad, const double* radii,
+                                 int n_sens) {
+    SweepLayout L;
+    if (sweep_layout(N, H, W, n_mag, mags, n_rad, radii, n_sens, true, &L) != NPX_OK) return 0;
+    return L.ws_bytes;
+}
+
+int npx_esrrf_sweep(const void* d_in, int dtype, int N, int H, int W, int n_mag, const int* mags, int n_rad,
+                    const double* radii, int n_sens, const double* sens, int iw, float* const* d_avg, void* d_work,
+                    size_t work_bytes, void* stream) {
+    if (!sens || !d_avg) return fail(NPX_ERR_ARG, "NULL sensitivity or output array");
+    for (int mi = 0; mi < n_mag && mags; ++mi) {
+        const int rc = check_common(d_in, dtype, N, H, W, mags[mi]);
+        if (rc) return rc;
+    }
+    SweepLayout L;
+    int rc = sweep_layout(N, H, W, n_mag, mags, n_rad, radii, n_sens, d_work != nullptr, &L);
+    if (rc) return rc;
+    if (L.ws_bytes > 0 && work_bytes < L.ws_bytes)
+        return fail(NPX_ERR_WORKSPACE, "workspace smaller than npx_esrrf_sweep_workspace()");
+    for (int k = 0; k < n_mag * n_rad * n_sens; ++k)
+        if (!d_avg[k]) return fail(NPX_ERR_ARG, "NULL output map in the sweep output array");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nw = npx::kBatchThreads / 64;
+    CallEvents ev{};
+    const bool timed = timing_begin(st, &ev);
+    std::vector<std::vector<float*>> merge_outs;
+    for (size_t j0 = 0; j0 < L.jobs.size(); j0 += npx::kMaxJobs) {
+        const size_t nj = std::min<size_t>(npx::kMaxJobs, L.jobs.size() - j0);
+        int kmax = 0;
+        for (size_t j = j0; j < j0 + nj; ++j) kmax = std::max(kmax, L.jobs[j].kmax);
+        const int KL = npx::batched_lattice_k(kmax);
+        npx::BatchArgs b{};
+        b.njobs = static_cast<int>(nj);
+        int blocks = 0;
+        for (size_t jj = 0; jj < nj; ++jj) {
+            const SweepJob& sj = L.jobs[j0 + jj];
+            Shape sh{nw, sj.ngroups, sj.nregx, sj.nregy};
+            npx::FusedArgs a{};
+            for (int r = 0; r < sj.nr; ++r) {
+                Plan* p = nullptr;
+                const npx::DevTables* t = nullptr;
+                rc = plan_and_tables(H, W, sj.M, radii[sj.r0 + r], &p, &t, st);
+                if (rc) return rc;
+                if (r == 0) a = job_args(d_in, dtype, N, H, W, sj.M, p->host, t, sh, iw);
+                add_radius(a, p->host, t);
+            }
+            for (int s = 0; s < sj.nsn; ++s) add_sensitivity(a, sens[sj.s0 + s]);
+            a.KL = KL;
+            a.fpc = L.fpc;
+            a.nchunk = L.nchunk;
+            a.direct = L.nchunk == 1;
+            a.part = L.nchunk > 1 ? reinterpret_cast<float*>(static_cast<uint8_t*>(d_work) + sj.ws_off) : nullptr;
+            std::vector<float*> outs;
+            for (int r = 0; r < sj.nr; ++r)
+                for (int s = 0; s < sj.nsn; ++s) {
+                    float* o = d_avg[(static_cast<size_t>(sj.mi) * n_rad + sj.r0 + r) * n_sens + sj.s0 + s];
+                    a.sweep_avg[r * npx::kMaxSens + s] = o;
+                    outs.push_back(o);
+                }
+            merge_outs.push_back(outs);
+            b.job[jj] = a;
+            b.nregx[jj] = sj.nregx;
+            b.nregy[jj] = sj.nregy;
+            b.start[jj] = blocks;
+            blocks += sj.nregx * sj.nregy * sj.ngroups * L.nchunk;
+        }
+        b.start[nj] = blocks;
+        hipError_t e = npx::launch_batched(b, npx::kModeSweep, st);
+        if (e != hipSuccess) return hip_fail(e, "esrrf_batched(sweep) launch");
+    }
+    if (timed) (void)hipEventRecord(ev.e1, st);
+    if (L.nchunk > 1) {
+        for (size_t j = 0; j < L.jobs.size(); ++j) {
+            const SweepJob& sj = L.jobs[j];
+            const float* part = reinterpret_cast<const float*>(static_cast<uint8_t*>(d_work) + sj.ws_off);
+            hipError_t e = npx::launch_means_merge(part, L.nchunk, L.fpc, N, sj.P, sj.nr * sj.nsn,
+                                                   merge_outs[j].data(), st);
+            if (e != hipSuccess) return hip_fail(e, "means_merge launch");
+        }
+    }
+    if (timed && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
+    return NPX_OK;
+}
+
+// ------------------------------------------------------------------------------------------ graphs
+
+int npx_capture_begin(void** capture_stream) {
+    if (!capture_stream) return fail(NPX_ERR_ARG, "NULL stream output");
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    *capture_stream = s;
+    return NPX_OK;
+}
+
+int npx_capture_end(void* capture_stream, void** graph) {
+    if (!capture_stream || !graph) return fail(NPX_ERR_ARG, "NULL stream or graph output");
+    hipStream_t s = static_cast<hipStream_t>(capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph = x;
+    return NPX_OK;
+}
+
+int npx_graph_launch(void* graph, void* stream) {
+    if (!graph) return fail(NPX_ERR_ARG, "NULL graph");
+    hipError_t e = hipGraphLaunch(static_cast<hipGraphExec_t>(graph), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return NPX_OK;
+}
+
+int npx_graph_destroy(void* graph) {
+    if (!graph) return NPX_OK;
+    hipError_t e = hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph));
+    if (e != hipSuccess) return hip_fail(e, "hipGraphExecDestroy");
     return NPX_OK;
 }
 

@@ -1,0 +1,509 @@
+// Device body of the fused eSRRF kernels (included by npx_kernels.hip and npx_batched.hip).
+//
+// A block owns a 64-column x Q-row region of the input image and up to 16 wave slots; a wave slot is one
+// (radius, output phase) pair, one lane per input column, Q rows per lane. Every lane of a wavefront shares
+// the phase and the radius, so the ring geometry (A, B, w per window offset) is wave-uniform and is read
+// through scalar loads. Per frame the block stages the input tile + halo, the Roberts gradients and the
+// normalised gradient lattice in LDS (the lattice is stored in two column-parity planes so that the 64 lanes
+// of a read touch 64 consecutive float2). Ring sums accumulate in registers.
+//
+// Reference semantics: src/include/_c_sr_radial_gradient_convergence.c:17-90 (ring sum),
+// src/include/_c_interpolation_catmull_rom.c:6-52 (taps), src/include/_c_gradients.c:53-72 (Roberts),
+// src/nanopyx/core/transform/_le_esrrf.pyx:172-195 (composition),
+// src/nanopyx/core/transform/sr_temporal_correlations.py:46-75,205-221 (temporal moments),
+// src/nanopyx/core/analysis/parameter_sweep.py:33-37 (sweep: one eSRRF per (sensitivity, radius)).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "npx_kernels.h"
+
+namespace npx {
+
+using cfloat = const __attribute__((address_space(4))) float;  // constant address space: scalar loads
+
+__device__ __forceinline__ float apply_pow(float v, int pow_mode, float S) {
+    // reference: if RGC >= 0 && S > 1: RGC = pow(RGC, S) (:83-87); RGC here is never negative. Integer
+    // sensitivities are exact products; others use 2^(S log2 v) on the hardware log2/exp2 (a few ulp).
+    if (pow_mode == 2) return v * v;
+    if (pow_mode == 3) return v * v * v;
+    if (pow_mode == 4) {
+        const float s = v * v;
+        return s * s;
+    }
+    if (pow_mode == 1) return v > 0.f ? __builtin_amdgcn_exp2f(S * __builtin_amdgcn_logf(v)) : 0.f;
+    return v;
+}
+
+// One step of the streaming moments. State: mu (mean), m2 (sum of squared deviations), c (lag-1
+// co-moment sum_t (x_t - mu)(x_{t+1} - mu)), first and prev values; n = values already absorbed.
+template <bool WITH_M2 = true>
+__device__ __forceinline__ void moments_push(float v, int n, float inv_np1, float& mu, float& m2, float& c,
+                                             float& first, float& prev) {
+    if (n == 0) {
+        mu = v;
+        if (WITH_M2) m2 = 0.f;
+        c = 0.f;
+        first = v;
+        prev = v;
+        return;
+    }
+    const float delta = v - mu;
+    const float mun = mu + delta * inv_np1;
+    const float dmu = mun - mu;
+    c = c + dmu * ((prev - mu) + (first - mu)) + static_cast<float>(n - 1) * dmu * dmu + (prev - mun) * (v - mun);
+    if (WITH_M2) m2 = m2 + delta * (v - mun);
+    mu = mun;
+    prev = v;
+}
+
+__device__ __forceinline__ float load_px(const FusedArgs& a, int f, int gy, int gx) {
+    if (gy < 0 || gy >= a.H || gx < 0 || gx >= a.W) return 0.f;
+    const size_t o = (static_cast<size_t>(f) * a.H + gy) * a.W + gx;
+    return a.u16 ? static_cast<float>(static_cast<const uint16_t*>(a.in)[o]) : static_cast<const float*>(a.in)[o];
+}
+
+// LDS carving of the fused kernels (one dynamic buffer; byte offsets, 16-byte aligned segments).
+template <int K, int Q>
+struct Lds {
+    using G = Geometry<K, Q>;
+    static constexpr int al(int b) { return (b + 15) / 16 * 16; }
+    static constexpr int IN = 0;                                 // float [TIR*TIC]
+    static constexpr int RX = IN + al(4 * G::TIR * G::TIC);      // float [LR*TGC]
+    static constexpr int RY = RX + al(4 * G::LR * G::TGC);       // float [LR*TGC]
+    static constexpr int LAT = RY + al(4 * G::LR * G::TGC);      // float2 [2*LR*LP]
+    static constexpr int LRW = LAT + al(8 * 2 * G::LR * G::LP);  // float4 [LR]
+    static constexpr int LRB = LRW + al(16 * G::LR);             // int [LR]
+    static constexpr int LCW = LRB + al(4 * G::LR);              // float4 [LB]
+    static constexpr int LCB = LCW + al(16 * G::LB);             // int [LB]
+    static constexpr int CRW = LCB + al(4 * G::LB);              // float4 [16*Q]
+    static constexpr int CRB = CRW + al(16 * 16 * Q);            // int [16*Q]
+    static constexpr int RING = CRB + al(4 * 16 * Q);            // float [nwaves*RN], then per-pixel state
+    static_assert(2 * G::TGR * G::TGC <= 4 * G::LR * G::LP, "gradient tile must fit in the lattice storage");
+};
+
+template <int K, int Q>
+inline size_t lds_bytes_kq(int nwaves, int mode) {
+    using L = Lds<K, Q>;
+    size_t b = L::RING + static_cast<size_t>(nwaves) * Geometry<K, Q>::RN * 4;
+    if (mode == kModeTemporal || mode == kModeTemporalVar) b += 2 * static_cast<size_t>(nwaves) * Q * 64 * 4;
+    return b;
+}
+
+// Ring sums of one wave (radius half-width K) over a lattice built for half-width KL >= K. Sample (j, i) of
+// pixel q reads lattice row 2q + ey + j + (KL - K), column 2*lane + ex + i + (KL - K). With u = g.(dx, dy)
+// and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's contribution
+// [G.d < 0] * w * (1 - |G x d| / (|G| d)) is t * med3(0, -u, d - |v|). g.x*dx and g.y*dx are formed once per
+// lattice point of the column and shared by the samples that read it.
+template <int K, int KL, int Q>
+__device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
+                                         cfloat* rg) {
+    using GL = Geometry<KL, Q>;
+    constexpr int NW = 2 * K + 1;
+    constexpr int NR = 2 * Q + 2 * K - 1;
+    constexpr int LP = GL::LP, LPL = GL::LR * GL::LP;
+    constexpr int DK = KL - K;
+    constexpr int RDX = NW, RDT = 2 * NW;
+    static_assert(K <= KL, "ring wider than the lattice");
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0.f;
+    float dyv[NW];
+#pragma unroll
+    for (int jj = 0; jj < NW; ++jj) dyv[jj] = rg[jj];
+#pragma unroll 1
+    for (int i = 0; i < NW; ++i) {
+        const int c = ex + i + DK;
+        const float2* col = s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
+        const float dxi = rg[RDX + i];
+        cfloat* dt = rg + RDT + 2 * i * NW;
+        float dv[NW], tv[NW];
+#pragma unroll
+        for (int jj = 0; jj < NW; ++jj) {
+            dv[jj] = dt[2 * jj];
+            tv[jj] = dt[2 * jj + 1];
+        }
+        // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
+        float2 g1 = col[0];
+        float2 g2 = col[LP];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const float2 g = g1;
+            g1 = g2;
+            if (r + 2 < NR) g2 = col[(r + 2) * LP];
+            const float px_ = g.x * dxi;
+            const float py_ = g.y * dxi;
+#pragma unroll
+            for (int jj = 0; jj < NW; ++jj) {
+                const int rq = r - jj;
+                if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
+                const int q = rq >> 1;
+                const float u = fmaf(g.y, dyv[jj], px_);
+                const float v = fmaf(-g.x, dyv[jj], py_);
+                const float m = __builtin_amdgcn_fmed3f(0.f, -u, dv[jj] - fabsf(v));
+                acc[q] = fmaf(tv[jj], m, acc[q]);
+            }
+        }
+    }
+}
+
+// Dispatch on the wave's (uniform) half-width for lattices of half-width KL.
+template <int KL, int Q>
+__device__ __forceinline__ void ring_sum_any(int K, float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
+                                             cfloat* rg) {
+    switch (K) {
+#define NPX_RING_CASE(k)                                                             \
+    case k:                                                                          \
+        if constexpr (k <= KL) ring_sum<(k <= KL ? k : KL), KL, Q>(acc, s_lat, ey, ex, lane, rg); \
+        break;
+        NPX_RING_CASE(0)
+        NPX_RING_CASE(1)
+        NPX_RING_CASE(2)
+        NPX_RING_CASE(3)
+        NPX_RING_CASE(4)
+        NPX_RING_CASE(5)
+        NPX_RING_CASE(6)
+#undef NPX_RING_CASE
+        default: break;
+    }
+}
+
+// One block of the fused computation: region (bx, by) of 64 input columns x Q input rows, wave-slot group
+// `group`, frames of chunk `chunk`. Steps per frame: (1) input tile + halo, (2) Roberts cross, (3a/3b)
+// Catmull-Rom x2M lattice along rows then columns + normalisation, (4) ring sums, (5) normalise /
+// sensitivity / intensity weight and store (frames) or fold into the moments (temporal) or the per-
+// sensitivity means (sweep). VARK: waves may have ring half-widths K < KL (batched launches); otherwise
+// every wave has K = KL, fixed at compile time.
+template <int KL, int Q, int MODE, bool VARK>
+__device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, int group, int chunk,
+                                            unsigned char* smem) {
+    using G = Geometry<KL, Q>;
+    using L = Lds<KL, Q>;
+    constexpr int RNL = G::RN;
+    constexpr int HG = G::HG, HI = G::HI;
+    constexpr int TIR = G::TIR, TIC = G::TIC;
+    constexpr int TGR = G::TGR, TGC = G::TGC;
+    constexpr int LR = G::LR, LP = G::LP, LPL = LR * LP, LB = G::LB;
+    constexpr bool TEMPORAL = MODE == kModeTemporal || MODE == kModeTemporalVar;
+    constexpr bool SWEEP = MODE == kModeSweep;
+    constexpr bool WITH_M2 = MODE == kModeTemporalVar;
+    constexpr int NSM = SWEEP ? kMaxSens : 1;
+
+    float* const s_in = reinterpret_cast<float*>(smem + L::IN);
+    float* const s_rx = reinterpret_cast<float*>(smem + L::RX);
+    float* const s_ry = reinterpret_cast<float*>(smem + L::RY);
+    float2* const s_lat = reinterpret_cast<float2*>(smem + L::LAT);
+    float* const s_gc = reinterpret_cast<float*>(s_lat);  // Roberts gradients share the lattice storage
+    float* const s_gr = s_gc + TGR * TGC;
+    float4* const s_lrw = reinterpret_cast<float4*>(smem + L::LRW);
+    int* const s_lrb = reinterpret_cast<int*>(smem + L::LRB);
+    float4* const s_lcw = reinterpret_cast<float4*>(smem + L::LCW);
+    int* const s_lcb = reinterpret_cast<int*>(smem + L::LCB);
+    float4* const s_crw = reinterpret_cast<float4*>(smem + L::CRW);
+    int* const s_crb = reinterpret_cast<int*>(smem + L::CRB);
+    float* const s_ring = reinterpret_cast<float*>(smem + L::RING);
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int x0 = bx * 64;
+    const int y0 = by * Q;
+    const int f_begin = a.f0 + chunk * a.fpc;
+    const int f_end = min(a.N, f_begin + a.fpc);
+    const int slot = __builtin_amdgcn_readfirstlane(group * a.nwaves + wave);
+    const int rad = slot / a.nphase;
+    const int ph = slot - rad * a.nphase;
+    const bool active = rad < a.nrad;
+    const int M = a.M;
+    const int py = active ? ph / M : 0;
+    const int px = active ? ph % M : 0;
+    const int ey = (2 * py + 1) / M;
+    const int ex = (2 * px + 1) / M;
+    const int xi = x0 + lane;
+    const int cM = M * xi + px;
+    const int a_lo = 2 * y0 - KL;
+    const int b_lo = 2 * x0 - KL;
+    const int Kw = VARK ? (active ? a.K[rad] : KL) : KL;
+    const int NWw = 2 * Kw + 1;
+    const int margin = active ? a.margin[rad] : 0;
+
+    // ---- stage the frame-invariant tables. Per wave: the normaliser part of its ring block (wsum, then the
+    // (NW+1)^2 prefix sums of w); the ring constants themselves are read through scalar loads.
+    for (int k = tid; k < a.nwaves * RNL; k += nthr) {
+        const int w = k / RNL, e = k - w * RNL;
+        const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
+        float v = 0.f;
+        if (rw < a.nrad) {
+            const int nw = 2 * a.K[rw] + 1;
+            if (e < 1 + (nw + 1) * (nw + 1))
+                v = a.ring[rw][static_cast<size_t>(pw) * a.rs[rw] + 2 * nw + 3 * nw * nw + e];
+        }
+        s_ring[k] = v;
+    }
+    for (int la = tid; la < LR; la += nthr) {
+        const int aa = a_lo + la;
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        int b = 0;
+        if (aa >= 1 && aa <= 2 * a.HM - 2) {  // rows a >= 2H read the clamped last row (:45-52)
+            const int ai = min(aa, 2 * a.H);
+            w = a.lrw[ai];
+            b = a.lrb[ai] - (y0 - HG);
+        }
+        s_lrw[la] = w;
+        s_lrb[la] = b;
+    }
+    for (int lb = tid; lb < LB; lb += nthr) {
+        const int bb = b_lo + lb;
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        int b = 0;
+        if (bb >= 1 && bb <= 2 * a.WM - 2) {
+            const int bi = min(bb, 2 * a.W);
+            w = a.lcw[bi];
+            b = a.lcb[bi] - (x0 - HG);
+        }
+        s_lcw[lb] = w;
+        s_lcb[lb] = b;
+    }
+    for (int k = tid; k < a.nwaves * Q; k += nthr) {
+        const int w = k / Q, q = k - w * Q;
+        const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
+        const int rM = min(M * (y0 + q) + (rw < a.nrad ? pw / M : 0), a.HM - 1);
+        s_crw[k] = a.crw[rM];
+        s_crb[k] = a.crb[rM] - (y0 - HI);
+    }
+    const int cMs = min(cM, a.WM - 1);
+    const float4 wcol = a.ccw[cMs];
+    const int ccol = a.ccb[cMs] - (x0 - HI);
+
+    // per-pixel state: temporal moments (mean, lag-1 co-moment, M2 if VAR) or one mean per sensitivity
+    float mu[SWEEP ? NSM * Q : Q], m2[WITH_M2 ? Q : 1], cc[TEMPORAL ? Q : 1];
+#pragma unroll
+    for (int q = 0; q < (SWEEP ? NSM * Q : Q); ++q) mu[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < (WITH_M2 ? Q : 1); ++q) m2[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < (TEMPORAL ? Q : 1); ++q) cc[q] = 0.f;
+    // first and previous frame's value of every pixel of the wave: [wave][q][lane] and [nwaves + wave][q][lane]
+    float* const s_first = s_ring + a.nwaves * RNL + (wave * Q) * 64 + lane;
+    float* const s_prev = s_first + a.nwaves * Q * 64;
+
+    // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
+    float pf[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+        const int idx = tid + k * nthr;
+        pf[k] = 0.f;
+        if (idx < TIR * TIC && f_begin < f_end) {
+            const int r = idx / TIC, c = idx - r * TIC;
+            pf[k] = load_px(a, f_begin, y0 - HI + r, x0 - HI + c);
+        }
+    }
+    const float* ringL = s_ring + wave * RNL;
+    cfloat* rg = (cfloat*)(a.ring[active ? rad : 0] + static_cast<size_t>(ph) * a.rs[active ? rad : 0]);
+
+    for (int f = f_begin; f < f_end; ++f) {
+        // (1) input tile + halo of frame f (zero outside the image; uint16 converted as a plain cast)
+#pragma unroll
+        for (int k = 0; k < kPrefetch; ++k) {
+            const int idx = tid + k * nthr;
+            if (idx < TIR * TIC) s_in[idx] = pf[k];
+        }
+        for (int idx = tid + kPrefetch * nthr; idx < TIR * TIC; idx += nthr) {
+            const int r = idx / TIC, c = idx - r * TIC;
+            s_in[idx] = load_px(a, f, y0 - HI + r, x0 - HI + c);
+        }
+        __syncthreads();
+        // prefetch frame f + 1 while this frame is processed (consumed at the top of the next iteration)
+        if (f + 1 < f_end) {
+#pragma unroll
+            for (int k = 0; k < kPrefetch; ++k) {
+                const int idx = tid + k * nthr;
+                if (idx < TIR * TIC) {
+                    const int r = idx / TIC, c = idx - r * TIC;
+                    pf[k] = load_px(a, f + 1, y0 - HI + r, x0 - HI + c);
+                }
+            }
+        }
+        // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
+        for (int idx = tid; idx < TGR * TGC; idx += nthr) {
+            const int r = idx / TGC, c = idx - r * TGC;
+            const int gy = y0 - HG + r, gx = x0 - HG + c;
+            float gcv = 0.f, grv = 0.f;
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                const int r1 = gy - (y0 - HI), c1 = gx - (x0 - HI);
+                const int r0 = (gy > 0 ? gy - 1 : 0) - (y0 - HI);
+                const int c0 = (gx > 0 ? gx - 1 : 0) - (x0 - HI);
+                const float a_r0c1 = s_in[r0 * TIC + c1];
+                const float a_r1c0 = s_in[r1 * TIC + c0];
+                const float a_r0c0 = s_in[r0 * TIC + c0];
+                const float a_r1c1 = s_in[r1 * TIC + c1];
+                gcv = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0;
+                grv = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0;
+            }
+            s_gc[idx] = gcv;
+            s_gr[idx] = grv;
+        }
+        __syncthreads();
+        // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
+        for (int idx = tid; idx < LR * TGC; idx += nthr) {
+            const int la = idx / TGC, c = idx - la * TGC;
+            const float4 wr = s_lrw[la];
+            const int rb = s_lrb[la];
+            const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int tr = min(max(rb + u, 0), TGR - 1);
+                sx += s_gc[tr * TGC + c] * wrv[u];
+                sy += s_gr[tr * TGC + c] * wrv[u];
+            }
+            s_rx[idx] = sx;
+            s_ry[idx] = sy;
+        }
+        __syncthreads();
+        // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
+        // column-parity planes so the 64 lanes of one read touch 64 consecutive float2
+        for (int idx = tid; idx < 2 * LPL; idx += nthr) {
+            const int plane = idx / LPL;
+            const int rem = idx - plane * LPL;
+            const int la = rem / LP;
+            const int pos = rem - la * LP;
+            const int lb = 2 * pos + plane;
+            const float4 wc = s_lcw[lb];
+            const int cb = s_lcb[lb];
+            const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
+            float gx = 0.f, gy = 0.f;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int tc = min(max(cb + v, 0), TGC - 1);
+                gx += s_rx[la * TGC + tc] * wcv[v];
+                gy += s_ry[la * TGC + tc] * wcv[v];
+            }
+            const float n2 = gx * gx + gy * gy;
+            float2 g = make_float2(0.f, 0.f);
+            if (n2 > 0.f) {
+                const float inv = rsqrtf(n2);
+                g = make_float2(gx * inv, gy * inv);
+            }
+            s_lat[idx] = g;
+        }
+        __syncthreads();
+
+        if (active) {
+            // (4) ring sums
+            float acc[Q];
+            if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, rg);
+            else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, rg);
+            // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Branch-free per
+            // pixel: pixels whose window crosses the border take their normaliser from the prefix-sum table.
+            const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
+            const float inv_np1 = 1.0f / static_cast<float>(f - f_begin + 1);
+            const float wsum = ringL[0];
+            const float* P = ringL + 1;
+            const int b0 = 2 * xi + ex;
+            const int i0 = max(0, 1 - (b0 - Kw));
+            const int i1 = max(0, min(NWw, 2 * a.WM - 2 - (b0 - Kw) + 1));  // lanes past the image: empty range
+            const bool col_ok = xi < a.W && cM >= margin && cM < a.WM - margin;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int yi = y0 + q;
+                const int rM = M * yi + py;
+                const int a0 = 2 * yi + ey;
+                const int j0 = max(0, 1 - (a0 - Kw));
+                const int j1 = max(0, min(NWw, 2 * a.HM - 2 - (a0 - Kw) + 1));
+                const bool row_ok = yi < a.H && rM >= margin && rM < a.HM - margin;  // uniform
+                float den = wsum;
+                if (j0 > 0 || j1 < NWw || i0 > 0 || i1 < NWw) {
+                    const int jl = min(j0, j1), il = min(i0, i1);
+                    den = (P[j1 * (NWw + 1) + i1] - P[jl * (NWw + 1) + i1]) -
+                          (P[j1 * (NWw + 1) + il] - P[jl * (NWw + 1) + il]);
+                }
+                const float rgc0 = den > 0.f ? acc[q] / den : 0.f;
+                float inten = 1.f;
+                if (a.iw) {
+                    const float4 wrow = s_crw[wave * Q + q];
+                    const int crow = s_crb[wave * Q + q];
+                    const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
+                    float sacc = 0.f;
+#pragma unroll
+                    for (int v2 = 0; v2 < 4; ++v2) {
+                        const int tc = min(max(ccol + v2, 0), TIC - 1);
+                        float pp = 0.f;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int tr = min(max(crow + u, 0), TIR - 1);
+                            pp += s_in[tr * TIC + tc] * wrv[u];
+                        }
+                        sacc += pp * wcv[v2];
+                    }
+                    inten = sacc;
+                }
+                const bool ok = row_ok && col_ok;
+                if constexpr (SWEEP) {
+#pragma unroll
+                    for (int s = 0; s < NSM; ++s) {
+                        if (s < a.ns) {
+                            float val = apply_pow(rgc0, a.sens_pow[s], a.sens[s]);
+                            if (a.iw) val = val * inten;
+                            val = ok ? val : 0.f;
+                            mu[s * Q + q] += (val - mu[s * Q + q]) * inv_np1;
+                        }
+                    }
+                } else {
+                    float val = apply_pow(rgc0, a.sens_pow[0], a.sens[0]);
+                    if (a.iw) val = val * inten;
+                    val = ok ? val : 0.f;
+                    if constexpr (MODE == kModeFrames) {
+                        if (xi < a.W && yi < a.H) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
+                    } else {
+                        float fq = val, pq = val;
+                        if (f == f_begin) {
+                            s_first[q * 64] = val;
+                        } else {
+                            fq = s_first[q * 64];
+                            pq = s_prev[q * 64];
+                        }
+                        moments_push<WITH_M2>(val, f - f_begin, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
+                        s_prev[q * 64] = val;
+                    }
+                }
+                // one pixel at a time: keeps the finalise's live registers low (no interleaving across q)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();
+    }
+
+    if (MODE != kModeFrames && active && f_end > f_begin) {
+        const size_t P = static_cast<size_t>(a.HM) * a.WM;
+        const int pslot = a.chunk0 + chunk;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int yi = y0 + q;
+            if (xi >= a.W || yi >= a.H) continue;
+            const size_t pix = static_cast<size_t>(M * yi + py) * a.WM + cM;
+            if constexpr (SWEEP) {
+#pragma unroll
+                for (int s = 0; s < NSM; ++s) {
+                    if (s >= a.ns) continue;
+                    if (a.direct) a.sweep_avg[rad * kMaxSens + s][pix] = mu[s * Q + q];
+                    else a.part[((static_cast<size_t>(pslot) * a.nrad + rad) * a.ns + s) * P + pix] = mu[s * Q + q];
+                }
+            } else if (a.direct) {
+                const float n = static_cast<float>(f_end - f_begin);
+                if (a.avg) a.avg[pix] = mu[q];
+                if (a.var) a.var[pix] = WITH_M2 ? m2[WITH_M2 ? q : 0] / n : 0.f;
+                if (a.tac2) a.tac2[pix] = cc[q] / (n - 1.0f);
+            } else {
+                float* base = a.part + static_cast<size_t>(pslot) * 5 * P + pix;
+                base[0] = mu[q];
+                base[P] = WITH_M2 ? m2[WITH_M2 ? q : 0] : 0.f;
+                base[2 * P] = cc[q];
+                base[3 * P] = s_first[q * 64];
+                base[4 * P] = s_prev[q * 64];
+            }
+        }
+    }
+}
+
+}  // namespace npx

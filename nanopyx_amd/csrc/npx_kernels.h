@@ -5,14 +5,20 @@
 
 namespace npx {
 
-constexpr int kModeFrames = 0;    // per-frame RGC maps (eSRRF_ST.run contract)
-constexpr int kModeTemporal = 1;  // AVG / TAC2 moments streamed over the frames
+constexpr int kModeFrames = 0;       // per-frame RGC maps (eSRRF_ST.run contract)
+constexpr int kModeTemporal = 1;     // AVG / TAC2 moments streamed over the frames
 constexpr int kModeTemporalVar = 2;  // ... plus the sum of squared deviations for VAR
-constexpr int kQFrames = 16;      // input rows per lane, per-frame-map kernel
-constexpr int kQTemporal = 8;     // input rows per lane, temporal kernel (5 moments per pixel in registers)
-constexpr int kPrefetch = 4;      // input-tile elements per thread prefetched for the next frame
+constexpr int kModeSweep = 3;        // AVG for up to kMaxRad radii x kMaxSens sensitivities of one magnification
+constexpr int kMaxRad = 4;           // radii of one sweep job (they share the gradient lattice)
+constexpr int kMaxSens = 4;          // sensitivities of one sweep job (they share the ring sums)
+constexpr int kMaxJobs = 8;          // jobs of one batched launch (passed by value as kernel arguments)
+constexpr int kMaxK = 6;             // largest ring half-width int(2 * radius) the kernels are built for
+constexpr int kQFrames = 16;         // input rows per lane, per-frame-map kernel
+constexpr int kQTemporal = 8;        // input rows per lane, temporal and sweep kernels
+constexpr int kPrefetch = 4;         // input-tile elements per thread prefetched for the next frame
+constexpr int kBatchThreads = 1024;  // block size of batched launches (16 wavefronts)
 
-// LDS tile geometry of the fused kernel for window half-width K and Q rows per lane.
+// LDS tile geometry of the fused kernel for lattice half-width K and Q rows per lane.
 template <int K, int Q>
 struct Geometry {
     static constexpr int HG = K / 2 + 3;          // gradient-tile halo (input pixels)
@@ -22,12 +28,16 @@ struct Geometry {
     static constexpr int LR = 2 * Q + 2 * K;      // lattice rows per block
     static constexpr int LP = 64 + K;             // lattice float2 per row in one column-parity plane
     static constexpr int NW = 2 * K + 1;
+    static constexpr int LB = 128 + 2 * K;        // lattice columns per block
     static constexpr int RS = ((2 * NW + 3 * NW * NW + 1 + (NW + 1) * (NW + 1)) + 15) / 16 * 16;  // ring block floats
+    static constexpr int RN = ((1 + (NW + 1) * (NW + 1)) + 3) / 4 * 4;  // LDS floats per wave: wsum + prefix sums
 };
+
+constexpr int kLdsLimit = 160 * 1024;  // LDS bytes one workgroup may use on gfx950
 
 // Device copies of the plan tables (npx_plan.h).
 struct DevTables {
-    const float* ring;   // [phase] blocks: dy[j], dx[i], (d, t)[i][j], w[i][j]
+    const float* ring;   // [phase] blocks: dy[j], dx[i], (d, t)[i][j], w[i][j], wsum, prefix sums of w
     const float* wsum;   // [phase]
     const float4* lrw;   // lattice row taps [2H+1]
     const int* lrb;
@@ -39,24 +49,69 @@ struct DevTables {
     const int* ccb;
 };
 
+// One eSRRF job: a stack (or a frame range of it) with its parameters, tables and outputs. The single-job
+// kernels take it by value; a batched launch takes up to kMaxJobs of them by value.
+//
+// Waves of a block are assigned output phases radius by radius: wave slot g = group * nwaves + wave covers
+// radius g / nphase and phase g % nphase, so every radius of a sweep job reads the same gradient lattice.
 struct FusedArgs {
-    const void* in;
+    const void* in;   // (N_total, H, W) float32 or uint16
     int u16;
-    int N, H, W, M, HM, WM, margin;
+    int N, H, W, M, HM, WM;  // N: one past the last frame this job reads
+    int f0;                  // first frame this job reads
     int nphase, nwaves, ngroups;
-    int fpc, nchunk;  // frames per chunk (blockIdx.z / ngroups = chunk)
-    int pow_mode;
-    float S;
+    int fpc, nchunk;         // frames per chunk; chunk c covers [f0 + c * fpc, min(N, f0 + (c + 1) * fpc))
+    int chunk0;              // partial-result slot of chunk 0 (frame slices of one stack share one partial array)
+    int direct;              // 1: write the final maps; 0: write per-chunk partials to `part`
     int iw;
-    DevTables t;
+    int KL;                  // lattice half-width (max of K[])
+    int nrad;
+    int ns;
+    // taps of the gradient lattice (depend on H, W, M) and of the intensity Catmull-Rom (H, W, M)
+    const float4* lrw;
+    const int* lrb;
+    const float4* lcw;
+    const int* lcb;
+    const float4* crw;
+    const int* crb;
+    const float4* ccw;
+    const int* ccb;
+    // per radius: ring blocks [phase][rs], half-width, block stride, margin
+    const float* ring[kMaxRad];
+    int K[kMaxRad];
+    int rs[kMaxRad];
+    int margin[kMaxRad];
+    // per sensitivity: value and power mode (0: none, 1: general, 2/3/4: integer)
+    float sens[kMaxSens];
+    int sens_pow[kMaxSens];
+    // outputs
     float* out;   // frames mode: (N, HM, WM)
-    float* part;  // temporal mode, nchunk > 1: [nchunk][5][HM*WM]
+    float* part;  // temporal: [slot][5][HM*WM]; sweep: [slot][nrad][ns][HM*WM]
     float* avg;
     float* var;
     float* tac2;
+    float* sweep_avg[kMaxRad * kMaxSens];  // sweep: AVG map of (radius r, sensitivity s) at [r * kMaxSens + s]
 };
 
-hipError_t launch_fused(const FusedArgs& a, int K, dim3 grid, dim3 block, int mode, hipStream_t st);
+// Several jobs in one launch: job j owns blocks [start[j], start[j + 1]), laid out as
+// ((chunk * ngroups + group) * nregy + by) * nregx + bx.
+struct BatchArgs {
+    int njobs;
+    int start[kMaxJobs + 1];
+    int nregx[kMaxJobs];
+    int nregy[kMaxJobs];
+    FusedArgs job[kMaxJobs];
+};
+static_assert(sizeof(BatchArgs) <= 4096, "batched kernel arguments must fit the 4 KiB kernel-argument limit");
+
+// Dynamic LDS bytes of the fused kernels for lattice half-width KL, Q rows per lane, nwaves per block.
+size_t fused_lds_bytes(int KL, int Q, int nwaves, int mode);
+
+hipError_t launch_fused(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st);
+hipError_t launch_batched(const BatchArgs& b, int mode, hipStream_t st);
+int batched_lattice_k(int kmax);  // lattice half-width of a batched launch whose widest ring has half-width kmax
+hipError_t launch_means_merge(const float* part, int nchunk, int fpc, int N, size_t P, int nout, float* const* outs,
+                              hipStream_t st);
 hipError_t launch_merge(const float* part, int nchunk, int fpc, int N, size_t P, float* avg, float* var, float* tac2,
                         hipStream_t st);
 hipError_t launch_rgc_arrays(const float* gx, const float* gy, const float* img, int N, int HM, int WM, int M,

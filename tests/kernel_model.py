@@ -18,25 +18,36 @@ f32 = np.float32
 
 def model_frames(stack, M, R, S, iw=True, Q=None):
     """Per-frame maps as the kernel computes them with Q input rows per lane (default: the per-frame kernel's)."""
+    return model_multi(stack, M, [R], [S], iw=iw, Q=Q)[(float(R), float(S))]
+
+
+def _pow_mode(Sf):
+    return 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
+
+
+def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
+    """Per-frame maps of every (radius, sensitivity), computed as one block of the batched kernel does: the lattice
+    is built for half-width KL (default: the largest K of the radii) and every wave runs the ring of its own radius
+    at offset KL - K into it. Returns {(R, S): (N, HM, WM) float32}."""
     stack = np.asarray(stack)
     N, H, W = stack.shape
-    t = _hip.plan_tables(H, W, M, R)
-    d = t["info"]
-    K, HG, HI, LP = d["K"], d["HG"], d["HI"], d["LP"]
+    tabs = [_hip.plan_tables(H, W, M, R) for R in radii]
+    infos = [t["info"] for t in tabs]
+    t = tabs[0]
+    d = infos[0]
+    Ks = [i["K"] for i in infos]
+    KL = max(Ks) if KL is None else KL
+    assert KL >= max(Ks)
+    HG = KL // 2 + 3
+    HI = HG + 1
+    LP = 64 + KL
     Q = d["Q_frames"] if Q is None else Q
     TIR, TIC = Q + 2 * HI, 64 + 2 * HI
     TGR, TGC = Q + 2 * HG, 64 + 2 * HG
-    LR = 2 * Q + 2 * K
-    HM, WM, margin, nphase = d["HM"], d["WM"], d["margin"], d["nphase"]
-    nwaves, ngroups = d["nwaves"], d["ngroups"]
-    NW = 2 * K + 1
-    NR = 2 * Q + 2 * K - 1
+    LR = 2 * Q + 2 * KL
+    HM, WM, nphase = d["HM"], d["WM"], d["nphase"]
     LPL = LR * LP
-    dyt, dxt, dtab, ttab, wtab = t["dy"], t["dx"], t["d"], t["t"], t["w"]
-    wsum = t["wsum"]
-    Sf = f32(S)
-    pow_mode = 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
-    out = np.zeros((N, HM, WM), np.float32)
+    outs = {(float(R), float(S)): np.zeros((N, HM, WM), np.float32) for R in radii for S in sens}
     nregx, nregy = (W + 63) // 64, (H + Q - 1) // Q
     lanes = np.arange(64)
     for f in range(N):
@@ -73,7 +84,7 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                     s_gc[r, ok] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0
                     s_gr[r, ok] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0
                 # (3a) rows: Catmull-Rom of the gradient tile along rows for every lattice row
-                a_lo, b_lo = 2 * y0 - K, 2 * x0 - K
+                a_lo, b_lo = 2 * y0 - KL, 2 * x0 - KL
                 s_rx = np.zeros((LR, TGC), np.float32)
                 s_ry = np.zeros((LR, TGC), np.float32)
                 for la in range(LR):
@@ -115,22 +126,29 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                     if n2 > 0:
                         inv = f32(1.0 / np.sqrt(np.float64(n2)))
                         s_lat[idx] = (gxv * inv, gyv * inv)
-                # (4)+(5) waves
-                for group in range(ngroups):
-                    for wave in range(nwaves):
-                        ph = group * nwaves + wave
-                        if ph >= nphase:
-                            continue
+                # (4)+(5) waves: one per (radius, phase)
+                for ri, R in enumerate(radii):
+                    tr_ = tabs[ri]
+                    K = Ks[ri]
+                    DK = KL - K
+                    NW = 2 * K + 1
+                    NR = 2 * Q + 2 * K - 1
+                    margin = infos[ri]["margin"]
+                    dyt, dxt, dtab, ttab = tr_["dy"], tr_["dx"], tr_["d"], tr_["t"]
+                    wsum = tr_["wsum"]
+                    for ph in range(nphase):
                         py, px = divmod(ph, M)
                         ey, ex = (2 * py + 1) // M, (2 * px + 1) // M
                         xi = x0 + lanes
                         cM = M * xi + px
                         acc = np.zeros((Q, 64), np.float32)
                         for i in range(NW):
-                            c = ex + i
-                            base = (c & 1) * LPL + ey * LP + lanes + (c >> 1)
+                            c = ex + i + DK
+                            base = (c & 1) * LPL + (ey + DK) * LP + lanes + (c >> 1)
                             rows = base[None, :] + np.arange(NR)[:, None] * LP
                             assert rows.min() >= 0 and rows.max() < 2 * LPL
+                            # each read stays inside its own column-parity plane
+                            assert ((rows - (c & 1) * LPL) < LPL).all()
                             g = s_lat[rows]                       # (NR, 64, 2)
                             dxi = dxt[ph, i]
                             for r in range(NR):
@@ -157,10 +175,9 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                             valid = inside & (rM >= margin) & (rM < HM - margin) & (cM >= margin) & (cM < WM - margin)
                             a0 = 2 * yi + ey
                             b0 = 2 * xi + ex
-                            interior = (a0 - K >= 1) & (a0 + K <= 2 * HM - 2) & (b0 - K >= 1) & (b0 + K <= 2 * WM - 2)
                             den = np.full(64, wsum[ph], np.float32)
                             # border pixels: normaliser from the 2D prefix sums of w over the valid window part
-                            P = t["prefix"][ph]
+                            P = tr_["prefix"][ph]
                             j0 = max(0, 1 - (a0 - K))
                             j1 = max(0, min(NW, 2 * HM - 2 - (a0 - K) + 1))
                             i0 = np.maximum(0, 1 - (b0 - K))
@@ -172,20 +189,13 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                             pd = ((P[j1, i1] - P[jl, i1]) - (P[j1, il] - P[jl, il])).astype(np.float32)
                             den = np.where(edge, pd, den).astype(np.float32)
                             with np.errstate(invalid="ignore", divide="ignore"):
-                                rgc = np.where(den > 0, acc[q] / den, 0).astype(np.float32)
-                            if pow_mode == 2:
-                                rgc = rgc * rgc
-                            elif pow_mode == 3:
-                                rgc = rgc * rgc * rgc
-                            elif pow_mode == 4:
-                                rgc = (rgc * rgc) * (rgc * rgc)
-                            elif pow_mode == 1:
-                                rgc = np.where(rgc > 0, np.exp2(Sf * np.log2(np.maximum(rgc, 1e-38))), 0).astype(np.float32)
+                                rgc0 = np.where(den > 0, acc[q] / den, 0).astype(np.float32)
+                            inten = None
                             if iw:
                                 rMs = min(rM, HM - 1)
                                 wrow = t["crw"][rMs]
                                 crow = t["crb"][rMs] - (y0 - HI)
-                                s = np.zeros(64, np.float32)
+                                sacc = np.zeros(64, np.float32)
                                 for v in range(4):
                                     tc = np.clip(ccol + v, 0, TIC - 1)
                                     bad = (wcol[:, v] != 0) & ((ccol + v < 0) | (ccol + v >= TIC)) & valid
@@ -196,11 +206,26 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
                                         if wrow[u] != 0 and valid.any():
                                             assert 0 <= crow + u < TIR, "intensity row tap outside tile"
                                         p = p + s_in[tr, tc] * wrow[u]
-                                    s = s + p * wcol[:, v]
-                                rgc = rgc * s
-                            v_out = np.where(valid, rgc, 0).astype(np.float32)
-                            sel = inside
-                            if sel.any():
-                                assert rM < HM and (cM[sel] < WM).all()
-                                out[f, rM, cM[sel]] = v_out[sel]
-    return out
+                                    sacc = sacc + p * wcol[:, v]
+                                inten = sacc
+                            for S in sens:
+                                Sf = f32(S)
+                                pow_mode = _pow_mode(Sf)
+                                rgc = rgc0
+                                if pow_mode == 2:
+                                    rgc = rgc * rgc
+                                elif pow_mode == 3:
+                                    rgc = rgc * rgc * rgc
+                                elif pow_mode == 4:
+                                    rgc = (rgc * rgc) * (rgc * rgc)
+                                elif pow_mode == 1:
+                                    rgc = np.where(rgc > 0, np.exp2(Sf * np.log2(np.maximum(rgc, 1e-38))),
+                                                   0).astype(np.float32)
+                                if inten is not None:
+                                    rgc = rgc * inten
+                                v_out = np.where(valid, rgc, 0).astype(np.float32)
+                                sel = inside
+                                if sel.any():
+                                    assert rM < HM and (cM[sel] < WM).all()
+                                    outs[(float(R), float(S))][f, rM, cM[sel]] = v_out[sel]
+    return outs

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN_CASES, rel_err
-from kernel_model import model_frames
+from kernel_model import model_frames, model_multi
 
 
 @pytest.mark.parametrize("name", ["m5_r15_s1", "m2_r10_s3_noiw", "m3_r25_s17", "m8_r12_s2", "m1_r15_s1"])
@@ -31,3 +31,21 @@ def test_model_edge_shapes(oracle_built, shape, M, R, S, Q):
     out = model_frames(x, M, R, S, Q=Q)
     ref = oracle_built.esrrf(x, M, R, S)
     assert rel_err(out, ref)[0] <= 1e-5
+
+
+@pytest.mark.parametrize("shape,M,radii,KL", [
+    ((1, 10, 70), 2, [1.0, 1.5, 2.0, 2.5], None),  # sweep radii: K 2..5 on a lattice of K 5
+    ((1, 9, 13), 3, [1.0, 2.0], 6),               # lattice wider than every ring (batched launch of K 6)
+    ((1, 5, 4), 4, [0.4, 3.4], None),             # K 0 beside K 6
+])
+def test_model_shared_lattice(oracle_built, shape, M, radii, KL):
+    """Batched/sweep blocks: one lattice of half-width KL, each wave reads its own ring at offset KL - K; every
+    (radius, sensitivity) must match the oracle's single-parameter run."""
+    rng = np.random.default_rng(2)
+    x = (rng.random(shape) * 100 + 1).astype(np.float32)
+    sens = [1.0, 2.0, 2.5]
+    outs = model_multi(x, M, radii, sens, Q=8, KL=KL)
+    for R in radii:
+        for S in sens:
+            ref = oracle_built.esrrf(x, M, R, S)
+            assert rel_err(outs[(float(R), float(S))], ref)[0] <= 1e-5, (R, S)
