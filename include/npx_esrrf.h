@@ -112,6 +112,40 @@ int npx_graph_launch(void *graph, void *stream);
 int npx_graph_destroy(void *graph);
 
 /*
+ * Serving session (continuous batching of eSRRF + temporal-correlation jobs). No reference counterpart: NanoPyx has
+ * no serving layer; every job computes what one timepoint of the eSRRF workflow computes,
+ * src/nanopyx/methods/esrrf/eSRRF_workflow.py:98-119 (eSRRF_ST.run, then calculate_eSRRF_temporal_correlations,
+ * src/nanopyx/core/transform/sr_temporal_correlations.py:46-75,205-221), with the results npx_esrrf_temporal gives.
+ *
+ * npx_session_create allocates one device pool of pool_bytes that holds, per admitted job, its geometry tables, its
+ * running temporal moments (5 * H*M * W*M floats) and — for jobs submitted from host memory — its stack. At most
+ * max_active jobs are admitted at once; one step aims to process about px_frames_per_step output pixel-frames.
+ *
+ * npx_session_submit queues a job (never blocks). src is a device pointer (src_is_host 0; it must stay valid until
+ * the job is done) or a host pointer, preferably pinned (src_is_host 1; copied into the pool on admission, it must
+ * stay valid until then). Outputs are caller-owned device maps (H*M, W*M); those of unset mode bits may be NULL.
+ *
+ * npx_session_step retires finished jobs, admits queued jobs that fit the pool (arrival order; a job that does not
+ * fit yet lets later ones pass), then launches one frame slice of every admitted job whose copy has landed, packed
+ * up to 8 jobs per launch, on `stream`. It never blocks; it launches nothing while two earlier steps are still
+ * running on the GPU. *launches receives the number of launches made.
+ *
+ * npx_session_poll returns 0 (queued), 1 (admitted / running) or 2 (done: maps complete); npx_session_forget drops a
+ * done job's record. npx_session_stats fills stats[6]: queued jobs, admitted jobs, free pool bytes, launches,
+ * slices, steps. npx_session_destroy waits for the device and frees the pool. A session's calls are serialised by
+ * its own mutex; one session drives one stream.
+ */
+int npx_session_create(size_t pool_bytes, int max_active, long long px_frames_per_step, void **session);
+int npx_session_submit(void *session, const void *src, int src_is_host, int dtype, int N, int H, int W, int M,
+                       double R, double S, int iw, unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2,
+                       long long *job_id);
+int npx_session_step(void *session, void *stream, int *launches);
+int npx_session_poll(void *session, long long job_id);
+int npx_session_forget(void *session, long long job_id);
+int npx_session_stats(void *session, long long *stats);
+int npx_session_destroy(void *session);
+
+/*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
  */

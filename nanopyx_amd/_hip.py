@@ -74,6 +74,15 @@ def lib():
         L.npx_capture_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
         L.npx_graph_launch.argtypes = [_c_void_p, _c_void_p]
         L.npx_graph_destroy.argtypes = [_c_void_p]
+        L.npx_session_create.argtypes = [_c_size_t, _c_int, ctypes.c_longlong, ctypes.POINTER(_c_void_p)]
+        L.npx_session_submit.argtypes = [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                         _c_double, _c_double, _c_int, ctypes.c_uint, _c_void_p, _c_void_p,
+                                         _c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.npx_session_step.argtypes = [_c_void_p, _c_void_p, _ip]
+        L.npx_session_poll.argtypes = [_c_void_p, ctypes.c_longlong]
+        L.npx_session_forget.argtypes = [_c_void_p, ctypes.c_longlong]
+        L.npx_session_stats.argtypes = [_c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.npx_session_destroy.argtypes = [_c_void_p]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -251,6 +260,71 @@ class Graph:
         if self._exec is not None:
             lib().npx_graph_destroy(self._exec)
             self._exec = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Session:
+    """A serving session of the library (npx_session_*): continuous batching of eSRRF + temporal jobs."""
+
+    QUEUED, RUNNING, DONE = 0, 1, 2
+
+    def __init__(self, pool_bytes: int, max_active: int = 16, px_frames_per_step: int = 1 << 29):
+        _torch()
+        h = _c_void_p()
+        _check(lib().npx_session_create(int(pool_bytes), int(max_active), int(px_frames_per_step), ctypes.byref(h)))
+        self._h = h.value
+
+    def submit(self, stack, magnification, radius, sensitivity, iw, modes, outputs):
+        """Queues a job. stack: (N, H, W) float32 / uint16 tensor on the device, or on the host (pinned memory
+        preferred; copied into the session pool on admission). outputs: {mode: device tensor (H*M, W*M)}.
+        The caller keeps stack and outputs alive until the job is done."""
+        torch = _torch()
+        if stack.dim() != 3:
+            raise ValueError("Image must be 2D and 3D (sequence of 2D images)")
+        if stack.dtype not in (torch.float32, torch.uint16) or not stack.is_contiguous():
+            raise ValueError("served stacks must be contiguous float32 or uint16 tensors")
+        N, H, W = stack.shape
+        mask = 0
+        for m in modes:
+            if m not in MODE_BITS:
+                raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+            mask |= MODE_BITS[m]
+        ptr = lambda k: outputs[k].data_ptr() if k in outputs else None
+        jid = ctypes.c_longlong(0)
+        _check(lib().npx_session_submit(self._h, stack.data_ptr(), int(not stack.is_cuda), _dtype_code(stack), N, H,
+                                        W, int(magnification), float(radius), float(sensitivity), int(bool(iw)), mask,
+                                        ptr("AVG"), ptr("VAR"), ptr("TAC2"), ctypes.byref(jid)))
+        return jid.value
+
+    def step(self, stream=None) -> int:
+        torch = _torch()
+        n = ctypes.c_int(0)
+        _check(lib().npx_session_step(self._h, _stream(torch, stream), ctypes.byref(n)))
+        return n.value
+
+    def poll(self, job_id) -> int:
+        rc = lib().npx_session_poll(self._h, int(job_id))
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def forget(self, job_id) -> None:
+        _check(lib().npx_session_forget(self._h, int(job_id)))
+
+    def stats(self):
+        a = (ctypes.c_longlong * 6)()
+        _check(lib().npx_session_stats(self._h, a))
+        return dict(zip(("queued", "active", "free_bytes", "launches", "slices", "steps"), list(a)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            _check(lib().npx_session_destroy(h))
 
     def __del__(self):
         try:

@@ -13,9 +13,11 @@
 
 #include "../../include/npx_esrrf.h"
 #include "npx_kernels.h"
+#include "npx_host.h"
 #include "npx_plan.h"
 
-namespace {
+namespace npx {
+namespace host {
 
 thread_local std::string g_err;
 
@@ -68,18 +70,10 @@ Plan* get_plan(int H, int W, int M, double R) {
 
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-// Uploads the plan's tables to the current device once (blocking; completes before returning).
-int device_tables(Plan* p, const npx::DevTables** out) {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    auto it = p->dev.find(dev);
-    if (it != p->dev.end()) {
-        *out = &it->second.t;
-        return NPX_OK;
-    }
-    const npx::PlanHost& h = p->host;
-    auto taps = [](const std::vector<npx::CrTap>& v, std::vector<float>& w, std::vector<int>& b) {
+// One contiguous blob of the plan's tables (ring blocks, wsum, the four tap tables as float4 weights + int bases),
+// every segment 256-byte aligned; tables_in(base) gives the device view of the blob copied to `base`.
+void pack_tables(const PlanHost& h, TableBlob* out) {
+    auto taps = [](const std::vector<CrTap>& v, std::vector<float>& w, std::vector<int>& b) {
         w.resize(v.size() * 4);
         b.resize(v.size());
         for (size_t i = 0; i < v.size(); ++i) {
@@ -93,47 +87,60 @@ int device_tables(Plan* p, const npx::DevTables** out) {
     taps(h.lat_col, lcw, lcb);
     taps(h.cr_row, crw, crb);
     taps(h.cr_col, ccw, ccb);
-    struct Seg {
-        const void* src;
-        size_t bytes;
-        size_t off;
-    };
-    std::vector<Seg> segs = {
-        {h.ring.data(), h.ring.size() * 4, 0}, {h.wsum.data(), h.wsum.size() * 4, 0},
-        {lrw.data(), lrw.size() * 4, 0},       {lrb.data(), lrb.size() * 4, 0},
-        {lcw.data(), lcw.size() * 4, 0},       {lcb.data(), lcb.size() * 4, 0},
-        {crw.data(), crw.size() * 4, 0},       {crb.data(), crb.size() * 4, 0},
-        {ccw.data(), ccw.size() * 4, 0},       {ccb.data(), ccb.size() * 4, 0},
-    };
+    const void* src[10] = {h.ring.data(), h.wsum.data(), lrw.data(), lrb.data(), lcw.data(),
+                           lcb.data(),    crw.data(),    crb.data(), ccw.data(), ccb.data()};
+    const size_t bytes[10] = {h.ring.size() * 4, h.wsum.size() * 4, lrw.size() * 4, lrb.size() * 4,
+                              lcw.size() * 4,    lcb.size() * 4,    crw.size() * 4, crb.size() * 4,
+                              ccw.size() * 4,    ccb.size() * 4};
     size_t total = 0;
-    for (auto& s : segs) {
-        s.off = total;
-        total += align256(s.bytes);
+    for (int i = 0; i < 10; ++i) {
+        out->off[i] = total;
+        total += align256(bytes[i]);
     }
-    std::vector<uint8_t> blob(total, 0);
-    for (auto& s : segs) std::memcpy(blob.data() + s.off, s.src, s.bytes);
+    out->bytes.assign(total, 0);
+    for (int i = 0; i < 10; ++i) std::memcpy(out->bytes.data() + out->off[i], src[i], bytes[i]);
+}
+
+DevTables TableBlob::tables_in(void* mem) const {
+    auto* base = static_cast<uint8_t*>(mem);
+    DevTables t;
+    t.ring = reinterpret_cast<const float*>(base + off[0]);
+    t.wsum = reinterpret_cast<const float*>(base + off[1]);
+    t.lrw = reinterpret_cast<const float4*>(base + off[2]);
+    t.lrb = reinterpret_cast<const int*>(base + off[3]);
+    t.lcw = reinterpret_cast<const float4*>(base + off[4]);
+    t.lcb = reinterpret_cast<const int*>(base + off[5]);
+    t.crw = reinterpret_cast<const float4*>(base + off[6]);
+    t.crb = reinterpret_cast<const int*>(base + off[7]);
+    t.ccw = reinterpret_cast<const float4*>(base + off[8]);
+    t.ccb = reinterpret_cast<const int*>(base + off[9]);
+    return t;
+}
+
+// Uploads the plan's tables to the current device once (blocking; completes before returning).
+int device_tables(Plan* p, const npx::DevTables** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    auto it = p->dev.find(dev);
+    if (it != p->dev.end()) {
+        *out = &it->second.t;
+        return NPX_OK;
+    }
+    TableBlob blob;
+    pack_tables(p->host, &blob);
     void* mem = nullptr;
-    e = hipMalloc(&mem, total);
+    e = hipMalloc(&mem, blob.bytes.size());
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan tables)");
-    e = hipMemcpy(mem, blob.data(), total, hipMemcpyHostToDevice);
+    e = hipMemcpy(mem, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         (void)hipFree(mem);
         return hip_fail(e, "upload of plan tables");
     }
-    auto* base = static_cast<uint8_t*>(mem);
     DevCopy dc;
     dc.mem = mem;
-    dc.t.ring = reinterpret_cast<const float*>(base + segs[0].off);
-    dc.t.wsum = reinterpret_cast<const float*>(base + segs[1].off);
-    dc.t.lrw = reinterpret_cast<const float4*>(base + segs[2].off);
-    dc.t.lrb = reinterpret_cast<const int*>(base + segs[3].off);
-    dc.t.lcw = reinterpret_cast<const float4*>(base + segs[4].off);
-    dc.t.lcb = reinterpret_cast<const int*>(base + segs[5].off);
-    dc.t.crw = reinterpret_cast<const float4*>(base + segs[6].off);
-    dc.t.crb = reinterpret_cast<const int*>(base + segs[7].off);
-    dc.t.ccw = reinterpret_cast<const float4*>(base + segs[8].off);
-    dc.t.ccb = reinterpret_cast<const int*>(base + segs[9].off);
+    dc.t = blob.tables_in(mem);
     auto res = p->dev.emplace(dev, dc);
     *out = &res.first->second.t;
     return NPX_OK;
@@ -148,8 +155,6 @@ struct CallEvents {
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<CallEvents> g_events;
-
-bool capturing(hipStream_t st);
 
 bool timing_begin(hipStream_t st, CallEvents* ev) {
     std::lock_guard<std::mutex> lk(g_tmu);
@@ -166,10 +171,6 @@ void timing_end(const CallEvents& ev) {
 }
 
 // ------------------------------------------------------------------------------------------ launch shape
-
-struct Shape {
-    int nwaves, ngroups, nregx, nregy;
-};
 
 // Launch-shape knobs (npx_set_launch): wavefronts per block (one output phase each) and the number of
 // blocks the temporal kernel aims for when it splits the frames into chunks.
@@ -364,7 +365,26 @@ int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, con
     return NPX_OK;
 }
 
-}  // namespace
+int host_plan(int H, int W, int M, double R, const PlanHost** h) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Plan* p = get_plan(H, W, M, R);
+    if (!p) return NPX_ERR_UNSUPPORTED;
+    *h = &p->host;
+    return NPX_OK;
+}
+
+int tables_for(int H, int W, int M, double R, const PlanHost** h, const DevTables** t, hipStream_t st) {
+    Plan* p = nullptr;
+    const int rc = plan_and_tables(H, W, M, R, &p, t, st);
+    if (rc) return rc;
+    *h = &p->host;
+    return NPX_OK;
+}
+
+}  // namespace host
+}  // namespace npx
+
+using namespace npx::host;
 
 extern "C" {
 

@@ -287,6 +287,25 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     float* const s_first = s_ring + a.nwaves * RNL + (wave * Q) * 64 + lane;
     float* const s_prev = s_first + a.nwaves * Q * 64;
 
+    // frame slices of a served job resume the moments folded so far
+    if constexpr (TEMPORAL) {
+        if (a.state && a.n0 > 0 && active) {
+            const size_t P = static_cast<size_t>(a.HM) * a.WM;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int yi = y0 + q;
+                if (xi >= a.W || yi >= a.H) continue;
+                const size_t pix = static_cast<size_t>(M * yi + py) * a.WM + cM;
+                mu[q] = a.state[pix];
+                if (WITH_M2) m2[WITH_M2 ? q : 0] = a.state[P + pix];
+                cc[q] = a.state[2 * P + pix];
+                s_first[q * 64] = a.state[3 * P + pix];
+                s_prev[q * 64] = a.state[4 * P + pix];
+            }
+        }
+    }
+    const int n0 = TEMPORAL && a.state ? a.n0 : 0;
+
     // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
     float pf[kPrefetch];
 #pragma unroll
@@ -397,7 +416,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Branch-free per
             // pixel: pixels whose window crosses the border take their normaliser from the prefix-sum table.
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
-            const float inv_np1 = 1.0f / static_cast<float>(f - f_begin + 1);
+            const int nabs = n0 + (f - f_begin);  // values already folded into the moments
+            const float inv_np1 = 1.0f / static_cast<float>(nabs + 1);
             const float wsum = ringL[0];
             const float* P = ringL + 1;
             const int b0 = 2 * xi + ex;
@@ -457,13 +477,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                         if (xi < a.W && yi < a.H) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
                     } else {
                         float fq = val, pq = val;
-                        if (f == f_begin) {
+                        if (nabs == 0) {
                             s_first[q * 64] = val;
                         } else {
                             fq = s_first[q * 64];
                             pq = s_prev[q * 64];
                         }
-                        moments_push<WITH_M2>(val, f - f_begin, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
+                        moments_push<WITH_M2>(val, nabs, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
                         s_prev[q * 64] = val;
                     }
                 }
@@ -490,10 +510,16 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                     else a.part[((static_cast<size_t>(pslot) * a.nrad + rad) * a.ns + s) * P + pix] = mu[s * Q + q];
                 }
             } else if (a.direct) {
-                const float n = static_cast<float>(f_end - f_begin);
+                const float n = static_cast<float>(n0 + (f_end - f_begin));
                 if (a.avg) a.avg[pix] = mu[q];
                 if (a.var) a.var[pix] = WITH_M2 ? m2[WITH_M2 ? q : 0] / n : 0.f;
                 if (a.tac2) a.tac2[pix] = cc[q] / (n - 1.0f);
+            } else if (a.state) {
+                a.state[pix] = mu[q];
+                if (WITH_M2) a.state[P + pix] = m2[WITH_M2 ? q : 0];
+                a.state[2 * P + pix] = cc[q];
+                a.state[3 * P + pix] = s_first[q * 64];
+                a.state[4 * P + pix] = s_prev[q * 64];
             } else {
                 float* base = a.part + static_cast<size_t>(pslot) * 5 * P + pix;
                 base[0] = mu[q];

@@ -62,7 +62,8 @@ struct FusedArgs {
     int nphase, nwaves, ngroups;
     int fpc, nchunk;         // frames per chunk; chunk c covers [f0 + c * fpc, min(N, f0 + (c + 1) * fpc))
     int chunk0;              // partial-result slot of chunk 0 (frame slices of one stack share one partial array)
-    int direct;              // 1: write the final maps; 0: write per-chunk partials to `part`
+    int direct;              // 1: write the final maps; 0: write per-chunk partials to `part` or the running state
+    int n0;                  // temporal: frames already folded into `state` (resumed when > 0)
     int iw;
     int KL;                  // lattice half-width (max of K[])
     int nrad;
@@ -87,6 +88,8 @@ struct FusedArgs {
     // outputs
     float* out;   // frames mode: (N, HM, WM)
     float* part;  // temporal: [slot][5][HM*WM]; sweep: [slot][nrad][ns][HM*WM]
+    float* state; // temporal, frame slices of a served job: running moments [5][HM*WM] (mean, M2, co-moment,
+                  // first value, previous value), read when n0 > 0 and written unless `direct`
     float* avg;
     float* var;
     float* tac2;

@@ -114,3 +114,47 @@ def request_scene(rng: np.random.Generator, seed: int) -> Scene:
     return Scene(N, H, W, max(1, int(round(0.02 * H * W))), 1.5, 500, 0.3, 100, 0.0, 2.0,
                  100.0 if dtype == "uint16" else 0.0, dtype, seed,
                  dict(magnification=M, radius=R, sensitivity=S, doIntensityWeighting=True, temporal=(mode,)))
+
+
+def generate_device(scene: Scene, frames: int | None = None, device="cuda"):
+    """The recipe of ``generate`` rendered on the GPU with torch's generator seeded by ``scene.seed`` (same scene
+    model and statistics, not the same draws as ``generate``). Returns an (N, H, W) float32 or uint16 device tensor;
+    used for the served-stream workload, whose many large stacks would take minutes to render on the host."""
+    import torch
+    N = scene.frames if frames is None else min(frames, scene.frames)
+    H, W, E = scene.height, scene.width, scene.emitters
+    g = torch.Generator(device=device)
+    g.manual_seed(int(scene.seed))
+    ys = torch.rand(E, generator=g, device=device, dtype=torch.float64) * H
+    xs = torch.rand(E, generator=g, device=device, dtype=torch.float64) * W
+
+    def profiles(center, n):
+        edges = torch.arange(n + 1, device=device, dtype=torch.float64)
+        cdf = 0.5 * (1.0 + torch.special.erf((edges[None, :] - center[:, None]) / (sqrt(2.0) * scene.sigma)))
+        return (cdf[:, 1:] - cdf[:, :-1]).to(torch.float32)  # (E, n)
+
+    Ay, Ax = profiles(ys, H), profiles(xs, W)
+    bg_row = torch.full((W,), float(scene.background), device=device, dtype=torch.float32)
+    if scene.bg_gradient:
+        gr = scene.bg_gradient
+        bg_row = scene.background * (1 - gr / 2 + gr * torch.arange(W, device=device, dtype=torch.float32) /
+                                     max(W - 1, 1))
+    u16 = scene.dtype == "uint16"
+    out = torch.empty((N, H, W), dtype=torch.float32, device=device)
+    AyT = Ay.t().contiguous()                                   # (H, E)
+    fc = max(1, (1 << 27) // max(1, H * E))
+    for f0 in range(0, N, fc):
+        f1 = min(N, f0 + fc)
+        on = (torch.rand((f1 - f0, E), generator=g, device=device) < scene.p_on).to(torch.float32) * scene.photons
+        expected = torch.matmul(AyT[None] * on[:, None, :], Ax) + bg_row[None, None, :]
+        img = torch.poisson(expected, generator=g)
+        if scene.read_noise:
+            img = img + scene.read_noise * torch.randn(img.shape, generator=g, device=device)
+        out[f0:f1] = img + scene.offset
+    if u16:
+        out = torch.clamp(torch.round(out), 0, 65535)
+        try:
+            return out.to(torch.uint16)
+        except (RuntimeError, TypeError):
+            return torch.from_numpy(out.cpu().numpy().astype(np.uint16)).to(device)
+    return out

@@ -1,0 +1,147 @@
+"""The serving session (npx_session_*): every job's maps are the same alone and in a mix (continuous batching with
+frame slices resumed from resident moments), equal the plain fused call, match the oracle on small jobs, and jobs
+wait for pool room instead of failing."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    assert _torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return _torch
+
+
+def _report(tag, x, ref):
+    mx, mean = rel_err(x, ref)
+    print(f"{tag}: max {mx:.3e} mean {mean:.3e}", flush=True)
+    return mx
+
+
+# (N, H, W, M, R, S, mode, dtype, host)
+JOBS = [
+    (40, 24, 70, 2, 1.0, 1, "AVG", "float32", True),
+    (33, 17, 20, 3, 2.5, 2, "VAR", "uint16", False),
+    (57, 40, 64, 5, 1.7, 3, "TAC2", "float32", True),
+    (9, 12, 9, 8, 1.2, 2, "VAR", "float32", False),
+    (64, 32, 96, 4, 2.0, 2, "TAC2", "uint16", True),
+    (21, 70, 30, 2, 2.3, 1, "AVG", "uint16", False),
+]
+
+
+def _requests(torch):
+    from nanopyx_amd import serving, synthetic
+    reqs = []
+    for i, (N, H, W, M, R, S, mode, dtype, host) in enumerate(JOBS):
+        sc = synthetic.Scene(N, H, W, max(1, int(0.02 * H * W)), 1.5, 500, 0.3, 100, 0.0, 2.0,
+                             100.0 if dtype == "uint16" else 0.0, dtype, 40 + i)
+        x = synthetic.generate_device(sc)
+        if host:
+            h = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+            h.copy_(x)
+            x = h
+        reqs.append(serving.Request(x, M, R, S, mode, True))
+    torch.cuda.synchronize()
+    return reqs
+
+
+def _run(torch, reqs, idx, px_frames_per_step, pool_bytes=8 << 30, max_active=32):
+    """Submits the requests `idx` at once and steps the session until all are done; returns their maps."""
+    from nanopyx_amd import _hip
+    sess = _hip.Session(pool_bytes, max_active, px_frames_per_step)
+    outs, ids = {}, {}
+    for i in idx:
+        r = reqs[i]
+        N, H, W = r.stack.shape
+        outs[i] = {r.mode: torch.full((H * r.magnification, W * r.magnification), -7.0, device="cuda")}
+        ids[i] = sess.submit(r.stack, r.magnification, r.radius, r.sensitivity, r.iw, (r.mode,), outs[i])
+    saw_queue = False
+    for _ in range(100000):
+        st = sess.stats()
+        saw_queue = saw_queue or st["queued"] > 0 and st["active"] > 0
+        sess.step()
+        if all(sess.poll(j) == _hip.Session.DONE for j in ids.values()):
+            break
+    else:
+        raise AssertionError("session did not finish")
+    torch.cuda.synchronize()
+    stats = sess.stats()
+    sess.close()
+    return {i: outs[i][reqs[i].mode].cpu().numpy() for i in idx}, stats, saw_queue
+
+
+def test_alone_equals_mixed(torch):
+    reqs = _requests(torch)
+    idx = list(range(len(reqs)))
+    mixed, st, _ = _run(torch, reqs, idx, px_frames_per_step=1 << 16)  # small steps: several slices per job
+    print("mixed session stats", st, flush=True)
+    assert st["slices"] > len(reqs) and st["launches"] < st["slices"]  # jobs shared launches, sliced
+    for i in idx:
+        alone, _, _ = _run(torch, reqs, [i], px_frames_per_step=1 << 40)  # one slice
+        assert np.array_equal(alone[i], mixed[i]), f"job {i} differs alone vs mixed"
+        print(f"job {i}: alone == mixed", flush=True)
+
+
+def test_served_equals_plain_call(torch):
+    from nanopyx_amd import _hip
+    reqs = _requests(torch)
+    served, _, _ = _run(torch, reqs, list(range(len(reqs))), px_frames_per_step=1 << 18)
+    for i, r in enumerate(reqs):
+        plain = _hip.esrrf_temporal(r.stack.cuda(), r.magnification, r.radius, r.sensitivity, True, (r.mode,),
+                                    workspace=False)[r.mode].cpu().numpy()
+        assert _report(f"job {i} served vs plain", served[i], plain) <= 1e-6
+
+
+def test_served_matches_oracle(torch, oracle_built):
+    reqs = _requests(torch)
+    idx = [1, 3, 5]
+    served, _, _ = _run(torch, reqs, idx, px_frames_per_step=1 << 15)
+    for i in idx:
+        r = reqs[i]
+        x = r.stack.cpu().numpy().astype(np.float32)
+        ref = oracle_built.temporal(oracle_built.esrrf(x, r.magnification, r.radius, r.sensitivity, True), r.mode)
+        assert _report(f"job {i} ({r.mode}) served vs oracle", served[i], ref) <= TOL
+
+
+def test_admission_waits_for_pool(torch):
+    """A pool that holds about one job at a time: the rest queue, then run as room frees up."""
+    reqs = _requests(torch)
+    idx = [0, 2, 4]
+    full, _, _ = _run(torch, reqs, idx, px_frames_per_step=1 << 40)
+    tight, st, saw_queue = _run(torch, reqs, idx, px_frames_per_step=1 << 40, pool_bytes=2 << 20)
+    assert saw_queue
+    for i in idx:
+        assert np.array_equal(full[i], tight[i])
+
+
+def test_session_errors(torch):
+    from nanopyx_amd import _hip
+    sess = _hip.Session(1 << 20, 4, 1 << 20)
+    x = torch.zeros((10, 256, 256), device="cuda")
+    out = {"AVG": torch.empty((1024, 1024), device="cuda")}
+    with pytest.raises(ValueError):
+        sess.submit(x, 4, 1.5, 1, True, ("AVG",), out)  # state alone exceeds the 1 MiB pool
+    with pytest.raises(ValueError):
+        sess.submit(x, 4, 1.5, 1, True, ("MEDIAN",), out)
+    sess.close()
+    with pytest.raises(ValueError):
+        _hip.Session(0, 4, 1)
+
+
+def test_serve_loop_small(torch):
+    """The request loop end to end on a short seeded stream (every request completes, latencies recorded)."""
+    from nanopyx_amd import serving
+    reqs = serving.make_requests(6, seed=11, host=True, max_frames=60)
+    summary, lat = serving.serve(reqs, rate_hz=200.0, seed=11)
+    print("serve summary", summary, flush=True)
+    assert summary["requests"] == 6 and np.all(lat > 0)
+    for r in reqs:
+        plain = __import__("nanopyx_amd")._hip.esrrf_temporal(r.stack.cuda(), r.magnification, r.radius,
+                                                                r.sensitivity, True, (r.mode,), workspace=False)
+        assert rel_err(r.outputs[r.mode].cpu().numpy(), plain[r.mode].cpu().numpy())[0] <= 1e-6
