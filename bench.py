@@ -99,6 +99,40 @@ def sweep_bench(reps):
             "frac_fp32_peak": round(flops / k_s / 1e12 / FP32_PEAK_TFLOPS, 4)}
 
 
+def serving_bench(n, rate_hz, seed=5):
+    """The served stream: n seeded requests (frames log-uniform 50-1000, H and W from {64..1024}, M from
+    {2, 3, 4, 5, 8}, R uniform on [1, 2.5], S from {1, 2, 3}, one of AVG / VAR / TAC2, float32 or uint16) arriving as
+    a Poisson process at rate_hz, stacks in pinned host memory; served by the continuous-batching session and, on
+    the same arrival trace, by one plain fused call per request."""
+    import torch
+    from nanopyx_amd import serving
+    t0 = time.perf_counter()
+    reqs = serving.make_requests(n, seed=seed, host=True)
+    log(f"serving: {n} requests rendered in {time.perf_counter() - t0:.1f} s")
+    t_arr = serving.arrivals(n, rate_hz, seed)
+    warm = serving.make_requests(2, seed=seed + 1, host=True, max_frames=16)
+    serving.serve(warm, 1000.0, seed)  # first-use costs (code objects, pool) outside the measured stream
+    served, _ = serving.serve(reqs, rate_hz, seed, t_arr=t_arr)
+    log(f"served: {served}")
+    plain, _ = serving.serve_plain(reqs, rate_hz, seed, t_arr=t_arr)
+    log(f"plain: {plain}")
+    # saturated: every request arrives at t = 0 (throughput of the packed launches vs one call per request)
+    t0s = np.zeros(n)
+    served_sat, _ = serving.serve(reqs, rate_hz, seed, t_arr=t0s)
+    plain_sat, _ = serving.serve_plain(reqs, rate_hz, seed, t_arr=t0s)
+    log(f"saturated: served {served_sat} plain {plain_sat}")
+    px = sum(r.out_px_frames for r in reqs)
+    stack_gb = sum(r.stack.numel() * r.stack.element_size() for r in reqs) / 1e9
+    del reqs
+    torch.cuda.empty_cache()
+    return {"workload": f"{n} requests, Poisson {rate_hz}/s, seed {seed}: {px / 1e9:.1f} G output px-frames, "
+                        f"{stack_gb:.1f} GB of stacks in pinned host memory",
+            "rate_hz": rate_hz, "served": served, "plain_per_request": plain,
+            "saturated_served": served_sat, "saturated_plain_per_request": plain_sat,
+            "p99_speedup": round(plain["p99_ms"] / served["p99_ms"], 2),
+            "throughput_speedup": round(served["throughput_mpx_frames_s"] / plain["throughput_mpx_frames_s"], 2)}
+
+
 def cpu_baseline(x, e, seconds):
     """The reference CPU chain (CR xM, Roberts, CR x2M twice, RGC, temporal) on the first frames of the same
     stack, chunk after chunk until `seconds` of wall clock have passed."""
@@ -147,6 +181,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--serving-rate", type=float, default=50.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the eSRRF path is built and measured on a single MI355X "
@@ -222,11 +259,15 @@ def main():
                      "hbm_gbs_algorithmic": round(alg_bytes / k_s / 1e9, 1)},
         "cpu_baseline": None,
         "sweep": None,
+        "serving": None,
     }
     if not args.no_sweep:
         log("parameter sweep ...")
         result["sweep"] = sweep_bench(max(3, args.steps // 2))
         log(f"sweep: {result['sweep']}")
+    if not args.no_serving:
+        log("served stream ...")
+        result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
     if not args.no_cpu:
         log("timing the reference CPU path ...")
         result["cpu_baseline"] = cpu_baseline(x_host, e, args.cpu_seconds)

@@ -118,7 +118,8 @@ int npx_graph_destroy(void *graph);
  * src/nanopyx/core/transform/sr_temporal_correlations.py:46-75,205-221), with the results npx_esrrf_temporal gives.
  *
  * npx_session_create allocates one device pool of pool_bytes that holds, per admitted job, its geometry tables, its
- * running temporal moments (5 * H*M * W*M floats) and — for jobs submitted from host memory — its stack. At most
+ * running temporal moments (5 * H*M * W*M floats), partial moments of the frame chunks a slice of a small job runs
+ * as, and — for jobs submitted from host memory — its stack (npx_session_job_bytes gives the total). At most
  * max_active jobs are admitted at once; one step aims to process about px_frames_per_step output pixel-frames.
  *
  * npx_session_submit queues a job (never blocks). src is a device pointer (src_is_host 0; it must stay valid until
@@ -139,6 +140,8 @@ int npx_session_create(size_t pool_bytes, int max_active, long long px_frames_pe
 int npx_session_submit(void *session, const void *src, int src_is_host, int dtype, int N, int H, int W, int M,
                        double R, double S, int iw, unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2,
                        long long *job_id);
+/* Pool bytes npx_session_submit would need for such a job (0 for invalid parameters). */
+size_t npx_session_job_bytes(int N, int H, int W, int M, double R, int dtype, int src_is_host);
 int npx_session_step(void *session, void *stream, int *launches);
 int npx_session_poll(void *session, long long job_id);
 int npx_session_forget(void *session, long long job_id);

@@ -78,6 +78,8 @@ def lib():
         L.npx_session_submit.argtypes = [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                          _c_double, _c_double, _c_int, ctypes.c_uint, _c_void_p, _c_void_p,
                                          _c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.npx_session_job_bytes.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
+        L.npx_session_job_bytes.restype = _c_size_t
         L.npx_session_step.argtypes = [_c_void_p, _c_void_p, _ip]
         L.npx_session_poll.argtypes = [_c_void_p, ctypes.c_longlong]
         L.npx_session_forget.argtypes = [_c_void_p, ctypes.c_longlong]
@@ -300,6 +302,12 @@ class Session:
                                         W, int(magnification), float(radius), float(sensitivity), int(bool(iw)), mask,
                                         ptr("AVG"), ptr("VAR"), ptr("TAC2"), ctypes.byref(jid)))
         return jid.value
+
+    @staticmethod
+    def job_bytes(N, H, W, magnification, radius, dtype_u16=False, host=True) -> int:
+        """Pool bytes a job needs (tables, running moments, chunk partials, stack when submitted from the host)."""
+        return int(lib().npx_session_job_bytes(int(N), int(H), int(W), int(magnification), float(radius),
+                                               DTYPE_U16 if dtype_u16 else DTYPE_F32, int(bool(host))))
 
     def step(self, stream=None) -> int:
         torch = _torch()

@@ -117,6 +117,9 @@ hipError_t launch_means_merge(const float* part, int nchunk, int fpc, int N, siz
                               hipStream_t st);
 hipError_t launch_merge(const float* part, int nchunk, int fpc, int N, size_t P, float* avg, float* var, float* tac2,
                         hipStream_t st);
+// ... continuing a running state of n0 frames ([5][P]); to_state: write the merged moments back to the state.
+hipError_t launch_merge_state(const float* part, int nchunk, int fpc, int N, size_t P, float* state, int n0,
+                              int to_state, float* avg, float* var, float* tac2, hipStream_t st);
 hipError_t launch_rgc_arrays(const float* gx, const float* gy, const float* img, int N, int HM, int WM, int M,
                              float fwhm, float tSO, float tSS, int pow_mode, float S, int iw, int margin, float* out,
                              hipStream_t st);

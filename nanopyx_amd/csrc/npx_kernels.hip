@@ -70,12 +70,22 @@ __global__ __launch_bounds__(256) void means_merge(const float* __restrict__ par
 }
 
 // Merges per-chunk moments in frame order (Chan for the mean/M2; the lag-1 co-moment is re-centred on the
-// merged mean and gains the cross term of the two frames that meet at the chunk boundary).
+// merged mean and gains the cross term of the two frames that meet at the chunk boundary). With a running state
+// of n0 > 0 frames, the chunks continue it; the result goes to the maps or (to_state) back to the state.
 __global__ __launch_bounds__(256) void moments_merge(const float* __restrict__ part, int nchunk, int fpc, int N,
-                                                     size_t P, float* avg, float* var, float* tac2) {
+                                                     size_t P, float* avg, float* var, float* tac2, float* state,
+                                                     int n0, int to_state) {
     const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= P) return;
     float n = 0.f, mu = 0.f, m2 = 0.f, c = 0.f, first = 0.f, last = 0.f;
+    if (state && n0 > 0) {
+        n = static_cast<float>(n0);
+        mu = state[k];
+        m2 = state[P + k];
+        c = state[2 * P + k];
+        first = state[3 * P + k];
+        last = state[4 * P + k];
+    }
     for (int ch = 0; ch < nchunk; ++ch) {
         const int f0 = ch * fpc;
         const int nb_i = min(N, f0 + fpc) - f0;
@@ -104,6 +114,14 @@ __global__ __launch_bounds__(256) void moments_merge(const float* __restrict__ p
         mu = mun;
         n = nn;
         last = lb;
+    }
+    if (to_state) {
+        state[k] = mu;
+        state[P + k] = m2;
+        state[2 * P + k] = c;
+        state[3 * P + k] = first;
+        state[4 * P + k] = last;
+        return;
     }
     if (avg) avg[k] = mu;
     if (var) var[k] = m2 / n;
@@ -270,8 +288,14 @@ hipError_t launch_means_merge(const float* part, int nchunk, int fpc, int N, siz
 
 hipError_t launch_merge(const float* part, int nchunk, int fpc, int N, size_t P, float* avg, float* var, float* tac2,
                         hipStream_t st) {
+    return launch_merge_state(part, nchunk, fpc, N, P, nullptr, 0, 0, avg, var, tac2, st);
+}
+
+hipError_t launch_merge_state(const float* part, int nchunk, int fpc, int N, size_t P, float* state, int n0,
+                              int to_state, float* avg, float* var, float* tac2, hipStream_t st) {
     const unsigned blocks = static_cast<unsigned>((P + 255) / 256);
-    hipLaunchKernelGGL(moments_merge, dim3(blocks), dim3(256), 0, st, part, nchunk, fpc, N, P, avg, var, tac2);
+    hipLaunchKernelGGL(moments_merge, dim3(blocks), dim3(256), 0, st, part, nchunk, fpc, N, P, avg, var, tac2, state,
+                       n0, to_state);
     return hipGetLastError();
 }
 

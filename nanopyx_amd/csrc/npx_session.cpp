@@ -103,8 +103,11 @@ struct Job {
     float *avg = nullptr, *var = nullptr, *tac2 = nullptr;
     const PlanHost* plan = nullptr;
     // resident resources
-    size_t off = 0, bytes = 0;  // one pool block: [tables][state][stack if copied]
-    size_t state_at = 0, stack_at = 0;
+    size_t off = 0, bytes = 0;  // one pool block: [tables][state][chunk partials][stack if copied]
+    size_t state_at = 0, part_at = 0, stack_at = 0;
+    int cmax = 1;               // frame chunks one slice may run as (partials merged into the state)
+    int cfpc = 0;               // cmax > 1: frames per chunk, fixed per job so chunk boundaries never depend on
+                                // how the job is sliced (its maps are then bit-identical alone and in any mix)
     host::TableBlob blob;       // host copy of the tables (kept until retirement: source of the async copy)
     DevTables t{};
     const void* d_in = nullptr;
@@ -131,7 +134,9 @@ struct Session {
 };
 
 constexpr size_t kMaxInflight = 2;  // steps queued on the GPU ahead of the host: new jobs wait at most this many
-constexpr int kMinSliceFrames = 8;  // frames per slice at least (bounds the state traffic per frame)
+constexpr int kMinSliceFrames = 8;  // frames per slice (and per chunk of a slice) at least
+constexpr int kMaxChunks = 8;       // frame chunks of one slice at most
+constexpr int kTargetBlocks = 1024; // blocks a lone job's slice aims for (chunks add blocks to small jobs)
 
 // Releases a finished job's pool block and events (its maps are written and complete).
 void retire(Session& s, Job& j) {
@@ -206,8 +211,43 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
         if (j.state == kAdmitted && !j.final_launched && hipEventQuery(j.copied) == hipSuccess) run.push_back(&j);
     }
     if (run.empty()) return NPX_OK;
-    const long long per_job = std::max<long long>(1, s.round_px / static_cast<long long>(run.size()));
+    // Frames of each job this step. Small jobs (remaining output pixel-frames at most a quarter of the step's
+    // budget) go first and finish in this step; the rest of the budget goes to the other jobs oldest first, so a
+    // large job runs nearly as fast as alone while small arrivals never wait behind it. A job that gets a slice
+    // gets at least kMinSliceFrames frames (or what remains); once the budget is spent, later large jobs wait.
+    auto remaining = [](const Job* j) {
+        return static_cast<long long>(j->N - j->next_frame) * j->plan->HM * j->plan->WM;
+    };
+    const long long small_cut = s.round_px / 4;
+    std::sort(run.begin(), run.end(), [&](const Job* a, const Job* b) {
+        const bool sa = remaining(a) <= small_cut, sb = remaining(b) <= small_cut;
+        if (sa != sb) return sa;
+        if (sa) return remaining(a) != remaining(b) ? remaining(a) < remaining(b) : a->id < b->id;
+        return a->id < b->id;
+    });
+    std::vector<int> frames;
+    long long left = s.round_px;
+    size_t keep = 0;
+    for (size_t k = 0; k < run.size(); ++k) {
+        const Job& j = *run[k];
+        const bool small = remaining(&j) <= small_cut;
+        if (!small && left <= 0) break;
+        const long long P = static_cast<long long>(j.plan->HM) * j.plan->WM;
+        long long f = std::max<long long>(kMinSliceFrames, left / P);
+        if (j.cmax > 1) f = std::max<long long>(1, f / j.cfpc) * j.cfpc;  // whole chunks
+        f = std::min<long long>(j.N - j.next_frame, f);
+        frames.push_back(static_cast<int>(f));
+        left -= f * P;
+        keep = k + 1;
+    }
+    run.resize(keep);
     const int nw = kBatchThreads / 64;
+    struct Merge {
+        Job* job;
+        int nchunk, fpc, nframes, n0;
+        bool final_;
+    };
+    std::vector<Merge> merges;
     for (size_t j0 = 0; j0 < run.size(); j0 += kMaxJobs) {
         const size_t nj = std::min<size_t>(kMaxJobs, run.size() - j0);
         int kmax = 0;
@@ -223,33 +263,43 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
         for (size_t k = 0; k < nj; ++k) {
             Job& j = *run[j0 + k];
             const PlanHost& h = *j.plan;
-            const long long P = static_cast<long long>(h.HM) * h.WM;
-            const int frames = static_cast<int>(
-                std::min<long long>(j.N - j.next_frame, std::max<long long>(kMinSliceFrames, per_job / P)));
+            const int nf = frames[j0 + k];
             host::Shape sh;
             sh.nwaves = nw;
             sh.ngroups = (h.nphase + nw - 1) / nw;
             sh.nregx = (j.W + 63) / 64;
             sh.nregy = (j.H + kQTemporal - 1) / kQTemporal;
-            FusedArgs a = host::job_args(j.d_in, j.dtype, j.next_frame + frames, j.H, j.W, j.M, h, &j.t, sh, j.iw);
+            FusedArgs a = host::job_args(j.d_in, j.dtype, j.next_frame + nf, j.H, j.W, j.M, h, &j.t, sh, j.iw);
             host::add_radius(a, h, &j.t);
             host::add_sensitivity(a, j.S);
             a.KL = KL;
             a.f0 = j.next_frame;
-            a.fpc = frames;
-            a.nchunk = 1;
-            a.n0 = j.next_frame;
-            a.state = reinterpret_cast<float*>(s.pool.at(j.off) + j.state_at);
-            a.direct = j.next_frame + frames == j.N;
             a.avg = (j.mask & NPX_AVG) ? j.avg : nullptr;
             a.var = (j.mask & NPX_VAR) ? j.var : nullptr;
             a.tac2 = (j.mask & NPX_TAC2) ? j.tac2 : nullptr;
+            Merge mg{&j, 1, nf, nf, j.next_frame, j.next_frame + nf == j.N};
+            if (j.cmax > 1) {
+                // frame chunks in parallel, each from a fresh start into its partial slot; merged after the launch
+                mg.fpc = j.cfpc;
+                mg.nchunk = (nf + mg.fpc - 1) / mg.fpc;
+                a.fpc = mg.fpc;
+                a.nchunk = mg.nchunk;
+                a.part = reinterpret_cast<float*>(s.pool.at(j.off) + j.part_at);
+                a.direct = 0;
+                merges.push_back(mg);
+            } else {
+                a.fpc = nf;
+                a.nchunk = 1;
+                a.n0 = j.next_frame;
+                a.state = reinterpret_cast<float*>(s.pool.at(j.off) + j.state_at);
+                a.direct = mg.final_;
+            }
             b.job[k] = a;
             b.nregx[k] = sh.nregx;
             b.nregy[k] = sh.nregy;
             b.start[k] = blocks;
-            blocks += sh.nregx * sh.nregy * sh.ngroups;
-            j.next_frame += frames;
+            blocks += sh.nregx * sh.nregy * sh.ngroups * a.nchunk;
+            j.next_frame += nf;
             ++s.slices;
         }
         b.start[nj] = blocks;
@@ -257,13 +307,23 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
         if (e != hipSuccess) return hip_fail(e, "esrrf_batched(served) launch");
         ++s.launches;
         ++*nlaunch;
-        for (size_t k = j0; k < j0 + nj; ++k) {
-            Job& j = *run[k];
-            if (j.next_frame == j.N) {
-                e = hipEventRecord(j.done, st);
-                if (e != hipSuccess) return hip_fail(e, "session completion event");
-                j.final_launched = true;
-            }
+    }
+    for (const Merge& mg : merges) {
+        Job& j = *mg.job;
+        const size_t P = static_cast<size_t>(j.plan->HM) * j.plan->WM;
+        hipError_t e = launch_merge_state(reinterpret_cast<float*>(s.pool.at(j.off) + j.part_at), mg.nchunk, mg.fpc,
+                                          mg.nframes, P, reinterpret_cast<float*>(s.pool.at(j.off) + j.state_at),
+                                          mg.n0, mg.final_ ? 0 : 1, (j.mask & NPX_AVG) ? j.avg : nullptr,
+                                          (j.mask & NPX_VAR) ? j.var : nullptr, (j.mask & NPX_TAC2) ? j.tac2 : nullptr,
+                                          st);
+        if (e != hipSuccess) return hip_fail(e, "session moments merge");
+    }
+    for (Job* jp : run) {
+        Job& j = *jp;
+        if (j.next_frame == j.N) {
+            hipError_t e = hipEventRecord(j.done, st);
+            if (e != hipSuccess) return hip_fail(e, "session completion event");
+            j.final_launched = true;
         }
     }
     hipEvent_t ev = nullptr;
@@ -278,6 +338,23 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
     if (e != hipSuccess) return hip_fail(e, "session step event");
     s.inflight.push_back(ev);
     return NPX_OK;
+}
+
+// Pool footprint of a job: [tables][running state 5P][chunk partials cmax * 5P][stack when copied from the host].
+// cmax: chunks a slice may run as, enough for about kTargetBlocks blocks when the job runs alone.
+void layout(Job& j, const PlanHost& plan) {
+    const size_t P = static_cast<size_t>(plan.HM) * plan.WM;
+    const int nw = kBatchThreads / 64;
+    const long long spatial = static_cast<long long>((j.W + 63) / 64) * ((j.H + kQTemporal - 1) / kQTemporal) *
+                              ((plan.nphase + nw - 1) / nw);
+    j.cmax = static_cast<int>(std::max<long long>(1, std::min<long long>(kMaxChunks, (kTargetBlocks + spatial - 1) /
+                                                                                         spatial)));
+    j.cmax = std::min(j.cmax, std::max(1, j.N / kMinSliceFrames));
+    j.cfpc = std::max(kMinSliceFrames, (j.N + j.cmax - 1) / j.cmax);
+    j.state_at = align_up(j.blob.bytes.size());
+    j.part_at = j.state_at + align_up(5 * P * sizeof(float));
+    j.stack_at = j.part_at + (j.cmax > 1 ? align_up(static_cast<size_t>(j.cmax) * 5 * P * sizeof(float)) : 0);
+    j.bytes = j.stack_at + (j.src_host ? static_cast<size_t>(j.N) * j.H * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4) : 0);
 }
 
 }  // namespace serve
@@ -338,10 +415,7 @@ int npx_session_submit(void* session, const void* src, int src_is_host, int dtyp
     j.tac2 = d_tac2;
     j.plan = plan;
     npx::host::pack_tables(*plan, &j.blob);
-    const size_t P = static_cast<size_t>(plan->HM) * plan->WM;
-    j.state_at = npx::serve::align_up(j.blob.bytes.size());
-    j.stack_at = j.state_at + npx::serve::align_up(5 * P * sizeof(float));
-    j.bytes = j.stack_at + (j.src_host ? static_cast<size_t>(N) * H * W * (dtype == NPX_DTYPE_U16 ? 2 : 4) : 0);
+    npx::serve::layout(j, *plan);
     std::lock_guard<std::mutex> lk(s.mu);
     if (npx::serve::align_up(j.bytes) > s.pool.size())
         return npx::host::fail(NPX_ERR_ARG, "job needs more device memory than the session pool holds");
@@ -350,6 +424,22 @@ int npx_session_submit(void* session, const void* src, int src_is_host, int dtyp
     s.queue.push_back(j.id);
     s.jobs.emplace(j.id, std::move(j));
     return NPX_OK;
+}
+
+size_t npx_session_job_bytes(int N, int H, int W, int M, double R, int dtype, int src_is_host) {
+    if (N < 1 || H < 1 || W < 1 || M < 1 || M > 64) return 0;
+    const npx::PlanHost* plan = nullptr;
+    if (npx::host::host_plan(H, W, M, R, &plan) != NPX_OK) return 0;
+    Job j;
+    j.N = N;
+    j.H = H;
+    j.W = W;
+    j.M = M;
+    j.dtype = dtype;
+    j.src_host = src_is_host != 0;
+    npx::host::pack_tables(*plan, &j.blob);
+    npx::serve::layout(j, *plan);
+    return npx::serve::align_up(j.bytes);
 }
 
 int npx_session_step(void* session, void* stream, int* launches) {

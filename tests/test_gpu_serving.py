@@ -113,8 +113,15 @@ def test_admission_waits_for_pool(torch):
     """A pool that holds about one job at a time: the rest queue, then run as room frees up."""
     reqs = _requests(torch)
     idx = [0, 2, 4]
+    from nanopyx_amd import _hip
+    need = []
+    for i in idx:
+        N, H, W, M, R, S, mode, dtype, host = JOBS[i]
+        need.append(_hip.Session.job_bytes(N, H, W, M, R, dtype == "uint16", host))
+    pool = max(need) + 4096
+    assert sum(need) > pool
     full, _, _ = _run(torch, reqs, idx, px_frames_per_step=1 << 40)
-    tight, st, saw_queue = _run(torch, reqs, idx, px_frames_per_step=1 << 40, pool_bytes=2 << 20)
+    tight, st, saw_queue = _run(torch, reqs, idx, px_frames_per_step=1 << 40, pool_bytes=pool)
     assert saw_queue
     for i in idx:
         assert np.array_equal(full[i], tight[i])
