@@ -181,6 +181,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="re-run the launch-shape autotuner on the workload")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=50.0)
@@ -206,6 +207,18 @@ def main():
     x = torch.from_numpy(x_host).cuda()
     torch.cuda.synchronize()
     log(f"device: {_hip.device_info()}")
+
+    from nanopyx_amd import autotune
+    from nanopyx_amd.tuning import PACKAGED
+    tune_info = {"store": os.path.relpath(PACKAGED, ROOT), "loaded": autotune.load(PACKAGED) if os.path.exists(
+        PACKAGED) else 0}
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune_info["live"] = autotune.tune(x, M, R, S, iw, modes, persist=False)
+        tune_info["tune_seconds"] = round(time.perf_counter() - t0, 2)
+    mw, tb, tuned = _hip.tune_get(H, W, M, R)
+    tune_info.update(max_waves=mw, target_blocks=tb, tuned=tuned)
+    log(f"launch shape: {tune_info}")
 
     def step():
         return _hip.esrrf_temporal(x, M, R, S, iw, modes)
@@ -257,6 +270,7 @@ def main():
                      "kernel_ms": round(k_s * 1e3, 4), "merge_ms": round(merge_ms / max(ncalls, 1), 4),
                      "algorithmic_bytes": alg_bytes,
                      "hbm_gbs_algorithmic": round(alg_bytes / k_s / 1e9, 1)},
+        "launch_shape": tune_info,
         "cpu_baseline": None,
         "sweep": None,
         "serving": None,

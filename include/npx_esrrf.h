@@ -174,6 +174,24 @@ int npx_plan_tables(int H, int W, int M, double R, float *ring, float *wsum, int
  * between calls changes the frame chunking and therefore the last bits of the temporal moments. */
 int npx_set_launch(int max_waves_per_block, int target_blocks);
 
+/*
+ * Autotuner of the launch shape, per (H, W, M, R). npx_autotune times every candidate shape — waves (output phases)
+ * per block from {16, 8, 4} and a target block count from {256 .. 8192}, which sets the frames per chunk — with
+ * npx_esrrf_temporal on the caller's stack and outputs (HIP events on `stream`; blocking), keeps the fastest for
+ * later calls with the same (H, W, M, R) and returns its time per call in *best_ms. Candidates whose chunking needs
+ * more than work_bytes are skipped; npx_autotune_workspace gives the workspace that admits all of them.
+ * npx_tune_set stores a shape directly (0, 0 forgets it); npx_tune_get writes the shape calls with these parameters
+ * use into cfg[2] and returns 1 if it was tuned, 0 for the defaults. No reference counterpart: it takes the place
+ * of the OpenCL path's fixed frames-per-chunk sizing from device memory (max_slices,
+ * src/nanopyx/core/transform/_le_esrrf.pyx:64-69) and of the Liquid Engine's run-type benchmarking.
+ */
+size_t npx_autotune_workspace(int N, int H, int W, int M, double R);
+int npx_autotune(const void *d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                 unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2, void *d_work, size_t work_bytes,
+                 void *stream, double *best_ms);
+int npx_tune_set(int H, int W, int M, double R, int max_waves_per_block, int target_blocks);
+int npx_tune_get(int H, int W, int M, double R, int *cfg);
+
 /* Turns per-call HIP-event timing of the eSRRF kernels on (1) or off (0). No reference counterpart: it lets a
  * caller (the benchmark) read the fused kernel's own duration on the stream it was launched on. */
 int npx_timing(int on);

@@ -85,6 +85,13 @@ def lib():
         L.npx_session_forget.argtypes = [_c_void_p, ctypes.c_longlong]
         L.npx_session_stats.argtypes = [_c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.npx_session_destroy.argtypes = [_c_void_p]
+        L.npx_autotune_workspace.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double]
+        L.npx_autotune_workspace.restype = _c_size_t
+        L.npx_autotune.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_double, _c_int,
+                                   ctypes.c_uint, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_size_t, _c_void_p,
+                                   ctypes.POINTER(_c_double)]
+        L.npx_tune_set.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
+        L.npx_tune_get.argtypes = [_c_int, _c_int, _c_int, _c_double, _ip]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -417,6 +424,45 @@ def plan_tables(H, W, M, R):
                  prefix=ring[:, 2 * NW + 3 * NW * NW + 1:2 * NW + 3 * NW * NW + 1 + (NW + 1) ** 2].reshape(P, NW + 1,
                                                                                                      NW + 1))
     return dict(info=d, ring=ring, wsum=wsum, **views, **arr)
+
+
+def autotune(stack, magnification, radius, sensitivity, iw, modes=("AVG",), stream=None):
+    """Times the candidate launch shapes on this stack (npx_autotune) and keeps the fastest for its (H, W, M, R).
+    Returns (max_waves_per_block, target_blocks, ms per call)."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    M = int(magnification)
+    mask, outs = 0, {}
+    for m in modes:
+        if m not in MODE_BITS:
+            raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+        mask |= MODE_BITS[m]
+        outs[m] = torch.empty((H * M, W * M), dtype=torch.float32, device=x.device)
+    nbytes = int(lib().npx_autotune_workspace(N, H, W, M, float(radius)))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=x.device) if nbytes else None
+    ptr = lambda k: outs[k].data_ptr() if k in outs else None
+    best = ctypes.c_double(0)
+    _check(lib().npx_autotune(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
+                              int(bool(iw)), mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"),
+                              work.data_ptr() if work is not None else None, nbytes, _stream(torch, stream),
+                              ctypes.byref(best)))
+    mw, tb, _ = tune_get(H, W, M, radius)
+    return mw, tb, best.value
+
+
+def tune_set(H, W, M, R, max_waves_per_block: int, target_blocks: int) -> None:
+    """Stores a launch shape for (H, W, M, R) (0, 0 forgets it)."""
+    _check(lib().npx_tune_set(int(H), int(W), int(M), float(R), int(max_waves_per_block), int(target_blocks)))
+
+
+def tune_get(H, W, M, R):
+    """(max_waves_per_block, target_blocks, tuned?) that calls with these parameters use."""
+    cfg = (ctypes.c_int * 2)()
+    rc = lib().npx_tune_get(int(H), int(W), int(M), float(R), cfg)
+    if rc < 0:
+        _check(rc)
+    return cfg[0], cfg[1], bool(rc)
 
 
 def set_launch(max_waves_per_block: int = 0, target_blocks: int = 0) -> None:

@@ -1,0 +1,65 @@
+"""Launch-shape autotuning with a persistent store.
+
+``tune(stack, M, R, S, iw, modes)`` runs the library's autotuner (npx_autotune: every candidate shape timed on the
+given stack) and records the winner for the stack's (H, W, M, R) in a JSON store; ``load()`` hands the stored shapes
+back to the library in a new process. The store is ``$NANOPYX_AMD_TUNE_FILE`` or
+``~/.cache/nanopyx_amd/launch_tuning.json``; entries are keyed by the device name as well, since the best shape is a
+property of the GPU. It plays the part of the Liquid Engine's per-run-type benchmark store (the reference times run
+types and lets its Agent pick the fastest; here the run type is fixed and the launch shape is what gets timed).
+"""
+from __future__ import annotations
+
+import json
+import os
+
+from . import _hip
+
+
+def store_path() -> str:
+    return os.environ.get("NANOPYX_AMD_TUNE_FILE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "nanopyx_amd", "launch_tuning.json"))
+
+
+def _key(device, H, W, M, R):
+    return f"{device}|{int(H)}|{int(W)}|{int(M)}|{float(R)!r}"
+
+
+def _read(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(stack, magnification, radius, sensitivity=1, iw=True, modes=("AVG",), persist=True):
+    """Autotunes the launch shape for this stack's (H, W, M, R); returns {"max_waves", "target_blocks", "ms"}."""
+    mw, tb, ms = _hip.autotune(stack, magnification, radius, sensitivity, iw, modes)
+    N, H, W = (stack.shape if len(stack.shape) == 3 else (1,) + tuple(stack.shape))
+    entry = {"max_waves": mw, "target_blocks": tb, "ms": round(ms, 4), "frames": int(N)}
+    if persist:
+        path = store_path()
+        data = _read(path)
+        data[_key(_hip.device_info().split(";")[0], H, W, magnification, radius)] = entry
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    return entry
+
+
+def load(path=None) -> int:
+    """Hands every stored shape of this device to the library; returns how many were loaded."""
+    data = _read(path or store_path())
+    dev = _hip.device_info().split(";")[0]
+    n = 0
+    for k, v in data.items():
+        parts = k.split("|")
+        if len(parts) != 5 or parts[0] != dev:
+            continue
+        _hip.tune_set(int(parts[1]), int(parts[2]), int(parts[3]), float(parts[4]), int(v["max_waves"]),
+                      int(v["target_blocks"]))
+        n += 1
+    return n

@@ -172,26 +172,42 @@ void timing_end(const CallEvents& ev) {
 
 // ------------------------------------------------------------------------------------------ launch shape
 
-// Launch-shape knobs (npx_set_launch): wavefronts per block (one output phase each) and the number of
-// blocks the temporal kernel aims for when it splits the frames into chunks.
-int g_max_waves = 16;
-int g_target_blocks = 2048;
+// Launch shape of the fused kernels: at most max_waves wavefronts (one output phase each) per block, and the
+// number of blocks the temporal kernel aims for when it splits the frames into chunks. Defaults (npx_set_launch)
+// apply unless the autotuner (npx_autotune / npx_tune_set) stored a shape for the (H, W, M, R) of the call.
+struct LaunchCfg {
+    int max_waves, target_blocks;
+};
+std::mutex g_cfg_mu;
+LaunchCfg g_default_cfg{16, 2048};
+std::map<Key, LaunchCfg> g_tuned;
 
-Shape fused_shape(int H, int W, int M) {
+LaunchCfg launch_cfg(int H, int W, int M, double R) {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    auto it = g_tuned.find(Key{H, W, M, bits(R)});
+    return it != g_tuned.end() ? it->second : g_default_cfg;
+}
+
+LaunchCfg default_cfg() {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    return g_default_cfg;
+}
+
+Shape fused_shape(int H, int W, int M, const LaunchCfg& c) {
     Shape s;
     const int nphase = M * M;
-    s.ngroups = (nphase + g_max_waves - 1) / g_max_waves;
+    s.ngroups = (nphase + c.max_waves - 1) / c.max_waves;
     s.nwaves = (nphase + s.ngroups - 1) / s.ngroups;
     s.nregx = (W + 63) / 64;
     s.nregy = (H + npx::kQTemporal - 1) / npx::kQTemporal;
     return s;
 }
 
-// Frame chunking of the temporal kernel: enough blocks to fill 256 CUs several times over, while each
-// chunk keeps >= 32 frames so the moment merge stays a small fraction of the work.
-void temporal_chunks(int N, const Shape& s, int* nchunk, int* fpc) {
+// Frame chunking of the temporal kernel: about target_blocks blocks, while each chunk keeps >= 32 frames so the
+// moment merge stays a small fraction of the work.
+void temporal_chunks(int N, const Shape& s, int target_blocks, int* nchunk, int* fpc) {
     const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
-    long long want = (g_target_blocks + spatial - 1) / spatial;
+    long long want = (target_blocks + spatial - 1) / spatial;
     want = std::min<long long>(want, std::max(1, N / 32));
     want = std::max<long long>(want, 1);
     int f = static_cast<int>((N + want - 1) / want);
@@ -350,7 +366,7 @@ int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, con
     for (const auto& j : L->jobs) spatial += static_cast<long long>(j.nregx) * j.nregy * j.ngroups;
     long long want = 1;
     if (chunks) {
-        want = (g_target_blocks + spatial - 1) / spatial;
+        want = (default_cfg().target_blocks + spatial - 1) / spatial;
         want = std::max<long long>(1, std::min<long long>(want, N / 32));
     }
     L->fpc = static_cast<int>((N + want - 1) / want);
@@ -413,7 +429,7 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     Plan* p = nullptr;
     rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
     if (rc) return rc;
-    Shape s = fused_shape(H, W, M);
+    Shape s = fused_shape(H, W, M, launch_cfg(H, W, M, R));
     s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
     // one frame per block unless the grid's z extent would overflow
@@ -433,11 +449,11 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
 }
 
 size_t npx_esrrf_temporal_workspace(int N, int H, int W, int M, double R) {
-    (void)R;
     if (N < 1 || H < 1 || W < 1 || M < 1) return 0;
-    const Shape s = fused_shape(H, W, M);
+    const LaunchCfg c = launch_cfg(H, W, M, R);
+    const Shape s = fused_shape(H, W, M, c);
     int nchunk, fpc;
-    temporal_chunks(N, s, &nchunk, &fpc);
+    temporal_chunks(N, s, c.target_blocks, &nchunk, &fpc);
     if (nchunk <= 1) return 0;
     return static_cast<size_t>(nchunk) * 5 * static_cast<size_t>(H) * M * static_cast<size_t>(W) * M * sizeof(float);
 }
@@ -454,13 +470,16 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
     Plan* p = nullptr;
     rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
     if (rc) return rc;
-    const Shape s = fused_shape(H, W, M);
+    const LaunchCfg c = launch_cfg(H, W, M, R);
+    const Shape s = fused_shape(H, W, M, c);
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
     int nchunk = 1, fpc = N;
-    const size_t need = npx_esrrf_temporal_workspace(N, H, W, M, R);
-    if (d_work && need > 0) {
+    if (d_work) {
+        temporal_chunks(N, s, c.target_blocks, &nchunk, &fpc);
+        const size_t need = nchunk > 1 ? static_cast<size_t>(nchunk) * 5 * static_cast<size_t>(p->host.HM) *
+                                             p->host.WM * sizeof(float)
+                                       : 0;
         if (work_bytes < need) return fail(NPX_ERR_WORKSPACE, "workspace smaller than npx_esrrf_temporal_workspace()");
-        temporal_chunks(N, s, &nchunk, &fpc);
     }
     a.fpc = fpc;
     a.nchunk = nchunk;
@@ -659,7 +678,7 @@ int npx_plan_info(int H, int W, int M, double R, int* info, float* finfo) {
     Plan* p = get_plan(H, W, M, R);
     if (!p) return NPX_ERR_UNSUPPORTED;
     const npx::PlanHost& h = p->host;
-    const Shape s = fused_shape(H, W, M);
+    const Shape s = fused_shape(H, W, M, launch_cfg(H, W, M, R));
     const int K = h.K;
     const int HG = K / 2 + 3, HI = HG + 1;
     const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::kQFrames, npx::kQTemporal, h.ring_stride, HG, HI,
@@ -695,11 +714,116 @@ int npx_plan_tables(int H, int W, int M, double R, float* ring, float* wsum, int
 }
 
 int npx_set_launch(int max_waves_per_block, int target_blocks) {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
     if (max_waves_per_block < 0 || max_waves_per_block > 16 || target_blocks < 0)
         return fail(NPX_ERR_ARG, "max_waves_per_block must be in [1, 16] (0 = default), target_blocks >= 0");
-    g_max_waves = max_waves_per_block ? max_waves_per_block : 16;
-    g_target_blocks = target_blocks ? target_blocks : 2048;
+    g_default_cfg.max_waves = max_waves_per_block ? max_waves_per_block : 16;
+    g_default_cfg.target_blocks = target_blocks ? target_blocks : 2048;
+    return NPX_OK;
+}
+
+int npx_tune_set(int H, int W, int M, double R, int max_waves_per_block, int target_blocks) {
+    if (max_waves_per_block < 0 || max_waves_per_block > 16 || target_blocks < 0)
+        return fail(NPX_ERR_ARG, "max_waves_per_block must be in [1, 16], target_blocks >= 1 (0, 0 = forget)");
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    const Key k{H, W, M, bits(R)};
+    if (max_waves_per_block == 0 || target_blocks == 0) g_tuned.erase(k);
+    else g_tuned[k] = LaunchCfg{max_waves_per_block, target_blocks};
+    return NPX_OK;
+}
+
+int npx_tune_get(int H, int W, int M, double R, int* cfg) {
+    if (!cfg) return fail(NPX_ERR_ARG, "NULL output");
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    auto it = g_tuned.find(Key{H, W, M, bits(R)});
+    const LaunchCfg c = it != g_tuned.end() ? it->second : g_default_cfg;
+    cfg[0] = c.max_waves;
+    cfg[1] = c.target_blocks;
+    return it != g_tuned.end() ? 1 : 0;
+}
+
+// Candidate launch shapes of the autotuner.
+static const int kTuneWaves[] = {16, 8, 4};
+static const int kTuneBlocks[] = {256, 512, 1024, 2048, 4096, 8192};
+
+size_t npx_autotune_workspace(int N, int H, int W, int M, double R) {
+    (void)R;
+    if (N < 1 || H < 1 || W < 1 || M < 1) return 0;
+    size_t most = 0;
+    for (int mw : kTuneWaves)
+        for (int tb : kTuneBlocks) {
+            const Shape s = fused_shape(H, W, M, LaunchCfg{mw, tb});
+            int nchunk, fpc;
+            temporal_chunks(N, s, tb, &nchunk, &fpc);
+            if (nchunk > 1)
+                most = std::max(most, static_cast<size_t>(nchunk) * 5 * static_cast<size_t>(H) * M *
+                                          static_cast<size_t>(W) * M * sizeof(float));
+        }
+    return most;
+}
+
+int npx_autotune(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                 unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2, void* d_work, size_t work_bytes,
+                 void* stream, double* best_ms) {
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune cannot run under stream capture");
+    const Key key{H, W, M, bits(R)};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate(autotune)");
+    LaunchCfg best{0, 0};
+    float best_t = 0.f;
+    int rc = NPX_OK;
+    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks) already timed
+    for (int mw : kTuneWaves) {
+        for (int tb : kTuneBlocks) {
+            const LaunchCfg c{mw, tb};
+            const Shape s = fused_shape(H, W, M, c);
+            int nchunk = 1, fpc = N;
+            if (d_work) temporal_chunks(N, s, tb, &nchunk, &fpc);
+            const size_t need = nchunk > 1 ? static_cast<size_t>(nchunk) * 5 * static_cast<size_t>(H) * M *
+                                                 static_cast<size_t>(W) * M * sizeof(float)
+                                           : 0;
+            if (need > work_bytes && nchunk > 1) continue;
+            const std::pair<int, int> id{s.nwaves, nchunk};
+            if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
+            seen.push_back(id);
+            {
+                std::lock_guard<std::mutex> lk(g_cfg_mu);
+                g_tuned[key] = c;
+            }
+            float t = 0.f;
+            for (int rep = 0; rep < 3 && rc == NPX_OK; ++rep) {  // one warm-up, then the best of two
+                e = hipEventRecord(e0, st);
+                if (e != hipSuccess) { rc = hip_fail(e, "hipEventRecord(autotune)"); break; }
+                rc = npx_esrrf_temporal(d_in, dtype, N, H, W, M, R, S, iw, mode_mask, d_avg, d_var, d_tac2, d_work,
+                                        work_bytes, stream);
+                if (rc) break;
+                e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) { rc = hip_fail(e, "autotune timing"); break; }
+                if (rep == 1 || (rep == 2 && ms < t)) t = ms;
+            }
+            if (rc) break;
+            if (best.max_waves == 0 || t < best_t) {
+                best = c;
+                best_t = t;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    if (rc != NPX_OK || best.max_waves == 0) {
+        g_tuned.erase(key);
+        return rc != NPX_OK ? rc : fail(NPX_ERR_ARG, "no launch shape fits the workspace");
+    }
+    g_tuned[key] = best;
+    if (best_ms) *best_ms = best_t;
     return NPX_OK;
 }
 

@@ -83,3 +83,22 @@ def test_lattice_taps_are_catmull_rom_at_half_integers():
             assert t["lrb"][a] == base and np.allclose(w, exp)
         else:                # n + 1/2: the pixel itself
             assert t["lrb"][a] == (a - 1) // 2 - 1 and np.allclose(w, [0, 1, 0, 0])
+
+
+def test_tune_table_host_only():
+    """npx_tune_set / npx_tune_get and the launch shape npx_plan_info reports (no GPU calls)."""
+    from nanopyx_amd import _hip
+    H, W, M, R = 48, 80, 4, 2.0
+    _hip.tune_set(H, W, M, R, 0, 0)
+    mw, tb, tuned = _hip.tune_get(H, W, M, R)
+    assert not tuned and mw == 16 and tb == 2048
+    assert _hip.plan_info(H, W, M, R)["nwaves"] == 16
+    _hip.tune_set(H, W, M, R, 4, 512)
+    assert _hip.tune_get(H, W, M, R) == (4, 512, True)
+    d = _hip.plan_info(H, W, M, R)
+    assert d["nwaves"] == 4 and d["ngroups"] == 4
+    assert _hip.tune_get(H, W, M, 2.5)[2] is False  # keyed by radius too
+    _hip.tune_set(H, W, M, R, 0, 0)
+    assert _hip.tune_get(H, W, M, R)[2] is False
+    with pytest.raises(ValueError):
+        _hip.tune_set(H, W, M, R, 17, 512)

@@ -203,3 +203,26 @@ def test_errors(torch):
         eSRRF_ST(verbose=False).run(np.ones((1, 2, 8, 8), np.float32))
     with pytest.raises(ValueError):
         eSRRF_ST(verbose=False).run(x, run_type="Threaded")
+
+
+def test_autotune_shape_and_store(torch, tmp_path, monkeypatch):
+    """The autotuner picks a shape, results under it equal the default shape's (within the chunking tolerance),
+    and the store hands it back after a reset."""
+    from nanopyx_amd import _hip, autotune
+    monkeypatch.setenv("NANOPYX_AMD_TUNE_FILE", str(tmp_path / "tune.json"))
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy((rng.random((96, 40, 72)) * 100).astype(np.float32)).cuda()
+    M, R, S = 3, 1.5, 2
+    _hip.tune_set(40, 72, M, R, 0, 0)
+    ref = _hip.esrrf_temporal(x, M, R, S, True, ("AVG", "TAC2"))
+    entry = autotune.tune(x, M, R, S, True, ("AVG", "TAC2"))
+    print("autotune", entry, flush=True)
+    assert entry["max_waves"] in (4, 8, 16) and entry["ms"] > 0
+    assert _hip.tune_get(40, 72, M, R)[2]
+    got = _hip.esrrf_temporal(x, M, R, S, True, ("AVG", "TAC2"))
+    for m in ("AVG", "TAC2"):
+        assert _report(f"tuned vs default {m}", got[m].cpu().numpy(), ref[m].cpu().numpy()) <= 1e-5
+    _hip.tune_set(40, 72, M, R, 0, 0)
+    assert autotune.load() == 1
+    assert _hip.tune_get(40, 72, M, R)[:2] == (entry["max_waves"], entry["target_blocks"])
+    _hip.tune_set(40, 72, M, R, 0, 0)
