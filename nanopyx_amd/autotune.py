@@ -3,8 +3,8 @@
 ``tune(stack, M, R, S, iw, modes)`` runs the library's autotuner (npx_autotune: every candidate shape timed on the
 given stack) and records the winner for the stack's (H, W, M, R) in a JSON store; ``load()`` hands the stored shapes
 back to the library in a new process. The store is ``$NANOPYX_AMD_TUNE_FILE`` or
-``~/.cache/nanopyx_amd/launch_tuning.json``; entries are keyed by the device name as well, since the best shape is a
-property of the GPU. It plays the part of the Liquid Engine's per-run-type benchmark store (the reference times run
+``~/.cache/nanopyx_amd/launch_tuning.json``; entries are keyed by the device (architecture and CU count) as well,
+since the best shape is a property of the GPU. It plays the part of the Liquid Engine's per-run-type benchmark store (the reference times run
 types and lets its Agent pick the fastest; here the run type is fixed and the launch shape is what gets timed).
 """
 from __future__ import annotations
@@ -18,6 +18,12 @@ from . import _hip
 def store_path() -> str:
     return os.environ.get("NANOPYX_AMD_TUNE_FILE",
                           os.path.join(os.path.expanduser("~"), ".cache", "nanopyx_amd", "launch_tuning.json"))
+
+
+def device_id() -> str:
+    """Architecture and CU count of the current device, e.g. "gfx950/256" (the product name HIP reports varies)."""
+    name, arch, cus = _hip.device_info().split(";")[:3]
+    return f"{arch.split(':')[0]}/{cus}"
 
 
 def _key(device, H, W, M, R):
@@ -41,7 +47,7 @@ def tune(stack, magnification, radius, sensitivity=1, iw=True, modes=("AVG",), p
     if persist:
         path = store_path()
         data = _read(path)
-        data[_key(_hip.device_info().split(";")[0], H, W, magnification, radius)] = entry
+        data[_key(device_id(), H, W, magnification, radius)] = entry
         os.makedirs(os.path.dirname(path), exist_ok=True)
         tmp = path + ".tmp"
         with open(tmp, "w") as f:
@@ -53,7 +59,7 @@ def tune(stack, magnification, radius, sensitivity=1, iw=True, modes=("AVG",), p
 def load(path=None) -> int:
     """Hands every stored shape of this device to the library; returns how many were loaded."""
     data = _read(path or store_path())
-    dev = _hip.device_info().split(";")[0]
+    dev = device_id()
     n = 0
     for k, v in data.items():
         parts = k.split("|")
