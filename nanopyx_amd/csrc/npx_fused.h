@@ -78,70 +78,103 @@ struct Lds {
     static constexpr int LCB = LCW + al(16 * G::LB);             // int [LB]
     static constexpr int CRW = LCB + al(4 * G::LB);              // float4 [16*Q]
     static constexpr int CRB = CRW + al(16 * 16 * Q);            // int [16*Q]
-    static constexpr int RING = CRB + al(4 * 16 * Q);            // float [nwaves*RN], then per-pixel state
+    static constexpr int RING = CRB + al(4 * 16 * Q);            // float [nwaves*RW], then per-pixel state
     static_assert(2 * G::TGR * G::TGC <= 4 * G::LR * G::LP, "gradient tile must fit in the lattice storage");
 };
 
 template <int K, int Q>
 inline size_t lds_bytes_kq(int nwaves, int mode) {
     using L = Lds<K, Q>;
-    size_t b = L::RING + static_cast<size_t>(nwaves) * Geometry<K, Q>::RN * 4;
+    size_t b = L::RING + static_cast<size_t>(nwaves) * Geometry<K, Q>::RW * 4;
     if (mode == kModeTemporal || mode == kModeTemporalVar) b += 2 * static_cast<size_t>(nwaves) * Q * 64 * 4;
     return b;
+}
+
+#ifndef NPX_RING_GROUPS
+#define NPX_RING_GROUPS 1
+#endif
+
+// Window rows [J0, J1) of the ring sums for column i: constants of those rows in VGPRs, lattice rows streamed.
+template <int K, int KL, int Q, int J0, int J1>
+__device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, float dxi, const float* rl, int i) {
+    using GL = Geometry<KL, Q>;
+    constexpr int NW = 2 * K + 1;
+    constexpr int LP = GL::LP;
+    constexpr int NJ = J1 - J0;
+    constexpr int R0 = J0, R1 = 2 * (Q - 1) + J1;  // lattice rows these window rows read: [R0, R1)
+    constexpr int RDT = 2 * NW;
+    float dyv[NJ], dv[NJ], tv[NJ];
+    const float2* dt = reinterpret_cast<const float2*>(rl + RDT) + i * NW + J0;
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj) {
+        dyv[jj] = rl[J0 + jj];
+        const float2 e = dt[jj];
+        dv[jj] = e.x;
+        tv[jj] = e.y;
+    }
+    // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
+    float2 g1 = col[R0 * LP];
+    float2 g2 = col[(R0 + 1 < R1 ? R0 + 1 : R0) * LP];
+#pragma unroll
+    for (int r = R0; r < R1; ++r) {
+        const float2 g = g1;
+        g1 = g2;
+        if (r + 2 < R1) g2 = col[(r + 2) * LP];
+        const float px_ = g.x * dxi;
+        const float py_ = g.y * dxi;
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            const int rq = r - (J0 + jj);
+            if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
+            const int q = rq >> 1;
+            const float u = fmaf(g.y, dyv[jj], px_);
+            const float v = fmaf(-g.x, dyv[jj], py_);
+            const float m = __builtin_amdgcn_fmed3f(0.f, -u, dv[jj] - fabsf(v));
+            acc[q] = fmaf(tv[jj], m, acc[q]);
+        }
+    }
 }
 
 // Ring sums of one wave (radius half-width K) over a lattice built for half-width KL >= K. Sample (j, i) of
 // pixel q reads lattice row 2q + ey + j + (KL - K), column 2*lane + ex + i + (KL - K). With u = g.(dx, dy)
 // and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's contribution
 // [G.d < 0] * w * (1 - |G x d| / (|G| d)) is t * med3(0, -u, d - |v|). g.x*dx and g.y*dx are formed once per
-// lattice point of the column and shared by the samples that read it.
+// lattice point of the column and shared by the samples that read it. The ring constants come from the wave's
+// LDS copy, so they sit in VGPRs (on gfx950 a VALU operation with an SGPR operand issues at about half the rate
+// of one whose operands are all VGPRs); the window rows are taken in NPX_RING_GROUPS groups so that only one
+// group's constants are live at a time.
 template <int K, int KL, int Q>
 __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
-                                         cfloat* rg) {
+                                         const float* rl) {
     using GL = Geometry<KL, Q>;
     constexpr int NW = 2 * K + 1;
-    constexpr int NR = 2 * Q + 2 * K - 1;
     constexpr int LP = GL::LP, LPL = GL::LR * GL::LP;
     constexpr int DK = KL - K;
-    constexpr int RDX = NW, RDT = 2 * NW;
+    constexpr int RDX = NW;
+    constexpr int NG = NPX_RING_GROUPS < NW ? NPX_RING_GROUPS : NW;
+    constexpr int JA = (NW + NG - 1) / NG;  // rows per group (the last group may be shorter)
     static_assert(K <= KL, "ring wider than the lattice");
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0.f;
-    float dyv[NW];
-#pragma unroll
-    for (int jj = 0; jj < NW; ++jj) dyv[jj] = rg[jj];
 #pragma unroll 1
     for (int i = 0; i < NW; ++i) {
         const int c = ex + i + DK;
         const float2* col = s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
-        const float dxi = rg[RDX + i];
-        cfloat* dt = rg + RDT + 2 * i * NW;
-        float dv[NW], tv[NW];
-#pragma unroll
-        for (int jj = 0; jj < NW; ++jj) {
-            dv[jj] = dt[2 * jj];
-            tv[jj] = dt[2 * jj + 1];
+        const float dxi = rl[RDX + i];
+        // scheduling barriers keep each group's loads inside the group (no hoisting that would keep every
+        // group's constants live at once)
+        ring_rows<K, KL, Q, 0, (JA < NW ? JA : NW)>(acc, col, dxi, rl, i);
+        if constexpr (JA < NW) {
+            __builtin_amdgcn_sched_barrier(0);
+            ring_rows<K, KL, Q, JA, (2 * JA < NW ? 2 * JA : NW)>(acc, col, dxi, rl, i);
         }
-        // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
-        float2 g1 = col[0];
-        float2 g2 = col[LP];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const float2 g = g1;
-            g1 = g2;
-            if (r + 2 < NR) g2 = col[(r + 2) * LP];
-            const float px_ = g.x * dxi;
-            const float py_ = g.y * dxi;
-#pragma unroll
-            for (int jj = 0; jj < NW; ++jj) {
-                const int rq = r - jj;
-                if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
-                const int q = rq >> 1;
-                const float u = fmaf(g.y, dyv[jj], px_);
-                const float v = fmaf(-g.x, dyv[jj], py_);
-                const float m = __builtin_amdgcn_fmed3f(0.f, -u, dv[jj] - fabsf(v));
-                acc[q] = fmaf(tv[jj], m, acc[q]);
-            }
+        if constexpr (2 * JA < NW) {
+            __builtin_amdgcn_sched_barrier(0);
+            ring_rows<K, KL, Q, 2 * JA, (3 * JA < NW ? 3 * JA : NW)>(acc, col, dxi, rl, i);
+        }
+        if constexpr (3 * JA < NW) {
+            __builtin_amdgcn_sched_barrier(0);
+            ring_rows<K, KL, Q, 3 * JA, NW>(acc, col, dxi, rl, i);
         }
     }
 }
@@ -149,7 +182,7 @@ __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, i
 // Dispatch on the wave's (uniform) half-width for lattices of half-width KL.
 template <int KL, int Q>
 __device__ __forceinline__ void ring_sum_any(int K, float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
-                                             cfloat* rg) {
+                                             const float* rg) {
     switch (K) {
 #define NPX_RING_CASE(k)                                                             \
     case k:                                                                          \
@@ -178,7 +211,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                                             unsigned char* smem) {
     using G = Geometry<KL, Q>;
     using L = Lds<KL, Q>;
-    constexpr int RNL = G::RN;
+    constexpr int RCL = G::RC, RWL = G::RW;
     constexpr int HG = G::HG, HI = G::HI;
     constexpr int TIR = G::TIR, TIC = G::TIC;
     constexpr int TGR = G::TGR, TGC = G::TGC;
@@ -227,16 +260,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     const int NWw = 2 * Kw + 1;
     const int margin = active ? a.margin[rad] : 0;
 
-    // ---- stage the frame-invariant tables. Per wave: the normaliser part of its ring block (wsum, then the
-    // (NW+1)^2 prefix sums of w); the ring constants themselves are read through scalar loads.
-    for (int k = tid; k < a.nwaves * RNL; k += nthr) {
-        const int w = k / RNL, e = k - w * RNL;
+    // ---- stage the frame-invariant tables. Per wave (block of RWL floats): the ring constants of its radius and
+    // phase (dy[j], dx[i], (d, t)[i][j]), then the normaliser part of its ring block (wsum, then the (NW+1)^2
+    // prefix sums of w).
+    for (int k = tid; k < a.nwaves * RWL; k += nthr) {
+        const int w = k / RWL, e = k - w * RWL;
         const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
         float v = 0.f;
         if (rw < a.nrad) {
             const int nw = 2 * a.K[rw] + 1;
-            if (e < 1 + (nw + 1) * (nw + 1))
-                v = a.ring[rw][static_cast<size_t>(pw) * a.rs[rw] + 2 * nw + 3 * nw * nw + e];
+            const float* blk = a.ring[rw] + static_cast<size_t>(pw) * a.rs[rw];
+            if (e < RCL) {
+                if (e < 2 * nw + 2 * nw * nw) v = blk[e];
+            } else if (e - RCL < 1 + (nw + 1) * (nw + 1)) {
+                v = blk[2 * nw + 3 * nw * nw + (e - RCL)];
+            }
         }
         s_ring[k] = v;
     }
@@ -284,7 +322,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 #pragma unroll
     for (int q = 0; q < (TEMPORAL ? Q : 1); ++q) cc[q] = 0.f;
     // first and previous frame's value of every pixel of the wave: [wave][q][lane] and [nwaves + wave][q][lane]
-    float* const s_first = s_ring + a.nwaves * RNL + (wave * Q) * 64 + lane;
+    float* const s_first = s_ring + a.nwaves * RWL + (wave * Q) * 64 + lane;
     float* const s_prev = s_first + a.nwaves * Q * 64;
 
     // frame slices of a served job resume the moments folded so far
@@ -317,8 +355,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             pf[k] = load_px(a, f_begin, y0 - HI + r, x0 - HI + c);
         }
     }
-    const float* ringL = s_ring + wave * RNL;
-    cfloat* rg = (cfloat*)(a.ring[active ? rad : 0] + static_cast<size_t>(ph) * a.rs[active ? rad : 0]);
+    const float* ringC = s_ring + wave * RWL;  // ring constants of this wave
+    const float* ringL = ringC + RCL;          // wsum, prefix sums
 
     for (int f = f_begin; f < f_end; ++f) {
         // (1) input tile + halo of frame f (zero outside the image; uint16 converted as a plain cast)
@@ -411,8 +449,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         if (active) {
             // (4) ring sums
             float acc[Q];
-            if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, rg);
-            else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, rg);
+            if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, ringC);
+            else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, ringC);
             // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Branch-free per
             // pixel: pixels whose window crosses the border take their normaliser from the prefix-sum table.
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};

@@ -30,7 +30,9 @@ struct Geometry {
     static constexpr int NW = 2 * K + 1;
     static constexpr int LB = 128 + 2 * K;        // lattice columns per block
     static constexpr int RS = ((2 * NW + 3 * NW * NW + 1 + (NW + 1) * (NW + 1)) + 15) / 16 * 16;  // ring block floats
-    static constexpr int RN = ((1 + (NW + 1) * (NW + 1)) + 3) / 4 * 4;  // LDS floats per wave: wsum + prefix sums
+    static constexpr int RC = ((2 * NW + 2 * NW * NW) + 3) / 4 * 4;    // LDS floats per wave: dy, dx, (d, t) pairs
+    static constexpr int RN = ((1 + (NW + 1) * (NW + 1)) + 3) / 4 * 4;  // ... then wsum + prefix sums
+    static constexpr int RW = RC + RN;
 };
 
 constexpr int kLdsLimit = 160 * 1024;  // LDS bytes one workgroup may use on gfx950
