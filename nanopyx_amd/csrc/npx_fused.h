@@ -22,6 +22,30 @@ namespace npx {
 
 using cfloat = const __attribute__((address_space(4))) float;  // constant address space: scalar loads
 
+// Output-phase rows among the wave slots of one block: at most kRowSets distinct phase rows py (16 consecutive
+// phases of one radius span at most 4 rows of M phases for every M; slots of several radii repeat rows).
+constexpr int kRowSets = 4;
+
+// The distinct phase rows of group `group`'s active slots, ascending, in rows[0..n); returns n.
+__device__ __forceinline__ int group_rows(const FusedArgs& a, int group, int (&rows)[kRowSets]) {
+    int n = 0;
+    for (int w = 0; w < a.nwaves; ++w) {
+        const int sw = group * a.nwaves + w, rw = sw / a.nphase;
+        if (rw >= a.nrad) break;
+        int py = (sw - rw * a.nphase) / a.M;
+        bool seen = false;
+        for (int k = 0; k < n; ++k) seen = seen || rows[k] == py;
+        if (seen || n == kRowSets) continue;
+        int k = n++;
+        while (k > 0 && rows[k - 1] > py) {  // insertion keeps rows ascending
+            rows[k] = rows[k - 1];
+            --k;
+        }
+        rows[k] = py;
+    }
+    return n;
+}
+
 __device__ __forceinline__ float apply_pow(float v, int pow_mode, float S) {
     // reference: if RGC >= 0 && S > 1: RGC = pow(RGC, S) (:83-87); RGC here is never negative. Integer
     // sensitivities are exact products; others use 2^(S log2 v) on the hardware log2/exp2 (a few ulp).
@@ -68,17 +92,19 @@ template <int K, int Q>
 struct Lds {
     using G = Geometry<K, Q>;
     static constexpr int al(int b) { return (b + 15) / 16 * 16; }
+    // lattice row pass (RX, RY: LR*TGC floats each), later in the frame the intensity row pass (kRowSets*Q*TIC)
+    static constexpr int NXY = 2 * G::LR * G::TGC > kRowSets * Q * G::TIC ? 2 * G::LR * G::TGC
+                                                                             : kRowSets * Q * G::TIC;
     static constexpr int IN = 0;                                 // float [TIR*TIC]
-    static constexpr int RX = IN + al(4 * G::TIR * G::TIC);      // float [LR*TGC]
-    static constexpr int RY = RX + al(4 * G::LR * G::TGC);       // float [LR*TGC]
-    static constexpr int LAT = RY + al(4 * G::LR * G::TGC);      // float2 [2*LR*LP]
+    static constexpr int RX = IN + al(4 * G::TIR * G::TIC);      // float [NXY]: RX, RY / intensity rows
+    static constexpr int LAT = RX + al(4 * NXY);                 // float2 [2*LR*LP]
     static constexpr int LRW = LAT + al(8 * 2 * G::LR * G::LP);  // float4 [LR]
     static constexpr int LRB = LRW + al(16 * G::LR);             // int [LR]
     static constexpr int LCW = LRB + al(4 * G::LR);              // float4 [LB]
     static constexpr int LCB = LCW + al(16 * G::LB);             // int [LB]
-    static constexpr int CRW = LCB + al(4 * G::LB);              // float4 [16*Q]
-    static constexpr int CRB = CRW + al(16 * 16 * Q);            // int [16*Q]
-    static constexpr int RING = CRB + al(4 * 16 * Q);            // float [nwaves*RW], then per-pixel state
+    static constexpr int CRW = LCB + al(4 * G::LB);              // float4 [kRowSets*Q] intensity row taps
+    static constexpr int CRB = CRW + al(16 * kRowSets * Q);      // int [kRowSets*Q]
+    static constexpr int RING = CRB + al(4 * kRowSets * Q);      // float [nwaves*RW], then per-pixel state
     static_assert(2 * G::TGR * G::TGC <= 4 * G::LR * G::LP, "gradient tile must fit in the lattice storage");
 };
 
@@ -223,7 +249,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 
     float* const s_in = reinterpret_cast<float*>(smem + L::IN);
     float* const s_rx = reinterpret_cast<float*>(smem + L::RX);
-    float* const s_ry = reinterpret_cast<float*>(smem + L::RY);
+    float* const s_ry = s_rx + LR * TGC;
+    float* const s_irow = s_rx;  // intensity row pass [set][q][TIC], written after the lattice is built
     float2* const s_lat = reinterpret_cast<float2*>(smem + L::LAT);
     float* const s_gc = reinterpret_cast<float*>(s_lat);  // Roberts gradients share the lattice storage
     float* const s_gr = s_gc + TGR * TGC;
@@ -302,13 +329,17 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         s_lcw[lb] = w;
         s_lcb[lb] = b;
     }
-    for (int k = tid; k < a.nwaves * Q; k += nthr) {
-        const int w = k / Q, q = k - w * Q;
-        const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
-        const int rM = min(M * (y0 + q) + (rw < a.nrad ? pw / M : 0), a.HM - 1);
+    // intensity Catmull-Rom row taps of every (phase row set, q); the wave's set is the index of its phase row
+    int rows[kRowSets];
+    const int nrows = group_rows(a, group, rows);
+    for (int k = tid; k < nrows * Q; k += nthr) {
+        const int rs = k / Q, q = k - rs * Q;
+        const int rM = min(M * (y0 + q) + rows[rs], a.HM - 1);
         s_crw[k] = a.crw[rM];
         s_crb[k] = a.crb[rM] - (y0 - HI);
     }
+    int rset = 0;
+    for (int k = 1; k < nrows; ++k) rset += rows[k] <= py ? 1 : 0;
     const int cMs = min(cM, a.WM - 1);
     const float4 wcol = a.ccw[cMs];
     const int ccol = a.ccb[cMs] - (x0 - HI);
@@ -445,6 +476,22 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             s_lat[idx] = g;
         }
         __syncthreads();
+        // intensity row pass, shared by the waves of one phase row: T[set][q][c] = sum_u w_u * tile[b + u][c]
+        // (the 4 x 4 taps lie inside the tile for every lane: tap base floor(r - 0.5) - 1 with r in [y0, y0 + Q),
+        // halo HI >= 2)
+        if (a.iw) {
+            for (int idx = tid; idx < nrows * Q * TIC; idx += nthr) {
+                const int k = idx / TIC, c = idx - k * TIC;
+                const float4 wr = s_crw[k];
+                const float* col = s_in + s_crb[k] * TIC + c;
+                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
+                float pp = 0.f;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pp += col[u * TIC] * wrv[u];
+                s_irow[idx] = pp;
+            }
+            __syncthreads();
+        }
 
         if (active) {
             // (4) ring sums
@@ -457,6 +504,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             const int nabs = n0 + (f - f_begin);  // values already folded into the moments
             const float inv_np1 = 1.0f / static_cast<float>(nabs + 1);
             const float wsum = ringL[0];
+            const float wsum_inv = wsum > 0.f ? 1.0f / wsum : 0.f;  // once per frame; pixels multiply by it
             const float* P = ringL + 1;
             const int b0 = 2 * xi + ex;
             const int i0 = max(0, 1 - (b0 - Kw));
@@ -470,30 +518,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 const int j0 = max(0, 1 - (a0 - Kw));
                 const int j1 = max(0, min(NWw, 2 * a.HM - 2 - (a0 - Kw) + 1));
                 const bool row_ok = yi < a.H && rM >= margin && rM < a.HM - margin;  // uniform
-                float den = wsum;
+                // RGC / sum(w) (:77-81) as RGC * (1 / sum(w)): within 2 ulp of the division
+                float den_inv = wsum_inv;
                 if (j0 > 0 || j1 < NWw || i0 > 0 || i1 < NWw) {
                     const int jl = min(j0, j1), il = min(i0, i1);
-                    den = (P[j1 * (NWw + 1) + i1] - P[jl * (NWw + 1) + i1]) -
-                          (P[j1 * (NWw + 1) + il] - P[jl * (NWw + 1) + il]);
+                    const float den = (P[j1 * (NWw + 1) + i1] - P[jl * (NWw + 1) + i1]) -
+                                      (P[j1 * (NWw + 1) + il] - P[jl * (NWw + 1) + il]);
+                    den_inv = den > 0.f ? __builtin_amdgcn_rcpf(den) : 0.f;
                 }
-                const float rgc0 = den > 0.f ? acc[q] / den : 0.f;
+                const float rgc0 = acc[q] * den_inv;
                 float inten = 1.f;
                 if (a.iw) {
-                    const float4 wrow = s_crw[wave * Q + q];
-                    const int crow = s_crb[wave * Q + q];
-                    const float wrv[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
+                    const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
                     float sacc = 0.f;
 #pragma unroll
-                    for (int v2 = 0; v2 < 4; ++v2) {
-                        const int tc = min(max(ccol + v2, 0), TIC - 1);
-                        float pp = 0.f;
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int tr = min(max(crow + u, 0), TIR - 1);
-                            pp += s_in[tr * TIC + tc] * wrv[u];
-                        }
-                        sacc += pp * wcv[v2];
-                    }
+                    for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
                     inten = sacc;
                 }
                 const bool ok = row_ok && col_ok;

@@ -197,14 +197,14 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                                 crow = t["crb"][rMs] - (y0 - HI)
                                 sacc = np.zeros(64, np.float32)
                                 for v in range(4):
-                                    tc = np.clip(ccol + v, 0, TIC - 1)
-                                    bad = (wcol[:, v] != 0) & ((ccol + v < 0) | (ccol + v >= TIC)) & valid
-                                    assert not bad.any(), "intensity column tap outside tile"
+                                    # every tap of every lane (also lanes past the image and zero-weight taps) reads
+                                    # inside the tile: the kernel reads them without clamping
+                                    assert ((ccol + v >= 0) & (ccol + v < TIC)).all(), "intensity column tap outside tile"
+                                    tc = ccol + v
                                     p = np.zeros(64, np.float32)
                                     for u in range(4):
-                                        tr = min(max(crow + u, 0), TIR - 1)
-                                        if wrow[u] != 0 and valid.any():
-                                            assert 0 <= crow + u < TIR, "intensity row tap outside tile"
+                                        assert 0 <= crow + u < TIR, "intensity row tap outside tile"
+                                        tr = crow + u
                                         p = p + s_in[tr, tc] * wrow[u]
                                     sacc = sacc + p * wcol[:, v]
                                 inten = sacc

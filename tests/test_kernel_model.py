@@ -49,3 +49,31 @@ def test_model_shared_lattice(oracle_built, shape, M, radii, KL):
         for S in sens:
             ref = oracle_built.esrrf(x, M, R, S)
             assert rel_err(outs[(float(R), float(S))], ref)[0] <= 1e-5, (R, S)
+
+
+def _group_rows(nphase, M, nwaves, nrad, group):
+    rows = set()
+    for w in range(nwaves):
+        sw = group * nwaves + w
+        rw = sw // nphase
+        if rw >= nrad:
+            break
+        rows.add((sw - rw * nphase) // M)
+    return rows
+
+
+@pytest.mark.parametrize("M", list(range(1, 21)) + [32, 64])
+def test_phase_row_sets_fit(M):
+    """The fused block shares the intensity row pass between waves of one phase row and keeps at most 4 such rows:
+    every launch shape the library makes (single job: nwaves = ceil(M^2 / ceil(M^2 / max_waves)) for each
+    autotuner candidate; batched: 16 slots over up to 4 radii) must have at most 4 distinct phase rows per group."""
+    nphase = M * M
+    shapes = []
+    for mw in (1, 2, 4, 8, 12, 16):
+        ngroups = -(-nphase // mw)
+        shapes.append((-(-nphase // ngroups), 1, ngroups))
+    for nrad in (1, 2, 3, 4):
+        shapes.append((16, nrad, -(-(nphase * nrad) // 16)))
+    for nwaves, nrad, ngroups in shapes:
+        for g in range(ngroups):
+            assert len(_group_rows(nphase, M, nwaves, nrad, g)) <= 4, (M, nwaves, nrad, g)
