@@ -191,6 +191,26 @@ def test_workflow_timepoints(torch, oracle_built):
         assert _report(f"eSRRF() T=3 {mode}", out, ref) <= TOL
 
 
+def test_cli_tiff_round_trip(torch, oracle_built, tmp_path):
+    """The eSRRF command line: uint16 TIFF in, float32 TIFF out, equal to the workflow call and to the oracle."""
+    import nanopyx_amd
+    from nanopyx_amd.cli.superresolution import linkinpy_eSRRF_main
+    from nanopyx_amd.io import imread, imwrite
+    rng = np.random.default_rng(12)
+    x = (rng.poisson(80, size=(16, 20, 30)) + 100).astype(np.uint16)
+    imwrite(tmp_path / "in.tif", x)
+    linkinpy_eSRRF_main([str(tmp_path / "in.tif"), str(tmp_path / "out.tif"), "--magnification", "3", "--radius",
+                         "1.2", "--sensitivity", "2", "--frames-per-timepoint", "8", "--temporal-correlation", "TAC2"])
+    out = imread(tmp_path / "out.tif")
+    direct = nanopyx_amd.eSRRF(x, magnification=3, radius=1.2, sensitivity=2, frames_per_timepoint=8,
+                               temporal_correlation="TAC2")
+    assert out.dtype == np.float32 and out.shape == (2, 60, 90)
+    assert np.array_equal(out, direct)
+    ref = oracle_built.esrrf_workflow(x.astype(np.float32), 3, 1.2, 2, frames_per_timepoint=8,
+                                      temporal_correlation="TAC2")
+    assert _report("CLI eSRRF TAC2 vs oracle", out, ref) <= TOL
+
+
 def test_errors(torch):
     import nanopyx_amd
     from nanopyx_amd.core.transform import eSRRF_ST
