@@ -109,11 +109,22 @@ struct Lds {
 };
 
 template <int K, int Q>
+constexpr size_t lds_base_bytes(int nwaves, int mode) {
+    return Lds<K, Q>::RING + static_cast<size_t>(nwaves) * Geometry<K, Q>::RW * 4 +
+           ((mode == kModeTemporal || mode == kModeTemporalVar) ? 2 * static_cast<size_t>(nwaves) * Q * 64 * 4 : 0);
+}
+
+// Whether the intensity row pass gets LDS of its own (decided for the largest block, 16 waves).
+template <int K, int Q>
+constexpr bool t_own(int mode) {
+    return lds_base_bytes<K, Q>(16, mode) + static_cast<size_t>(kRowSets) * Q * Geometry<K, Q>::TIC * 4 <=
+           static_cast<size_t>(kLdsLimit);
+}
+
+template <int K, int Q>
 inline size_t lds_bytes_kq(int nwaves, int mode) {
-    using L = Lds<K, Q>;
-    size_t b = L::RING + static_cast<size_t>(nwaves) * Geometry<K, Q>::RW * 4;
-    if (mode == kModeTemporal || mode == kModeTemporalVar) b += 2 * static_cast<size_t>(nwaves) * Q * 64 * 4;
-    return b;
+    return lds_base_bytes<K, Q>(nwaves, mode) +
+           (t_own<K, Q>(mode) ? static_cast<size_t>(kRowSets) * Q * Geometry<K, Q>::TIC * 4 : 0);
 }
 
 #ifndef NPX_RING_GROUPS
@@ -250,7 +261,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     float* const s_in = reinterpret_cast<float*>(smem + L::IN);
     float* const s_rx = reinterpret_cast<float*>(smem + L::RX);
     float* const s_ry = s_rx + LR * TGC;
-    float* const s_irow = s_rx;  // intensity row pass [set][q][TIC], written after the lattice is built
+    // intensity row pass [set][q][TIC]: own storage after the per-pixel state where the LDS budget allows (then it
+    // is computed beside Roberts), else the lattice-row scratch once the lattice is built
+    constexpr bool T_OWN = t_own<KL, Q>(MODE);
+    float* const s_irow = T_OWN ? reinterpret_cast<float*>(smem + L::RING) + a.nwaves * RWL +
+                                      (TEMPORAL ? 2 * a.nwaves * Q * 64 : 0)
+                                : s_rx;
     float2* const s_lat = reinterpret_cast<float2*>(smem + L::LAT);
     float* const s_gc = reinterpret_cast<float*>(s_lat);  // Roberts gradients share the lattice storage
     float* const s_gr = s_gc + TGR * TGC;
@@ -375,22 +391,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     }
     const int n0 = TEMPORAL && a.state ? a.n0 : 0;
 
-    // input-tile elements this thread stages; the first kPrefetch are fetched one frame ahead
+    // Input tile + halo of one frame (zero outside the image; uint16 converted as a plain cast). The first
+    // kPrefetch elements of each thread come through registers loaded a frame ahead. The tile of frame f + 1 is
+    // stored while frame f's ring sums run (its last reader, the intensity row pass, is behind a barrier by then).
     float pf[kPrefetch];
+    auto fetch = [&](int f) {
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
-        const int idx = tid + k * nthr;
-        pf[k] = 0.f;
-        if (idx < TIR * TIC && f_begin < f_end) {
-            const int r = idx / TIC, c = idx - r * TIC;
-            pf[k] = load_px(a, f_begin, y0 - HI + r, x0 - HI + c);
+        for (int k = 0; k < kPrefetch; ++k) {
+            const int idx = tid + k * nthr;
+            if (idx < TIR * TIC) {
+                const int r = idx / TIC, c = idx - r * TIC;
+                pf[k] = load_px(a, f, y0 - HI + r, x0 - HI + c);
+            }
         }
-    }
-    const float* ringC = s_ring + wave * RWL;  // ring constants of this wave
-    const float* ringL = ringC + RCL;          // wsum, prefix sums
-
-    for (int f = f_begin; f < f_end; ++f) {
-        // (1) input tile + halo of frame f (zero outside the image; uint16 converted as a plain cast)
+    };
+    auto store_tile = [&](int f) {
 #pragma unroll
         for (int k = 0; k < kPrefetch; ++k) {
             const int idx = tid + k * nthr;
@@ -400,18 +415,35 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             const int r = idx / TIC, c = idx - r * TIC;
             s_in[idx] = load_px(a, f, y0 - HI + r, x0 - HI + c);
         }
-        __syncthreads();
-        // prefetch frame f + 1 while this frame is processed (consumed at the top of the next iteration)
-        if (f + 1 < f_end) {
+    };
+    if (f_begin < f_end) {
+        fetch(f_begin);
+        store_tile(f_begin);
+        if (f_begin + 1 < f_end) fetch(f_begin + 1);
+    }
+    __syncthreads();
+    const float* ringC = s_ring + wave * RWL;  // ring constants of this wave
+    const float* ringL = ringC + RCL;          // wsum, prefix sums
+
+    // intensity Catmull-Rom row pass, shared by the waves of one phase row: T[set][q][c] = sum_u w_u tile[b+u][c]
+    // (the 4 x 4 taps lie inside the tile for every lane: tap base floor(r - 0.5) - 1 with r in [y0, y0 + Q),
+    // halo HI >= 2)
+    auto intensity_rows = [&]() {
+        for (int idx = tid; idx < nrows * Q * TIC; idx += nthr) {
+            const int k = idx / TIC, c = idx - k * TIC;
+            const float4 wr = s_crw[k];
+            const float* col = s_in + s_crb[k] * TIC + c;
+            const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
+            float pp = 0.f;
 #pragma unroll
-            for (int k = 0; k < kPrefetch; ++k) {
-                const int idx = tid + k * nthr;
-                if (idx < TIR * TIC) {
-                    const int r = idx / TIC, c = idx - r * TIC;
-                    pf[k] = load_px(a, f + 1, y0 - HI + r, x0 - HI + c);
-                }
-            }
+            for (int u = 0; u < 4; ++u) pp += col[u * TIC] * wrv[u];
+            s_irow[idx] = pp;
         }
+    };
+
+    for (int f = f_begin; f < f_end; ++f) {
+        // intensity rows with their own storage are computed here, beside Roberts (both only read the tile)
+        if (T_OWN && a.iw) intensity_rows();
         // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
         for (int idx = tid; idx < TGR * TGC; idx += nthr) {
             const int r = idx / TGC, c = idx - r * TGC;
@@ -476,21 +508,15 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             s_lat[idx] = g;
         }
         __syncthreads();
-        // intensity row pass, shared by the waves of one phase row: T[set][q][c] = sum_u w_u * tile[b + u][c]
-        // (the 4 x 4 taps lie inside the tile for every lane: tap base floor(r - 0.5) - 1 with r in [y0, y0 + Q),
-        // halo HI >= 2)
-        if (a.iw) {
-            for (int idx = tid; idx < nrows * Q * TIC; idx += nthr) {
-                const int k = idx / TIC, c = idx - k * TIC;
-                const float4 wr = s_crw[k];
-                const float* col = s_in + s_crb[k] * TIC + c;
-                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
-                float pp = 0.f;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) pp += col[u * TIC] * wrv[u];
-                s_irow[idx] = pp;
-            }
+        // otherwise they go to the lattice-row scratch, free from here on
+        if (!T_OWN && a.iw) {
+            intensity_rows();
             __syncthreads();
+        }
+        // the next frame's tile: the tile of frame f has no reader left
+        if (f + 1 < f_end) {
+            store_tile(f + 1);
+            if (f + 2 < f_end) fetch(f + 2);
         }
 
         if (active) {
