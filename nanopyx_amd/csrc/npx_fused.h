@@ -59,6 +59,28 @@ __device__ __forceinline__ float apply_pow(float v, int pow_mode, float S) {
     return v;
 }
 
+// apply_pow over N values with the (wave-uniform) mode tested once.
+template <int N>
+__device__ __forceinline__ void pow_all(float (&x)[N], int pow_mode, float S) {
+    if (pow_mode == 2) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = x[k] * x[k];
+    } else if (pow_mode == 3) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = x[k] * x[k] * x[k];
+    } else if (pow_mode == 4) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const float s = x[k] * x[k];
+            x[k] = s * s;
+        }
+    } else if (pow_mode == 1) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            x[k] = x[k] > 0.f ? __builtin_amdgcn_exp2f(S * __builtin_amdgcn_logf(x[k])) : 0.f;
+    }
+}
+
 // One step of the streaming moments. State: mu (mean), m2 (sum of squared deviations), c (lag-1
 // co-moment sum_t (x_t - mu)(x_{t+1} - mu)), first and prev values; n = values already absorbed.
 template <bool WITH_M2 = true>
@@ -524,8 +546,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             float acc[Q];
             if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, ringC);
             else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, ringC);
-            // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Branch-free per
-            // pixel: pixels whose window crosses the border take their normaliser from the prefix-sum table.
+            // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Each step runs over
+            // the lane's Q pixels, so the wave-uniform decisions (power, weighting, first frame) are taken once per
+            // frame; pixels whose window crosses the border take their normaliser from the prefix-sum table.
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
             const int nabs = n0 + (f - f_begin);  // values already folded into the moments
             const float inv_np1 = 1.0f / static_cast<float>(nabs + 1);
@@ -536,15 +559,16 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             const int i0 = max(0, 1 - (b0 - Kw));
             const int i1 = max(0, min(NWw, 2 * a.WM - 2 - (b0 - Kw) + 1));  // lanes past the image: empty range
             const bool col_ok = xi < a.W && cM >= margin && cM < a.WM - margin;
+            auto pixel_ok = [&](int q) {
+                const int yi = y0 + q, rM = M * yi + py;
+                return col_ok && yi < a.H && rM >= margin && rM < a.HM - margin;
+            };
+            // (5a) RGC / sum(w) (:77-81) as RGC * (1 / sum(w)): within 2 ulp of the division
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const int yi = y0 + q;
-                const int rM = M * yi + py;
-                const int a0 = 2 * yi + ey;
+                const int a0 = 2 * (y0 + q) + ey;
                 const int j0 = max(0, 1 - (a0 - Kw));
                 const int j1 = max(0, min(NWw, 2 * a.HM - 2 - (a0 - Kw) + 1));
-                const bool row_ok = yi < a.H && rM >= margin && rM < a.HM - margin;  // uniform
-                // RGC / sum(w) (:77-81) as RGC * (1 / sum(w)): within 2 ulp of the division
                 float den_inv = wsum_inv;
                 if (j0 > 0 || j1 < NWw || i0 > 0 || i1 < NWw) {
                     const int jl = min(j0, j1), il = min(i0, i1);
@@ -552,46 +576,71 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                                       (P[j1 * (NWw + 1) + il] - P[jl * (NWw + 1) + il]);
                     den_inv = den > 0.f ? __builtin_amdgcn_rcpf(den) : 0.f;
                 }
-                const float rgc0 = acc[q] * den_inv;
-                float inten = 1.f;
-                if (a.iw) {
+                acc[q] *= den_inv;
+            }
+            if constexpr (SWEEP) {
+                // every sensitivity re-applies power, weight and its running mean to the same ring sums
+                float inten[Q];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
                     const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
                     float sacc = 0.f;
 #pragma unroll
                     for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
-                    inten = sacc;
+                    inten[q] = a.iw ? sacc : 1.f;
                 }
-                const bool ok = row_ok && col_ok;
-                if constexpr (SWEEP) {
 #pragma unroll
-                    for (int s = 0; s < NSM; ++s) {
-                        if (s < a.ns) {
-                            float val = apply_pow(rgc0, a.sens_pow[s], a.sens[s]);
-                            if (a.iw) val = val * inten;
-                            val = ok ? val : 0.f;
-                            mu[s * Q + q] += (val - mu[s * Q + q]) * inv_np1;
+                for (int s = 0; s < NSM; ++s) {
+                    if (s < a.ns) {
+                        float val[Q];
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) val[q] = acc[q];
+                        pow_all<Q>(val, a.sens_pow[s], a.sens[s]);
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) {
+                            float v = a.iw ? val[q] * inten[q] : val[q];
+                            v = pixel_ok(q) ? v : 0.f;
+                            mu[s * Q + q] += (v - mu[s * Q + q]) * inv_np1;
                         }
+                    }
+                }
+            } else {
+                pow_all<Q>(acc, a.sens_pow[0], a.sens[0]);
+                if (a.iw) {
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
+                        float sacc = 0.f;
+#pragma unroll
+                        for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
+                        acc[q] = acc[q] * sacc;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] = pixel_ok(q) ? acc[q] : 0.f;
+                if constexpr (MODE == kModeFrames) {
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const int yi = y0 + q;
+                        if (xi < a.W && yi < a.H)
+                            a.out[(static_cast<size_t>(f) * a.HM + M * yi + py) * a.WM + cM] = acc[q];
+                    }
+                } else if (nabs == 0) {
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        float fq = 0.f, pq = 0.f;
+                        moments_push<WITH_M2>(acc[q], 0, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
+                        s_first[q * 64] = acc[q];
+                        s_prev[q * 64] = acc[q];
                     }
                 } else {
-                    float val = apply_pow(rgc0, a.sens_pow[0], a.sens[0]);
-                    if (a.iw) val = val * inten;
-                    val = ok ? val : 0.f;
-                    if constexpr (MODE == kModeFrames) {
-                        if (xi < a.W && yi < a.H) a.out[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cM] = val;
-                    } else {
-                        float fq = val, pq = val;
-                        if (nabs == 0) {
-                            s_first[q * 64] = val;
-                        } else {
-                            fq = s_first[q * 64];
-                            pq = s_prev[q * 64];
-                        }
-                        moments_push<WITH_M2>(val, nabs, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
-                        s_prev[q * 64] = val;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        float fq = s_first[q * 64], pq = s_prev[q * 64];
+                        moments_push<WITH_M2>(acc[q], nabs, inv_np1, mu[q], m2[WITH_M2 ? q : 0], cc[q], fq, pq);
+                        s_prev[q * 64] = acc[q];
                     }
                 }
-                // one pixel at a time: keeps the finalise's live registers low (no interleaving across q)
-                __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();
