@@ -282,7 +282,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 
     float* const s_in = reinterpret_cast<float*>(smem + L::IN);
     float* const s_rx = reinterpret_cast<float*>(smem + L::RX);
-    float* const s_ry = s_rx + LR * TGC;
+    float2* const s_rxy = reinterpret_cast<float2*>(s_rx);  // row pass of both gradients [LR][TGC]
     // intensity row pass [set][q][TIC]: own storage after the per-pixel state where the LDS budget allows (then it
     // is computed beside Roberts), else the lattice-row scratch once the lattice is built
     constexpr bool T_OWN = t_own<KL, Q>(MODE);
@@ -290,8 +290,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                                       (TEMPORAL ? 2 * a.nwaves * Q * 64 : 0)
                                 : s_rx;
     float2* const s_lat = reinterpret_cast<float2*>(smem + L::LAT);
-    float* const s_gc = reinterpret_cast<float*>(s_lat);  // Roberts gradients share the lattice storage
-    float* const s_gr = s_gc + TGR * TGC;
+    float2* const s_g = s_lat;  // Roberts gradients (col, row) share the lattice storage
     float4* const s_lrw = reinterpret_cast<float4*>(smem + L::LRW);
     int* const s_lrb = reinterpret_cast<int*>(smem + L::LRB);
     float4* const s_lcw = reinterpret_cast<float4*>(smem + L::LCW);
@@ -482,8 +481,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 gcv = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0;
                 grv = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0;
             }
-            s_gc[idx] = gcv;
-            s_gr[idx] = grv;
+            s_g[idx] = make_float2(gcv, grv);
         }
         __syncthreads();
         // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
@@ -496,11 +494,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int tr = min(max(rb + u, 0), TGR - 1);
-                sx += s_gc[tr * TGC + c] * wrv[u];
-                sy += s_gr[tr * TGC + c] * wrv[u];
+                const float2 gv = s_g[tr * TGC + c];
+                sx += gv.x * wrv[u];
+                sy += gv.y * wrv[u];
             }
-            s_rx[idx] = sx;
-            s_ry[idx] = sy;
+            s_rxy[idx] = make_float2(sx, sy);
         }
         __syncthreads();
         // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
@@ -518,8 +516,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int tc = min(max(cb + v, 0), TGC - 1);
-                gx += s_rx[la * TGC + tc] * wcv[v];
-                gy += s_ry[la * TGC + tc] * wcv[v];
+                const float2 rv = s_rxy[la * TGC + tc];
+                gx += rv.x * wcv[v];
+                gy += rv.y * wcv[v];
             }
             const float n2 = gx * gx + gy * gy;
             float2 g = make_float2(0.f, 0.f);
