@@ -429,12 +429,16 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     Plan* p = nullptr;
     rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
     if (rc) return rc;
-    Shape s = fused_shape(H, W, M, launch_cfg(H, W, M, R));
+    const LaunchCfg c = launch_cfg(H, W, M, R);
+    Shape s = fused_shape(H, W, M, c);
     s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
-    // one frame per block unless the grid's z extent would overflow
-    const long long zmax = 65535 / s.ngroups;
-    a.fpc = static_cast<int>((N + zmax - 1) / zmax);
+    // frames split into chunks (each block streams one chunk) for about target_blocks blocks; per-frame maps need
+    // no merge, so the only cost of a chunk is the block's set-up (tables, first tile)
+    const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
+    long long want = std::max<long long>(1, (c.target_blocks + spatial - 1) / spatial);
+    want = std::min<long long>({want, N, 65535 / s.ngroups});
+    a.fpc = static_cast<int>((N + want - 1) / want);
     a.nchunk = (N + a.fpc - 1) / a.fpc;
     a.out = d_out;
     dim3 grid(s.nregx, s.nregy, s.ngroups * a.nchunk);
