@@ -135,7 +135,8 @@ def serving_bench(n, rate_hz, seed=5):
 
 def cpu_baseline(x, e, seconds):
     """The reference CPU chain (CR xM, Roberts, CR x2M twice, RGC, temporal) on the first frames of the same
-    stack, chunk after chunk until `seconds` of wall clock have passed."""
+    stack, chunk after chunk until `seconds` of wall clock have passed. Returns the baseline record and the
+    reference's per-frame maps of the frames it processed (for the parity check)."""
     from oracle import ref32
     ref32.build()
     kind = "port"
@@ -146,12 +147,14 @@ def cpu_baseline(x, e, seconds):
     N, H, W = x.shape
     chunk = 4
     done = 0
+    kept = []
     t0 = time.perf_counter()
     while done < N:
         part = x[done:done + chunk].astype(np.float32)
         maps = ref32.esrrf(part, M, R, S, iw)
         for mode in e["temporal"]:
             ref32.temporal(maps, mode)
+        kept.append(maps)
         done += part.shape[0]
         if time.perf_counter() - t0 >= seconds:
             break
@@ -169,7 +172,47 @@ def cpu_baseline(x, e, seconds):
     return {"value": round(value, 3), "unit": "Mpx·frames/s", "cores": ref32.num_threads(), "kind": kind,
             "sample": f"first {done} of {N} frames ({H}x{W}, M{M}, R{R}, S{S}, {'+'.join(e['temporal'])}), "
                       f"reference CPU chain in {dt:.1f} s; {cpu}, nproc {os.cpu_count()}, "
-                      f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}"}
+                      f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}"}, np.concatenate(kept)
+
+
+def parity_on_sample(x_dev, ref_maps, e):
+    """Max and mean of |gpu - ref| / max|ref| for each temporal map of the frames the CPU baseline processed: the
+    fused GPU path on those frames against NumPy's temporal step over the reference's per-frame maps."""
+    from oracle import ref32
+    from nanopyx_amd import _hip
+    n = ref_maps.shape[0]
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    gpu = _hip.esrrf_temporal(x_dev[:n], M, R, S, iw, tuple(e["temporal"]))
+    out = {"frames": int(n), "against": "reference CPU chain (oracle/_ref kernels) + NumPy float32 temporal"}
+    for mode in e["temporal"]:
+        ref = ref32.temporal(ref_maps, mode).astype(np.float64)
+        g = gpu[mode].cpu().numpy().astype(np.float64)
+        err = np.abs(g - ref) / max(np.abs(ref).max(), 1e-30)
+        out[mode] = {"max": float(err.max()), "mean": float(err.mean())}
+    return out
+
+
+def end_to_end(x_host, e, reps=2):
+    """Host stack -> device copy -> fused eSRRF + temporal -> maps back on the host, wall clock per call."""
+    import torch
+    from nanopyx_amd import _hip
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    pinned = torch.from_numpy(x_host).pin_memory()
+    best = None
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xd = pinned.to("cuda", non_blocking=True)
+        outs = _hip.esrrf_temporal(xd, M, R, S, iw, tuple(e["temporal"]))
+        host = {k: v.cpu() for k, v in outs.items()}
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None or dt < best else best
+        del xd, outs, host
+    N, H, W = x_host.shape
+    return {"ms": round(best * 1e3, 3), "value": round(N * H * M * W * M / 1e6 / best, 1),
+            "unit": "Mpx·frames/s", "note": "pinned host stack copied in, maps copied out, best of "
+                                            f"{reps} after one warm-up"}
 
 
 def main():
@@ -282,9 +325,14 @@ def main():
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
+    log("end to end (host stack in, maps out) ...")
+    result["end_to_end_with_pcie"] = end_to_end(x_host, e)
     if not args.no_cpu:
         log("timing the reference CPU path ...")
-        result["cpu_baseline"] = cpu_baseline(x_host, e, args.cpu_seconds)
+        result["cpu_baseline"], ref_maps = cpu_baseline(x_host, e, args.cpu_seconds)
+        result["parity"] = parity_on_sample(x, ref_maps, e)
+        log(f"parity: {result['parity']}")
+        del ref_maps
     print(json.dumps(result), flush=True)
 
 
