@@ -1,20 +1,24 @@
-"""eSRRF parameter sweep on the GPU: every (sensitivity, radius) reconstruction of one stack from one batched launch.
+"""eSRRF parameter sweep: every (sensitivity, radius) reconstruction of one stack from one batched GPU launch, scored
+by the reference's QnR.
 
-Mirrors ``ParameterSweep`` of src/nanopyx/core/analysis/parameter_sweep.py:11-97. The reference runs one full
+Mirrors ``ParameterSweep`` of src/nanopyx/core/analysis/parameter_sweep.py:11-139. The reference runs one full
 ``eSRRF().run`` per (sensitivity, radius) pair (:33-37) and then ``calculate_eSRRF_temporal_correlations`` on its
-frames (:40); here all pairs (and several magnifications) come out of one ``npx_esrrf_sweep`` launch: the
-sensitivities share the ring sums and the radii of one magnification share the gradient lattice. AVG is computed
-in-kernel; VAR and TAC2 fall back to one fused ``npx_esrrf_temporal`` call per pair (still on the GPU).
+frames (:40, :47-50); here the reconstructions of all pairs come out of ``npx_esrrf_sweep`` launches (the
+sensitivities share the ring sums, the radii of one magnification share the gradient lattice): one over all frames
+(for RSP), one over the odd and one over the even frames (for FRC), one per ``n_frames`` slice when slicing. AVG is
+computed in-kernel; VAR and TAC2 use one fused ``npx_esrrf_temporal`` call per pair (still on the GPU).
 
-The reference's scoring of the reconstructions (ErrorMap RSP, FRC / decorrelation and the QnR map, :99-139) is not
-part of this package: ``run`` raises NotImplementedError when it is requested, and ``reconstructions`` returns the
-maps the scoring would consume.
+Scoring is the reference's, restated on the host as the reference runs it: RSP from ``ErrorMap`` (:99-102), the FIRE
+number of the odd/even reconstructions (:104-107), and QnR = 2 RSP nFRC / (RSP + nFRC) with nFRC the logistic
+rescaling of the FRC map (:109-139). The decorrelation alternative (``use_decorr=True``) is not built.
 """
 from __future__ import annotations
 
 import numpy as np
 
 from ... import _hip
+from ..transform.error_map import ErrorMap
+from .frc import FIRECalculator
 
 
 class ParameterSweep:
@@ -37,8 +41,7 @@ class ParameterSweep:
         if temporal_correlation == "AVG":
             out = _hip.esrrf_sweep(x, mags, radii, sens, iw=doIntensityWeighting)
             for M in mags:
-                maps = torch.stack([torch.stack([out.maps[(M, r, s)] for r in radii]) for s in sens])
-                result[M] = maps
+                result[M] = torch.stack([torch.stack([out.maps[(M, r, s)] for r in radii]) for s in sens])
         else:
             for M in mags:
                 rows = []
@@ -52,15 +55,72 @@ class ParameterSweep:
             result = {M: v.cpu().numpy() for M, v in result.items()}
         return result
 
-    def run(self, im, magnification, sensitivity_array, radius_array, temporal_correlation="AVG",
-            use_decorr=False, n_frames=None):
-        """Reference signature (:17-26). Scoring (RSP / FRC / QnR) is not built: with doErrorMapping and
-        doFRCMapping both False the reconstructions {M: (n_sens, n_rad, H*M, W*M)} are returned instead."""
-        if self.doErrorMapping or self.doFRCMapping or use_decorr:
-            raise NotImplementedError(
-                "sweep scoring (ErrorMap RSP, FRC / decorrelation, QnR) is not part of nanopyx_amd; use "
-                "ParameterSweep(doErrorMapping=False, doFRCMapping=False).run(...) or .reconstructions(...) for the "
-                "reconstructions it would score")
-        if n_frames is not None:
-            raise NotImplementedError("n_frames slicing feeds only the RSP scoring, which is not built")
-        return self.reconstructions(im, magnification, sensitivity_array, radius_array, temporal_correlation)
+    def run(self, im: np.ndarray, magnification: int, sensitivity_array: list, radius_array: list,
+            temporal_correlation: str = "AVG", use_decorr: bool = False, n_frames=None):
+        """QnR map (n_sens, n_rad) of the reference's sweep (:17-97)."""
+        if use_decorr:
+            raise NotImplementedError("decorrelation analysis (use_decorr=True) is not part of nanopyx_amd")
+        torch = _hip._torch()
+        im = np.asarray(im)
+        if im.ndim == 2:
+            im = im[None]
+        M = int(magnification)
+        ns, nr = len(sensitivity_array), len(radius_array)
+        rsp = np.zeros((ns, nr))
+        frc = np.zeros((ns, nr))
+        x = _hip._dev_stack(im)
+
+        def rec(stack):
+            return self.reconstructions(stack, M, sensitivity_array, radius_array, temporal_correlation)[M]
+
+        if self.doErrorMapping:
+            if n_frames is None:
+                full = rec(x)
+                for s in range(ns):
+                    for r in range(nr):
+                        rsp[s, r] = self.calculate_rsp(im, full[s, r])
+            else:
+                n_slices = im.shape[0] // n_frames + (1 if im.shape[0] % n_frames else 0)
+                per = np.zeros((n_slices, ns, nr))
+                for i in range(n_slices):
+                    part = rec(x[i * n_frames:(i + 1) * n_frames])
+                    for s in range(ns):
+                        for r in range(nr):
+                            per[i, s, r] = self.calculate_rsp(im[i * n_frames:(i + 1) * n_frames], part[s, r])
+                rsp = np.mean(per, axis=0)
+        if self.doFRCMapping:
+            odd = rec(x[1::2].contiguous())
+            even = rec(x[::2].contiguous())
+            for s in range(ns):
+                for r in range(nr):
+                    frc[s, r] = self.calculate_frc(odd[s, r], even[s, r])
+        torch.cuda.synchronize()
+        return self.calculate_qnr_score(rsp, frc)
+
+    def calculate_rsp(self, im, reconstruction):
+        """RSP of a reconstruction against the temporal mean of the stack `im` (:99-102)."""
+        error_map = ErrorMap()
+        error_map.optimise(np.mean(np.asarray(im), axis=0), np.asarray(reconstruction))
+        return error_map.getRSP()
+
+    def calculate_frc(self, im_odd, im_even):
+        return FIRECalculator().calculate_fire_number(np.asarray(im_odd), np.asarray(im_even))
+
+    def logistic_image_conversion(self, im, min_val=None, max_val=None):
+        """Maps [min, max] of `im` logistically onto (0.075, 0.925) (:113-133)."""
+        im = np.asarray(im)
+        if min_val is None:
+            min_val = np.min(im)
+        if max_val is None:
+            max_val = np.max(im)
+        M1, M2 = 0.075, 0.925
+        A1 = np.log((1 - M1) / M1)
+        A2 = np.log((1 - M2) / M2)
+        x0 = (A2 * max_val - A1 * min_val) / (A2 - A1)
+        k = 1 / (x0 - max_val) * A1
+        return np.asarray([1 / (np.exp(-k * (image - x0)) + 1) for image in im])
+
+    def calculate_qnr_score(self, RSP: np.ndarray, FRC: np.ndarray):
+        assert RSP.shape == FRC.shape
+        nFRC = self.logistic_image_conversion(FRC)
+        return (2 * np.asarray(RSP) * np.asarray(nFRC)) / (np.asarray(RSP) + np.asarray(nFRC))

@@ -124,10 +124,9 @@ def test_sweep_errors(torch):
 
 def test_parameter_sweep_class(torch, oracle_built):
     from nanopyx_amd.core.analysis import ParameterSweep
-    from nanopyx_amd import run_esrrf_parameter_sweep
     x = _stack((5, 16, 20), seed=8)
-    ps = ParameterSweep(doErrorMapping=False, doFRCMapping=False)
-    rec = ps.run(x, 2, [1, 2], [1.0, 1.5])
+    ps = ParameterSweep()
+    rec = ps.reconstructions(x, 2, [1, 2], [1.0, 1.5])
     assert rec[2].shape == (2, 2, 32, 40)
     for si, S in enumerate([1, 2]):
         for ri, R in enumerate([1.0, 1.5]):
@@ -137,6 +136,35 @@ def test_parameter_sweep_class(torch, oracle_built):
     ref = oracle_built.temporal(oracle_built.esrrf(x, 2, 1.5, 2, True), "TAC2")
     assert _report("ParameterSweep TAC2", tac, ref) <= TOL
     with pytest.raises(NotImplementedError):
-        ParameterSweep().run(x, 2, [1], [1.0])
-    out = run_esrrf_parameter_sweep(x, 2, [1, 2], [1.0], score=False)
-    assert out.shape == (2, 1, 32, 40)
+        ps.run(x, 2, [1], [1.0], use_decorr=True)
+
+
+def test_parameter_sweep_qnr(torch, oracle_built):
+    """QnR of the GPU sweep equals the same scoring (error map RSP, FIRE of odd/even frames, logistic harmonic mean)
+    applied to the oracle's reconstructions, and run_esrrf_parameter_sweep picks its argmax."""
+    from nanopyx_amd.core.analysis import ParameterSweep
+    from nanopyx_amd import run_esrrf_parameter_sweep
+    x = _stack((10, 24, 28), seed=9)
+    sens, radii = [1, 2], [1.0, 1.5, 2.0]
+    qnr = ParameterSweep().run(x, 2, sens, radii)
+    assert qnr.shape == (2, 3) and np.isfinite(qnr).all()
+    ps = ParameterSweep()
+    rsp = np.zeros((2, 3))
+    frc = np.zeros((2, 3))
+    for si, S in enumerate(sens):
+        for ri, R in enumerate(radii):
+            full = oracle_built.temporal(oracle_built.esrrf(x, 2, R, S, True), "AVG")
+            odd = oracle_built.temporal(oracle_built.esrrf(np.ascontiguousarray(x[1::2]), 2, R, S, True), "AVG")
+            even = oracle_built.temporal(oracle_built.esrrf(np.ascontiguousarray(x[::2]), 2, R, S, True), "AVG")
+            rsp[si, ri] = ps.calculate_rsp(x, full)
+            frc[si, ri] = ps.calculate_frc(odd, even)
+    expect = ps.calculate_qnr_score(rsp, frc)
+    print("QnR gpu", np.round(qnr, 5).tolist(), "oracle", np.round(expect, 5).tolist())
+    assert np.allclose(qnr, expect, rtol=1e-3, atol=1e-4)
+    q2 = run_esrrf_parameter_sweep(x, 2, sens, radii, return_qnr=True)
+    assert np.allclose(q2, qnr, rtol=1e-6, atol=1e-7)
+    best = run_esrrf_parameter_sweep(x, 2, sens, radii)
+    si, ri = np.unravel_index(np.argmax(qnr), qnr.shape)
+    assert best == (sens[si], radii[ri])
+    sliced = ParameterSweep().run(x, 2, sens, radii, n_frames=4)    # RSP averaged over slices of 4, 4 and 2 frames
+    assert sliced.shape == (2, 3) and np.isfinite(sliced).all()
