@@ -6,13 +6,16 @@ Drop-in entry points with the reference's signatures:
     run type ``HIP_gfx950`` calls hand-written HIP kernels through the ctypes C ABI in include/npx_esrrf.h
   * ``calculate_eSRRF_temporal_correlations``, ``macro_pixel_corrector``
   * ``run_esrrf_parameter_sweep`` / ``core.analysis.ParameterSweep`` — every (sensitivity, radius) reconstruction
-    in one batched launch (src/nanopyx/core/analysis/parameter_sweep.py:11-97; QnR scoring not built)
+    in one batched launch, scored by QnR on the host (src/nanopyx/core/analysis/parameter_sweep.py:11-139)
+  * ``calculate_frc``, ``calculate_decorr_analysis``, ``calculate_error_map`` — the metrics the sweep scores with
 """
-from .methods import eSRRF, Workflow, run_esrrf_parameter_sweep
+from .methods import (eSRRF, Workflow, calculate_decorr_analysis, calculate_error_map, calculate_frc,
+                      run_esrrf_parameter_sweep)
 from .core.transform import (RadialGradientConvergence, calculate_eSRRF_temporal_correlations, eSRRF_ST,
                              macro_pixel_corrector)
 
 __version__ = "0.1.0"
 
-__all__ = ["eSRRF", "Workflow", "run_esrrf_parameter_sweep", "eSRRF_ST", "RadialGradientConvergence", "calculate_eSRRF_temporal_correlations",
-           "macro_pixel_corrector"]
+__all__ = ["eSRRF", "Workflow", "run_esrrf_parameter_sweep", "eSRRF_ST", "RadialGradientConvergence",
+           "calculate_eSRRF_temporal_correlations", "macro_pixel_corrector", "calculate_frc",
+           "calculate_decorr_analysis", "calculate_error_map"]

@@ -1,3 +1,5 @@
+from .decorr import DecorrAnalysis
+from .frc import FIRECalculator
 from .parameter_sweep import ParameterSweep
 
-__all__ = ["ParameterSweep"]
+__all__ = ["ParameterSweep", "FIRECalculator", "DecorrAnalysis"]

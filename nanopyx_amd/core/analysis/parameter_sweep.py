@@ -10,7 +10,8 @@ computed in-kernel; VAR and TAC2 use one fused ``npx_esrrf_temporal`` call per p
 
 Scoring is the reference's, restated on the host as the reference runs it: RSP from ``ErrorMap`` (:99-102), the FIRE
 number of the odd/even reconstructions (:104-107), and QnR = 2 RSP nFRC / (RSP + nFRC) with nFRC the logistic
-rescaling of the FRC map (:109-139). The decorrelation alternative (``use_decorr=True``) is not built.
+rescaling of the FRC map (:109-139); with ``use_decorr=True`` the decorrelation resolution of the all-frames
+reconstruction replaces the FIRE number (:43-46).
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ import numpy as np
 
 from ... import _hip
 from ..transform.error_map import ErrorMap
+from .decorr import DecorrAnalysis
 from .frc import FIRECalculator
 
 
@@ -57,44 +59,51 @@ class ParameterSweep:
 
     def run(self, im: np.ndarray, magnification: int, sensitivity_array: list, radius_array: list,
             temporal_correlation: str = "AVG", use_decorr: bool = False, n_frames=None):
-        """QnR map (n_sens, n_rad) of the reference's sweep (:17-97)."""
-        if use_decorr:
-            raise NotImplementedError("decorrelation analysis (use_decorr=True) is not part of nanopyx_amd")
-        torch = _hip._torch()
+        """QnR map (n_sens, n_rad) of the reference's sweep (:17-97).
+
+        With ``n_frames`` the reference re-creates its (slices, sensitivities, radii) RSP array inside the loop over
+        pairs and replaces it by its mean over slices (:55-65), so after the loop only the last pair's sliced RSP is
+        non-zero and every other pair scores 0; that result is reproduced here."""
         im = np.asarray(im)
         if im.ndim == 2:
             im = im[None]
         M = int(magnification)
-        ns, nr = len(sensitivity_array), len(radius_array)
+        sens, radii = list(sensitivity_array), list(radius_array)
+        ns, nr = len(sens), len(radii)
         rsp = np.zeros((ns, nr))
         frc = np.zeros((ns, nr))
         x = _hip._dev_stack(im)
 
-        def rec(stack):
-            return self.reconstructions(stack, M, sensitivity_array, radius_array, temporal_correlation)[M]
+        def rec(stack, s=sens, r=radii):
+            return self.reconstructions(stack, M, s, r, temporal_correlation)[M]
 
+        full = None
+        if (self.doErrorMapping and n_frames is None) or (self.doFRCMapping and use_decorr):
+            full = rec(x)
         if self.doErrorMapping:
             if n_frames is None:
-                full = rec(x)
-                for s in range(ns):
-                    for r in range(nr):
-                        rsp[s, r] = self.calculate_rsp(im, full[s, r])
+                for si in range(ns):
+                    for ri in range(nr):
+                        rsp[si, ri] = self.calculate_rsp(im, full[si, ri])
             else:
                 n_slices = im.shape[0] // n_frames + (1 if im.shape[0] % n_frames else 0)
-                per = np.zeros((n_slices, ns, nr))
-                for i in range(n_slices):
-                    part = rec(x[i * n_frames:(i + 1) * n_frames])
-                    for s in range(ns):
-                        for r in range(nr):
-                            per[i, s, r] = self.calculate_rsp(im[i * n_frames:(i + 1) * n_frames], part[s, r])
-                rsp = np.mean(per, axis=0)
+                per = [self.calculate_rsp(im[i * n_frames:(i + 1) * n_frames],
+                                          rec(x[i * n_frames:(i + 1) * n_frames], [sens[-1]], [radii[-1]])[0, 0])
+                       for i in range(n_slices)]
+                rsp[ns - 1, nr - 1] = np.mean(per)
         if self.doFRCMapping:
-            odd = rec(x[1::2].contiguous())
-            even = rec(x[::2].contiguous())
-            for s in range(ns):
-                for r in range(nr):
-                    frc[s, r] = self.calculate_frc(odd[s, r], even[s, r])
-        torch.cuda.synchronize()
+            if use_decorr:
+                for si in range(ns):
+                    for ri in range(nr):
+                        decorr = DecorrAnalysis()
+                        decorr.run_analysis(full[si, ri])
+                        frc[si, ri] = decorr.resolution
+            else:
+                odd = rec(x[1::2].contiguous())
+                even = rec(x[::2].contiguous())
+                for si in range(ns):
+                    for ri in range(nr):
+                        frc[si, ri] = self.calculate_frc(odd[si, ri], even[si, ri])
         return self.calculate_qnr_score(rsp, frc)
 
     def calculate_rsp(self, im, reconstruction):

@@ -135,8 +135,6 @@ def test_parameter_sweep_class(torch, oracle_built):
     tac = ps.reconstructions(x, 2, [2], [1.5], temporal_correlation="TAC2")[2][0, 0]
     ref = oracle_built.temporal(oracle_built.esrrf(x, 2, 1.5, 2, True), "TAC2")
     assert _report("ParameterSweep TAC2", tac, ref) <= TOL
-    with pytest.raises(NotImplementedError):
-        ps.run(x, 2, [1], [1.0], use_decorr=True)
 
 
 def test_parameter_sweep_qnr(torch, oracle_built):
@@ -166,5 +164,50 @@ def test_parameter_sweep_qnr(torch, oracle_built):
     best = run_esrrf_parameter_sweep(x, 2, sens, radii)
     si, ri = np.unravel_index(np.argmax(qnr), qnr.shape)
     assert best == (sens[si], radii[ri])
-    sliced = ParameterSweep().run(x, 2, sens, radii, n_frames=4)    # RSP averaged over slices of 4, 4 and 2 frames
-    assert sliced.shape == (2, 3) and np.isfinite(sliced).all()
+
+
+
+def test_parameter_sweep_decorr_and_slices(torch, oracle_built):
+    """use_decorr=True scores the all-frames reconstruction by its decorrelation resolution; with n_frames only the
+    last (sensitivity, radius) pair keeps a non-zero sliced RSP, as in the reference. An emitter scene (noise-only
+    reconstructions have no finite decorrelation resolution)."""
+    from nanopyx_amd import synthetic
+    from nanopyx_amd.core.analysis import DecorrAnalysis, ParameterSweep
+    sc = synthetic.Scene(frames=12, height=48, width=56, emitters=60, sigma=1.5, photons=500, p_on=0.3,
+                         background=100, seed=14)
+    x = synthetic.generate(sc)
+    sens, radii = [1, 1.5], [1.0, 1.5, 2.0]
+    ps = ParameterSweep()
+    qnr = ps.run(x, 2, sens, radii, use_decorr=True)
+    rsp = np.zeros((2, 3))
+    res = np.zeros((2, 3))
+    frc = np.zeros((2, 3))
+    for si, S in enumerate(sens):
+        for ri, R in enumerate(radii):
+            full = oracle_built.temporal(oracle_built.esrrf(x, 2, R, S, True), "AVG")
+            rsp[si, ri] = ps.calculate_rsp(x, full)
+            d = DecorrAnalysis()
+            d.run_analysis(full)
+            res[si, ri] = d.resolution
+            odd = oracle_built.temporal(oracle_built.esrrf(np.ascontiguousarray(x[1::2]), 2, R, S, True), "AVG")
+            even = oracle_built.temporal(oracle_built.esrrf(np.ascontiguousarray(x[::2]), 2, R, S, True), "AVG")
+            frc[si, ri] = ps.calculate_frc(odd, even)
+    assert np.isfinite(res).all()
+    expect = ps.calculate_qnr_score(rsp, res)
+    print("decorr QnR gpu", np.round(qnr, 5).tolist(), "oracle", np.round(expect, 5).tolist(), flush=True)
+    assert np.allclose(qnr, expect, rtol=1e-3, atol=1e-4)
+    sliced = ps.run(x, 2, sens, radii, n_frames=4)
+    assert np.count_nonzero(sliced) == 1 and sliced[1, 2] > 0
+    per = [ps.calculate_rsp(x[i:i + 4], oracle_built.temporal(oracle_built.esrrf(
+        np.ascontiguousarray(x[i:i + 4]), 2, 2.0, 1.5, True), "AVG")) for i in (0, 4, 8)]
+    r = np.zeros((2, 3))
+    r[1, 2] = np.mean(per)
+    assert np.allclose(sliced, ps.calculate_qnr_score(r, frc), rtol=1e-3, atol=1e-4)
+
+
+def test_parameter_sweep_reference_call(torch):
+    """The reference's own sweep test call: float64 noise, two-frame slices, decorrelation scoring."""
+    from nanopyx_amd import run_esrrf_parameter_sweep
+    img = np.random.default_rng(11).random((10, 100, 100))
+    s, r = run_esrrf_parameter_sweep(img, n_frames=2, use_decorr=True, plot_sweep=False)
+    assert s in (1, 2) and r in (1, 1.5)
