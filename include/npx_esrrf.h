@@ -149,6 +149,34 @@ int npx_session_stats(void *session, long long *stats);
 int npx_session_destroy(void *session);
 
 /*
+ * SRRF radiality of a device (N, H, W) stack (dtype NPX_DTYPE_F32 or NPX_DTYPE_U16) into d_out (N, H*M, W*M) float32: 12 ring
+ * samples of radius ringRadius*M around each output pixel of [(1+border)M, (H-1-border)M) x [(1+border)M,
+ * (W-1-border)M), every other output pixel 0. With iw the map is multiplied by d_interp (N, H*M, W*M), or, when
+ * d_interp is NULL, by the xM Catmull-Rom magnification of the input computed in the same kernel.
+ * Replaces Radiality._run_threaded / _run_opencl, src/nanopyx/core/transform/_le_radiality.pyx:86-126 (per-pixel
+ * kernel src/include/_c_sr_radiality.c:15-51, gradients src/include/_c_gradients.c:3-25); with d_interp NULL it is
+ * also the CRShiftAndMagnify + Radiality pair of SRRF(), src/nanopyx/methods/srrf/SRRF_workflow.py:50-67.
+ */
+int npx_radiality(const void *d_img, int dtype, const float *d_interp, int N, int H, int W, int M, double ringRadius,
+                  int border, int positivity, int iw, float *d_out, void *stream);
+
+/*
+ * Host-only introspection of a radiality plan: info[12] = HM, WM, j_lo, j_hi, i_lo, i_hi, gradient tile rows and
+ * columns, output rows and columns per block, row bands, column tiles; ring[25] = ringRadius*M, xRing[12],
+ * yRing[12]. No reference counterpart: it exists so the host can check the ring and the launch geometry.
+ */
+int npx_radiality_info(int H, int W, int M, double ringRadius, int border, int *info, float *ring);
+
+/*
+ * Host-only copy of a radiality plan's tables: row (H*M x 80 floats), col (W*M x 80), tile_r0 (row bands),
+ * tile_c0 (column tiles); per entry the 12 ring samples' first tap (int bits, tile-relative), 4 weights each, the
+ * offsets xc - x0 (or yc - y0), then the intensity taps. No reference counterpart: it lets the host check the
+ * tables against the reference's arithmetic without a GPU.
+ */
+int npx_radiality_tables(int H, int W, int M, double ringRadius, int border, float *row, float *col, int *tile_r0,
+                         int *tile_c0);
+
+/*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
  */

@@ -92,6 +92,10 @@ def lib():
                                    ctypes.POINTER(_c_double)]
         L.npx_tune_set.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
         L.npx_tune_get.argtypes = [_c_int, _c_int, _c_int, _c_double, _ip]
+        L.npx_radiality.argtypes = [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int,
+                                    _c_int, _c_int, _c_void_p, _c_void_p]
+        L.npx_radiality_info.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _ip, _fp]
+        L.npx_radiality_tables.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _fp, _fp, _ip, _ip]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -388,6 +392,52 @@ def macro_pixel_(img, magnification, stream=None):
     cmap = torch.empty(S * M * M, dtype=torch.float32, device=img.device)
     _check(lib().npx_macro_pixel(img.data_ptr(), S, HM, WM, M, cmap.data_ptr(), _stream(torch, stream)))
     return img
+
+
+def radiality(stack, interp, magnification, ringRadius, border, positivity, iw, stream=None, out=None):
+    """SRRF radiality maps (N, H*M, W*M) as a device tensor. `interp` is the caller's (N, H*M, W*M) intensity image
+    for the weighting, or None to weight by the xM Catmull-Rom magnification computed in the kernel."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    M = int(magnification)
+    ip = 0
+    if interp is not None:
+        interp = _dev_stack(interp).to(torch.float32).contiguous()
+        if tuple(interp.shape) != (N, H * M, W * M):
+            raise ValueError("image_interp must be (N, H*magnification, W*magnification) for an (N, H, W) image")
+        ip = interp.data_ptr()
+    if out is None:
+        out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
+    _check(lib().npx_radiality(x.data_ptr(), _dtype_code(x), ip, N, H, W, M, float(ringRadius), int(border),
+                               int(bool(positivity)), int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+    return out
+
+
+def radiality_info(H, W, M, ringRadius, border):
+    """Host-only view of a radiality plan: ring coordinates and launch geometry."""
+    info = (ctypes.c_int * 12)()
+    ring = (ctypes.c_float * 25)()
+    _check(lib().npx_radiality_info(int(H), int(W), int(M), float(ringRadius), int(border), info, ring))
+    keys = ("HM", "WM", "j_lo", "j_hi", "i_lo", "i_hi", "TH", "TW", "TJ", "TI", "bands", "tiles")
+    d = dict(zip(keys, list(info)))
+    d["R"] = ring[0]
+    d["xRing"] = np.asarray(ring[1:13], np.float32)
+    d["yRing"] = np.asarray(ring[13:25], np.float32)
+    return d
+
+
+def radiality_tables(H, W, M, ringRadius, border):
+    """Host-only copy of a radiality plan's tables: (row (H*M, 80), col (W*M, 80), tile_r0, tile_c0)."""
+    d = radiality_info(H, W, M, ringRadius, border)
+    row = np.zeros((d["HM"], 80), np.float32)
+    col = np.zeros((d["WM"], 80), np.float32)
+    r0 = np.zeros(d["bands"], np.int32)
+    c0 = np.zeros(d["tiles"], np.int32)
+    _check(lib().npx_radiality_tables(int(H), int(W), int(M), float(ringRadius), int(border),
+                                      row.ctypes.data_as(_fp), col.ctypes.data_as(_fp), r0.ctypes.data_as(_ip),
+                                      c0.ctypes.data_as(_ip)))
+    return row, col, r0, c0
 
 
 def plan_info(H, W, M, R):

@@ -1,6 +1,7 @@
 // Host-side eSRRF plan tables (see npx_plan.h). Compiled for the host only; float64 is used here and
 // never on the GPU. The float32 roundings the reference performs are reproduced with explicit casts
 // (this file is built with -ffp-contract=off so no product is fused into a sum behind our back).
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -145,6 +146,130 @@ bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
     p.cr_col.resize(p.WM);
     for (int r = 0; r < p.HM; ++r) p.cr_row[r] = cr_taps(r, mag, H);
     for (int c = 0; c < p.WM; ++c) p.cr_col[c] = cr_taps(c, mag, W);
+    return true;
+}
+
+// Taps of one ring-sample coordinate v = (p + 0.5 + ring) / M of the gradient image of size n, as
+// _c_interpolate uses them: 0 weights when v is outside [0, n) or the tap is outside the image.
+static CrTap ring_taps(float v, int n) {
+    CrTap t{};
+    const int v_int = static_cast<int>(std::floor(static_cast<float>(static_cast<double>(v) - 0.5)));
+    t.base = v_int - 1;
+    for (int u = 0; u < 4; ++u) t.w[u] = 0.0f;
+    if (v < 0 || v >= static_cast<float>(n)) return t;
+    for (int u = 0; u < 4; ++u) {
+        const int vn = v_int - 1 + u;
+        if (vn < 0 || vn >= n) continue;
+        t.w[u] = static_cast<float>(cubic(static_cast<double>(v) - (vn + 0.5)));
+    }
+    return t;
+}
+
+bool build_srrf_plan(int H, int W, int M, double ringRadius, int border, int TJ, int TI, SrrfPlan& p,
+                     std::string* err) {
+    if (H < 1 || W < 1 || M < 1 || TJ < 1 || TI < 1) {
+        if (err) *err = "image size and magnification must be >= 1";
+        return false;
+    }
+    if (!(ringRadius >= 0) || !std::isfinite(ringRadius)) {
+        if (err) *err = "ringRadius must be a finite number >= 0";
+        return false;
+    }
+    p.H = H;
+    p.W = W;
+    p.M = M;
+    p.HM = H * M;
+    p.WM = W * M;
+    p.border = border;
+    p.TJ = TJ;
+    p.TI = TI;
+    // _le_radiality.pyx:95-103: _ringRadius = float(ringRadius * M); angleStep = float(2 pi / 12);
+    // coordinate = float(cos(float(angleStep * k)) * _ringRadius) (double product)
+    p.R = static_cast<float>(ringRadius * M);
+    const float step = static_cast<float>((M_PI * 2.) / 12);
+    for (int k = 0; k < 12; ++k) {
+        const float a = step * static_cast<float>(k);
+        p.xr[k] = static_cast<float>(std::cos(static_cast<double>(a)) * p.R);
+        p.yr[k] = static_cast<float>(std::sin(static_cast<double>(a)) * p.R);
+    }
+    // computed range (_le_radiality.pyx:117-118)
+    p.j_lo = (1 + border) * M;
+    p.j_hi = (H - 1 - border) * M;
+    p.i_lo = (1 + border) * M;
+    p.i_hi = (W - 1 - border) * M;
+    const float magf = static_cast<float>(M);
+    // entries in absolute image coordinates first
+    auto fill = [&](int n_out, int n_img, const float* ring, std::vector<float>& tab, std::vector<int>& lo_abs,
+                    std::vector<int>& hi_abs) {
+        tab.assign(static_cast<size_t>(n_out) * SrrfEntry::kFloats, 0.0f);
+        lo_abs.assign(n_out, 0);
+        hi_abs.assign(n_out, 0);
+        for (int q = 0; q < n_out; ++q) {
+            float* e = &tab[static_cast<size_t>(q) * SrrfEntry::kFloats];
+            const CrTap it = cr_taps(q, magf, n_img);
+            // xc = i + 0.5 (double, stored float); x0 = xc + ring (float); coordinate x0 / M (float)
+            const float c = static_cast<float>(q + 0.5);
+            int lo = it.base, hi = it.base + 3;
+            for (int k = 0; k < 12; ++k) {
+                const float x0 = c + ring[k];
+                const float v = x0 / magf;
+                CrTap t = ring_taps(v, n_img);
+                bool any = false;
+                for (int u = 0; u < 4; ++u) any = any || t.w[u] != 0.0f;
+                if (!any) t.base = it.base;  // all-zero sample: park its taps on the intensity taps
+                reinterpret_cast<int*>(e)[SrrfEntry::kBase + k] = t.base;
+                for (int u = 0; u < 4; ++u) e[SrrfEntry::kW + 4 * k + u] = t.w[u];
+                e[SrrfEntry::kD + k] = c - x0;
+                lo = std::min(lo, t.base);
+                hi = std::max(hi, t.base + 3);
+            }
+            reinterpret_cast<int*>(e)[SrrfEntry::kIBase] = it.base;
+            for (int u = 0; u < 4; ++u) e[SrrfEntry::kIW + u] = it.w[u];
+            lo_abs[q] = lo;
+            hi_abs[q] = hi;
+        }
+    };
+    std::vector<int> rlo, rhi, clo, chi;
+    fill(p.HM, H, p.yr, p.row, rlo, rhi);
+    fill(p.WM, W, p.xr, p.col, clo, chi);
+    // per block: gradient tile origin = lowest tap, extent = max over blocks; bases made tile-relative
+    auto tiles = [&](int n_out, int T, std::vector<float>& tab, const std::vector<int>& lo, const std::vector<int>& hi,
+                     std::vector<int>& t0, int& ext) {
+        const int nt = (n_out + T - 1) / T;
+        t0.assign(nt, 0);
+        ext = 0;
+        for (int b = 0; b < nt; ++b) {
+            int a = lo[b * T], z = hi[b * T];
+            for (int q = b * T; q < std::min(n_out, (b + 1) * T); ++q) {
+                a = std::min(a, lo[q]);
+                z = std::max(z, hi[q]);
+            }
+            t0[b] = a;
+            ext = std::max(ext, z - a + 1);
+            for (int q = b * T; q < std::min(n_out, (b + 1) * T); ++q) {
+                int* e = reinterpret_cast<int*>(&tab[static_cast<size_t>(q) * SrrfEntry::kFloats]);
+                for (int k = 0; k < 12; ++k) e[SrrfEntry::kBase + k] -= a;
+                e[SrrfEntry::kIBase] -= a - 1;  // the input tile starts one before the gradient tile
+            }
+        }
+    };
+    tiles(p.HM, TJ, p.row, rlo, rhi, p.tile_r0, p.TH);
+    tiles(p.WM, TI, p.col, clo, chi, p.tile_c0, p.TW);
+    // every tap the kernel reads lies inside the block's tiles: gradient taps in [0, T) and intensity taps in
+    // [0, T + 2) of the input tile
+    auto inside = [&](const std::vector<float>& tab, int n_out, int ext) {
+        for (int q = 0; q < n_out; ++q) {
+            const int* e = reinterpret_cast<const int*>(&tab[static_cast<size_t>(q) * SrrfEntry::kFloats]);
+            for (int k = 0; k < 12; ++k)
+                if (e[SrrfEntry::kBase + k] < 0 || e[SrrfEntry::kBase + k] + 3 >= ext) return false;
+            if (e[SrrfEntry::kIBase] < 0 || e[SrrfEntry::kIBase] + 3 >= ext + 2) return false;
+        }
+        return true;
+    };
+    if (!inside(p.row, p.HM, p.TH) || !inside(p.col, p.WM, p.TW)) {
+        if (err) *err = "internal: a radiality tap falls outside its tile";
+        return false;
+    }
     return true;
 }
 

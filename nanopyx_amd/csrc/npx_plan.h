@@ -53,6 +53,34 @@ struct RingLayout {
 // Builds the tables; returns false with a message on unsupported input.
 bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err);
 
+// ---- SRRF radiality -------------------------------------------------------------------------------------------
+// One table entry per output column (or row): for each of the 12 ring samples the first of the 4 Catmull-Rom taps
+// of the gradient (relative to the gradient tile of the block that owns the column/row), the 4 tap weights (0 for
+// taps outside the image and for samples outside the image, where the reference returns 0) and the offset
+// xc - x0 (or yc - y0) in float32; then the intensity taps (first tap relative to the block's input tile, which
+// starts one row/column before the gradient tile) for the xM Catmull-Rom magnification.
+struct SrrfEntry {
+    static constexpr int kRing = 12;
+    static constexpr int kBase = 0, kW = 12, kD = 60, kIBase = 72, kIW = 73, kFloats = 80;
+};
+
+struct SrrfPlan {
+    int H = 0, W = 0, M = 0, HM = 0, WM = 0, border = 0;
+    float R = 0;                            // ringRadius * M, float32
+    float xr[12] = {}, yr[12] = {};         // ring coordinates
+    int j_lo = 0, j_hi = 0, i_lo = 0, i_hi = 0;  // computed output range [lo, hi); everything else is 0
+    int TJ = 0, TI = 0;                     // output rows / columns per block
+    int TH = 0, TW = 0;                     // gradient tile rows / columns per block (max over blocks)
+    std::vector<int> tile_r0, tile_c0;      // [row band], [column tile]: image row / column of gradient tile 0
+    std::vector<float> row, col;            // [HM][80], [WM][80]
+};
+
+// Reference-exact ring, positions and Catmull-Rom taps of Radiality (src/nanopyx/core/transform/_le_radiality.pyx:
+// 86-126, src/include/_c_sr_radiality.c:15-51, src/include/_c_interpolation_catmull_rom.c:21-52). TJ output rows and
+// TI output columns per block.
+bool build_srrf_plan(int H, int W, int M, double ringRadius, int border, int TJ, int TI, SrrfPlan& p,
+                     std::string* err);
+
 // Reference-exact Catmull-Rom taps of one output coordinate i sampled at float(i) / float(mag)
 // (src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:165-168, _c_interpolation_catmull_rom.c:21-52).
 CrTap cr_taps(int i, float mag, int n);

@@ -1,6 +1,7 @@
 from .esrrf import eSRRF, run_esrrf_parameter_sweep
+from .srrf import SRRF
 from .squirrel import calculate_decorr_analysis, calculate_error_map, calculate_frc
 from .workflow import Workflow
 
-__all__ = ["eSRRF", "Workflow", "run_esrrf_parameter_sweep", "calculate_frc", "calculate_decorr_analysis",
+__all__ = ["eSRRF", "SRRF", "Workflow", "run_esrrf_parameter_sweep", "calculate_frc", "calculate_decorr_analysis",
            "calculate_error_map"]

@@ -275,3 +275,119 @@ int ora_esrrf(const float *in, int nFrames, int rows, int cols, int magnificatio
     return 0;
 }
 
+
+/* ---------------------------------------------------------------- SRRF radiality (restated) */
+
+/* src/include/_c_gradients.c:3-25 — central differences on the interior, borders left untouched (0). */
+static void ora_gradient_radiality(float *image, float *imGc, float *imGr, int rows, int cols) {
+    for (int j = 1; j < rows - 1; j++) {
+        int r0 = j * cols, r_m1 = (j - 1) * cols, r_p1 = (j + 1) * cols;
+        for (int i = 1; i < cols - 1; i++) {
+            imGc[r0 + i] = -image[r0 + i - 1] + image[r0 + i + 1];
+            imGr[r0 + i] = -image[r_m1 + i] + image[r_p1 + i];
+        }
+    }
+}
+
+/* src/include/_c_sr_radiality.c:6-13 — squared alignment term; the division runs in double. */
+static float ora_radiality_dk(float x, float y, float xc, float yc, float vGx, float vGy, float GMag,
+                              float ringRadius) {
+    float Dk = 0;
+    if (GMag != 0) {
+        Dk = 1 - (fabs(vGy * (xc - x) - vGx * (yc - y)) / GMag) / ringRadius;
+        Dk = Dk * Dk;
+    }
+    return Dk;
+}
+
+/* src/include/_c_sr_radiality.c:15-51 — 12 ring samples of the Catmull-Rom interpolated gradient at
+ * ((i + 0.5 + xRing) / M, (j + 0.5 + yRing) / M); +Dk for samples whose gradient points inwards, -Dk otherwise;
+ * mean over the ring, clamped at 0 with the positivity constraint. */
+static float ora_radiality_per_subpixel(int i, int j, float *imGx, float *imGy, float *xRing, float *yRing,
+                                        int magnification, float ringRadius, int nRing, int positivity, int h,
+                                        int w) {
+    float x0, y0, xc, yc, xr, yr, vGx, vGy, GMag, Dk, DivDFactor = 0, CGH = 0;
+    xc = i + 0.5;
+    yc = j + 0.5;
+    for (int s = 0; s < nRing; s++) {
+        xr = xRing[s];
+        yr = yRing[s];
+        x0 = xc + xr;
+        y0 = yc + yr;
+        vGx = k_interp(imGx, y0 / magnification, x0 / magnification, h, w);
+        vGy = k_interp(imGy, y0 / magnification, x0 / magnification, h, w);
+        GMag = sqrt(vGx * vGx + vGy * vGy);
+        Dk = ora_radiality_dk(x0, y0, xc, yc, vGx, vGy, GMag, ringRadius);
+        if ((vGx * xr + vGy * yr) > 0) {
+            DivDFactor -= Dk;
+        } else {
+            DivDFactor += Dk;
+        }
+    }
+    DivDFactor /= nRing;
+    if (positivity == 1) {
+        CGH = fmaxf(DivDFactor, 0);
+    } else {
+        CGH = DivDFactor;
+    }
+    return CGH;
+}
+
+typedef float (*radiality_fn_t)(int, int, float *, float *, float *, float *, int, float, int, int, int, int);
+typedef void (*grad_fn_t)(float *, float *, float *, int, int);
+static radiality_fn_t k_radiality = ora_radiality_per_subpixel;
+static grad_fn_t k_grad_radiality = ora_gradient_radiality;
+
+/* Swap the radiality kernels for the reference's _c_calculate_radiality_per_subpixel / _c_gradient_radiality
+ * (NULL restores the restatement). The reference's radiality calls its own _c_interpolate. */
+void ora_use_radiality_kernels(void *radiality, void *grad) {
+    k_radiality = radiality ? (radiality_fn_t)radiality : ora_radiality_per_subpixel;
+    k_grad_radiality = grad ? (grad_fn_t)grad : ora_gradient_radiality;
+}
+
+float ora_px_radiality(int i, int j, float *gx, float *gy, float *xRing, float *yRing, int M, float ringRadius,
+                       int nRing, int positivity, int h, int w) {
+    return k_radiality(i, j, gx, gy, xRing, yRing, M, ringRadius, nRing, positivity, h, w);
+}
+void ora_px_gradient_radiality(float *image, float *gc, float *gr, int rows, int cols) {
+    k_grad_radiality(image, gc, gr, rows, cols);
+}
+
+/* Ring coordinates of Radiality._run_threaded (src/nanopyx/core/transform/_le_radiality.pyx:95-103): 12 angles,
+ * angleStep = float(2 pi / 12), coordinates cos/sin(angleStep * k) * ringRadius * M in double, stored as float. */
+void ora_radiality_ring(double ringRadius, int magnification, float *xRing, float *yRing, float *ringR) {
+    float R = (float)(ringRadius * magnification);
+    float angleStep = (float)((M_PI * 2.) / 12);
+    for (int k = 0; k < 12; k++) {
+        xRing[k] = (float)(cos(angleStep * k) * R);
+        yRing[k] = (float)(sin(angleStep * k) * R);
+    }
+    *ringR = R;
+}
+
+/* Radiality._run_threaded: src/nanopyx/core/transform/_le_radiality.pyx:86-126. Per frame: gradients, then
+ * every output pixel of [(1+border)M, (h-1-border)M)^2 (prange over rows), times image_interp when
+ * doIntensityWeighting; everything else stays 0. */
+void ora_radiality(const float *image, const float *image_interp, int nFrames, int h, int w, int magnification,
+                   double ringRadius, int border, int positivity, int doIntensityWeighting, float *out) {
+    float xRing[12], yRing[12], R;
+    ora_radiality_ring(ringRadius, magnification, xRing, yRing, &R);
+    int M = magnification, hM = h * M, wM = w * M;
+    size_t p = (size_t)h * w, pM = (size_t)hM * wM;
+    float *gx = (float *)calloc(p, sizeof(float));
+    float *gy = (float *)calloc(p, sizeof(float));
+    for (int f = 0; f < nFrames; f++) {
+        k_grad_radiality((float *)image + f * p, gx, gy, h, w);
+        float *o = out + f * pM;
+        const float *ii = image_interp + f * pM;
+#pragma omp parallel for schedule(static)
+        for (int j = (1 + border) * M; j < (h - 1 - border) * M; j++) {
+            for (int i = (1 + border) * M; i < (w - 1 - border) * M; i++) {
+                float v = k_radiality(i, j, gx, gy, xRing, yRing, M, R, 12, positivity, h, w);
+                o[(size_t)j * wM + i] = doIntensityWeighting ? v * ii[(size_t)j * wM + i] : v;
+            }
+        }
+    }
+    free(gx);
+    free(gy);
+}

@@ -18,8 +18,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libnpx_oracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libnpx_ref.so")
+_REF_SRRF_PATH = os.path.join(_HERE, "_ref", "libnpx_ref_srrf.so")
 _lib = None
 _ref = None
+_ref_srrf = None
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -55,6 +57,14 @@ def lib():
         L.ora_px_rgc.restype = ctypes.c_float
         L.ora_px_roberts.argtypes = [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.ora_num_threads.restype = ctypes.c_int
+        L.ora_use_radiality_kernels.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ora_px_radiality.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, _fp, ctypes.c_int, ctypes.c_float,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ora_px_radiality.restype = ctypes.c_float
+        L.ora_px_gradient_radiality.argtypes = [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
+        L.ora_radiality_ring.argtypes = [ctypes.c_double, ctypes.c_int, _fp, _fp, _fp]
+        L.ora_radiality.argtypes = [_fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp]
         _lib = L
     return _lib
 
@@ -74,6 +84,24 @@ def use_reference_kernels(on: bool) -> None:
         _ref = ctypes.CDLL(_REF_PATH)
     addr = lambda name: ctypes.cast(getattr(_ref, name), ctypes.c_void_p).value
     L.ora_use_kernels(addr("_c_calculate_rgc"), addr("_c_interpolate"), addr("_c_gradient_roberts_cross"))
+
+
+def use_reference_radiality(on: bool) -> None:
+    """Route the radiality kernels to the reference's compiled _c_calculate_radiality_per_subpixel /
+    _c_gradient_radiality (True) or to the restatement (False)."""
+    global _ref_srrf
+    L = lib()
+    if not on:
+        L.ora_use_radiality_kernels(None, None)
+        return
+    if _ref_srrf is None:
+        _ref_srrf = ctypes.CDLL(_REF_SRRF_PATH)
+    addr = lambda name: ctypes.cast(getattr(_ref_srrf, name), ctypes.c_void_p).value
+    L.ora_use_radiality_kernels(addr("_c_calculate_radiality_per_subpixel"), addr("_c_gradient_radiality"))
+
+
+def reference_srrf_available() -> bool:
+    return os.path.exists(_REF_SRRF_PATH)
 
 
 def _f32(a):
@@ -156,6 +184,40 @@ def esrrf_workflow(image, magnification=5, radius=1.5, sensitivity=1, frames_per
         m = temporal(rgc_maps, temporal_correlation)[None]
         out[t] = macro_pixel(m, magnification)[0] if macro_pixel_correction else m[0]
     return np.squeeze(out.astype(np.float32))
+
+
+def radiality_ring(ringRadius, magnification):
+    """(xRing[12], yRing[12], ringRadius * M) of Radiality._run_threaded."""
+    x = np.zeros(12, np.float32)
+    y = np.zeros(12, np.float32)
+    r = np.zeros(1, np.float32)
+    lib().ora_radiality_ring(float(ringRadius), int(magnification), _p(x), _p(y), _p(r))
+    return x, y, float(r[0])
+
+
+def radiality(image, image_interp, magnification=5, ringRadius=0.5, border=0, radialityPositivityConstraint=True,
+              doIntensityWeighting=True):
+    """Radiality.run on the CPU ("Threaded"), (N, H*M, W*M) float32."""
+    image = _f32(image if np.ndim(image) == 3 else np.asarray(image)[None])
+    image_interp = _f32(image_interp if np.ndim(image_interp) == 3 else np.asarray(image_interp)[None])
+    N, H, W = image.shape
+    M = int(magnification)
+    assert image_interp.shape == (N, H * M, W * M)
+    out = np.zeros((N, H * M, W * M), np.float32)
+    lib().ora_radiality(_p(image), _p(image_interp), N, H, W, M, float(ringRadius), int(border),
+                        int(bool(radialityPositivityConstraint)), int(bool(doIntensityWeighting)), _p(out))
+    return out
+
+
+def srrf_workflow(image, magnification=5, ringRadius=0.5, border=0, radialityPositivityConstraint=True,
+                  doIntensityWeighting=True, macro_pixel_correction=True):
+    """SRRF() of src/nanopyx/methods/srrf/SRRF_workflow.py:9-76: CR xM magnification, radiality, macro-pixel
+    correction of every frame."""
+    image = _f32(image if np.ndim(image) == 3 else np.asarray(image)[None])
+    M = int(magnification)
+    interp = shift_magnify(image, M, M)
+    rad = radiality(image, interp, M, ringRadius, border, radialityPositivityConstraint, doIntensityWeighting)
+    return macro_pixel(rad, M) if macro_pixel_correction else rad
 
 
 def num_threads() -> int:

@@ -1,0 +1,106 @@
+"""SRRF radiality on the GPU (npx_radiality) against the oracle: the Radiality operator on caller arrays, the fused
+SRRF() workflow (in-kernel magnification + macro-pixel correction), uint16 input, frame chunking, errors."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+CASES = [  # (N, H, W, M, ringRadius, border, positivity, iw)
+    (3, 20, 24, 2, 0.5, 0, True, True),
+    (2, 18, 70, 5, 0.5, 0, True, True),
+    (2, 16, 30, 3, 1.7, 1, False, False),
+    (2, 21, 17, 4, 2.0, 0, True, True),
+    (2, 12, 12, 1, 1.0, 2, True, False),
+    (1, 40, 33, 8, 1.25, 0, False, True),
+]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    assert _torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return _torch
+
+
+def _stack(shape, seed=0):
+    rng = np.random.default_rng(seed)
+    return (rng.poisson(50, size=shape) + rng.random(shape) * 20).astype(np.float32)
+
+
+def _report(tag, x, ref):
+    mx, mean = rel_err(x, ref)
+    print(f"{tag}: max {mx:.3e} mean {mean:.3e}", flush=True)
+    return mx
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_radiality_matches_oracle(torch, oracle_built, case):
+    from nanopyx_amd import _hip
+    N, H, W, M, R, b, pos, iw = case
+    print(f"radiality case {case}", flush=True)
+    x = _stack((N, H, W), seed=M)
+    interp = oracle_built.shift_magnify(x, M, M)
+    ref = oracle_built.radiality(x, interp, M, R, b, pos, iw)
+    got = _hip.radiality(x, interp if iw else None, M, R, b, pos, iw).cpu().numpy()
+    assert got.shape == ref.shape
+    assert _report(f"radiality {case}", got, ref) <= TOL
+    # in-kernel magnification instead of the caller's image
+    fused = _hip.radiality(x, None, M, R, b, pos, iw).cpu().numpy()
+    assert _report(f"radiality fused {case}", fused, ref) <= TOL
+
+
+def test_radiality_uint16_and_tensor(torch, oracle_built):
+    from nanopyx_amd import _hip
+    from nanopyx_amd.core.transform import Radiality
+    rng = np.random.default_rng(5)
+    x16 = (rng.poisson(300, size=(3, 24, 28)) + 100).astype(np.uint16)
+    x = x16.astype(np.float32)
+    interp = oracle_built.shift_magnify(x, 4, 4)
+    ref = oracle_built.radiality(x, interp, 4, 0.5)
+    got = _hip.radiality(torch.from_numpy(x16.astype(np.int32)).to(torch.uint16).cuda(), None, 4, 0.5, 0, True,
+                         True)
+    assert _report("radiality u16", got.cpu().numpy(), ref) <= TOL
+    op = Radiality(verbose=False)
+    out = op.run(x, interp, magnification=4, ringRadius=0.5, run_type="HIP_gfx950")
+    assert isinstance(out, np.ndarray) and out.dtype == np.float32
+    assert _report("Radiality.run", out, ref) <= TOL
+    out_t = op.run(torch.from_numpy(x).cuda(), torch.from_numpy(interp).cuda(), magnification=4, ringRadius=0.5)
+    assert out_t.is_cuda and np.array_equal(out_t.cpu().numpy(), out)
+
+
+@pytest.mark.parametrize("mpc", [True, False])
+def test_srrf_workflow_matches_oracle(torch, oracle_built, mpc):
+    from nanopyx_amd import SRRF
+    x = _stack((4, 26, 31), seed=9)
+    ref = oracle_built.srrf_workflow(x, 5, 0.5, 0, True, True, mpc)
+    got = SRRF(x, magnification=5, macro_pixel_correction=mpc)
+    assert got.shape == ref.shape and got.dtype == np.float32
+    assert _report(f"SRRF mpc={mpc}", got, ref) <= TOL
+
+
+def test_radiality_chunking_and_size(torch, oracle_built):
+    """A 64-frame 128 x 160 stack at M = 5 (frames split over block chunks) equals per-frame launches, and its
+    first and last frames match the oracle."""
+    from nanopyx_amd import _hip
+    x = torch.from_numpy(_stack((64, 128, 160), seed=11)).cuda()
+    full = _hip.radiality(x, None, 5, 0.5, 0, True, True)
+    for f in (0, 31, 63):
+        one = _hip.radiality(x[f:f + 1], None, 5, 0.5, 0, True, True)
+        assert torch.equal(one[0], full[f]), f
+    xs = x[[0, 63]].cpu().numpy()
+    ref = oracle_built.radiality(xs, oracle_built.shift_magnify(xs, 5, 5), 5, 0.5)
+    assert _report("radiality 128x160 frames 0, 63", full[[0, 63]].cpu().numpy(), ref) <= TOL
+
+
+def test_radiality_errors(torch):
+    from nanopyx_amd import _hip
+    x = _stack((2, 10, 12))
+    with pytest.raises(ValueError):
+        _hip.radiality(x, np.zeros((2, 20, 20), np.float32), 2, 0.5, 0, True, True)   # wrong interp shape
+    with pytest.raises(ValueError):
+        _hip.radiality(x, None, 2, -0.5, 0, True, True)
+    with pytest.raises(ValueError):
+        _hip.radiality(x, None, 2, 0.5, -1, True, True)
