@@ -99,6 +99,70 @@ def sweep_bench(reps):
             "frac_fp32_peak": round(flops / k_s / 1e12 / FP32_PEAK_TFLOPS, 4)}
 
 
+SRRF_FLOPS_PER_PX = 12 * 95 + 43   # per output pixel-frame: 12 ring samples x 95, + 43 per pixel (see srrf_bench)
+
+
+def srrf_bench(reps, cpu_frames=2, workload="srrf_256"):
+    """SRRF radiality (npx_radiality: gradients, 12-sample Catmull-Rom ring, in-kernel xM intensity weighting) over
+    the srrf_256 stack, per-frame maps out. Kernel time from HIP events on the launch stream over `reps` launches.
+    Algorithmic flops per output pixel-frame: per ring sample two 16-tap Catmull-Rom interpolations (4x4 separable:
+    20 multiply-adds = 40 flops each) and the convergence term (|G|^2, cross and dot products, Dk, accumulate: 15),
+    12 x 95; per pixel the ring mean and clamp (2), the 16-tap intensity interpolation (40) and the weight (1).
+    The CPU baseline is the oracle's restatement of the reference's threaded radiality on `cpu_frames` frames."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    sc = synthetic.CONFIGS[workload]
+    e = sc.esrrf
+    M, R = e["magnification"], e["ringRadius"]
+    x_host = synthetic.generate(sc)
+    x = torch.from_numpy(x_host).cuda()
+    N, H, W = x.shape
+    out = torch.empty((N, H * M, W * M), dtype=torch.float32, device="cuda")
+
+    def launch():
+        _hip.radiality(x, None, M, R, e["border"], e["radialityPositivityConstraint"], e["doIntensityWeighting"],
+                       out=out)
+
+    for _ in range(2):
+        launch()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        launch()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    U = N * H * M * W * M
+    flops = U * float(SRRF_FLOPS_PER_PX)
+    achieved = flops / (ms / 1e3) / 1e12
+    res = {"workload": f"{workload}: {N} frames x {H}x{W} float32, SRRF M{M} ringRadius {R}, positivity, "
+                       f"intensity-weighted, per-frame maps {H * M}x{W * M}",
+           "kernel_ms": round(ms, 4), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mpx·frames/s",
+           "roofline": {"bound": "mfma", "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                        "flops_per_launch": flops, "flops_per_px": SRRF_FLOPS_PER_PX,
+                        "algorithmic_bytes": N * H * W * 4 + U * 4,
+                        "hbm_gbs_algorithmic": round((N * H * W * 4 + U * 4) / (ms / 1e3) / 1e9, 1)}}
+    if cpu_frames:
+        from oracle import ref32
+        ref32.build()
+        xs = x_host[:cpu_frames]
+        t0 = time.perf_counter()
+        ref = ref32.srrf_workflow(xs, M, R, e["border"], e["radialityPositivityConstraint"],
+                                  e["doIntensityWeighting"], macro_pixel_correction=False)
+        dt = time.perf_counter() - t0
+        Us = cpu_frames * H * M * W * M
+        res["cpu_baseline"] = {"value": round(Us / 1e6 / dt, 2), "unit": "Mpx·frames/s",
+                               "cores": ref32.num_threads(), "kind": "port",
+                               "sample": f"first {cpu_frames} frames (CR xM + radiality)"}
+        got = out[:cpu_frames].cpu().numpy()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        res["parity_max_rel_err"] = float(err)
+    return res
+
+
 def serving_bench(n, rate_hz, seed=5):
     """The served stream: n seeded requests (frames log-uniform 50-1000, H and W from {64..1024}, M from
     {2, 3, 4, 5, 8}, R uniform on [1, 2.5], S from {1, 2, 3}, one of AVG / VAR / TAC2, float32 or uint16) arriving as
@@ -226,6 +290,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="re-run the launch-shape autotuner on the workload")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-srrf", action="store_true")
+    ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=50.0)
     args = ap.parse_args()
@@ -233,6 +299,10 @@ def main():
         print(f"bench.py runs on one GPU: the eSRRF path is built and measured on a single MI355X "
               f"(--gpus {args.gpus} is not supported; multi-GPU sharding is a design note only)", flush=True)
         sys.exit(2)
+
+    if args.srrf_only:
+        print(json.dumps(srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)), flush=True)
+        return
 
     import torch
     from nanopyx_amd import _hip, synthetic
@@ -317,11 +387,16 @@ def main():
         "cpu_baseline": None,
         "sweep": None,
         "serving": None,
+        "srrf": None,
     }
     if not args.no_sweep:
         log("parameter sweep ...")
         result["sweep"] = sweep_bench(max(3, args.steps // 2))
         log(f"sweep: {result['sweep']}")
+    if not args.no_srrf:
+        log("SRRF radiality ...")
+        result["srrf"] = srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)
+        log(f"srrf: {result['srrf']}")
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)

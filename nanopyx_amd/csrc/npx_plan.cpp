@@ -219,7 +219,8 @@ bool build_srrf_plan(int H, int W, int M, double ringRadius, int border, int TJ,
                 if (!any) t.base = it.base;  // all-zero sample: park its taps on the intensity taps
                 reinterpret_cast<int*>(e)[SrrfEntry::kBase + k] = t.base;
                 for (int u = 0; u < 4; ++u) e[SrrfEntry::kW + 4 * k + u] = t.w[u];
-                e[SrrfEntry::kD + k] = c - x0;
+                // offset over the ring radius: the kernel forms |G x d| / R and the sign of G . ring from it
+                e[SrrfEntry::kD + k] = static_cast<float>(static_cast<double>(c - x0) / static_cast<double>(p.R));
                 lo = std::min(lo, t.base);
                 hi = std::max(hi, t.base + 3);
             }

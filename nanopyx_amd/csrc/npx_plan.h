@@ -57,7 +57,7 @@ bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err);
 // One table entry per output column (or row): for each of the 12 ring samples the first of the 4 Catmull-Rom taps
 // of the gradient (relative to the gradient tile of the block that owns the column/row), the 4 tap weights (0 for
 // taps outside the image and for samples outside the image, where the reference returns 0) and the offset
-// xc - x0 (or yc - y0) in float32; then the intensity taps (first tap relative to the block's input tile, which
+// (xc - x0) / R (or (yc - y0) / R) with R = ringRadius * M; then the intensity taps (first tap relative to the block's input tile, which
 // starts one row/column before the gradient tile) for the xM Catmull-Rom magnification.
 struct SrrfEntry {
     static constexpr int kRing = 12;

@@ -1,15 +1,17 @@
 // SRRF radiality on gfx950 (Radiality, src/nanopyx/core/transform/_le_radiality.pyx:86-126, per-pixel kernel
 // src/include/_c_sr_radiality.c:15-51), and its C ABI entry point npx_radiality.
 //
-// One block = TI (64, one per lane) output columns x TJ output rows of one frame chunk, 4 waves; wave w takes
-// the rows w, w + 4, ... For each frame the block stages its input tile (with a one-pixel halo, zero outside
-// the image) in LDS, forms the central-difference gradients (Gx, Gy) of the tile in LDS, and evaluates every
-// output pixel: 12 ring samples of the Catmull-Rom interpolated gradient, each a 4 x 4 tap sum read from the
-// gradient tile. The tap bases and weights of a sample depend on the output column (x) or row (y) only, so they
-// come from host tables built with the reference's float32 positions and float64 cubic weights: the column
-// table of a lane sits in its VGPRs for the whole block, the row table of the block's rows in LDS (all lanes of
-// a wave read the same row entry, a broadcast). Intensity weighting multiplies by the caller's xM image or, when
-// none is given (the SRRF workflow), by the xM Catmull-Rom magnification computed here from the input tile.
+// One block = TI (64, one per lane) output columns x TJ output rows of one frame chunk, 8 waves. For each frame the
+// block stages its input tile (one-pixel halo, zero outside the image) in LDS and forms the central-difference
+// gradients (Gx, Gy) of the tile. A ring sample is a 4 x 4 Catmull-Rom sum of the gradient at
+// ((i + 0.5 + xRing) / M, (j + 0.5 + yRing) / M); its x taps and weights depend on the output column i only and its
+// y taps on the output row j only, so the sum is taken in two passes per group of 4 samples: a row pass
+// interpolates every gradient-tile row at each lane's sample x position (RX, in LDS), then the column pass
+// combines 4 RX rows per output pixel and sample. Tap bases and weights come from host tables built with the
+// reference's float32 positions and float64 cubic weights: a lane's column entry sits in its VGPRs for the whole
+// block, the block's row entries in LDS (one address per wave: a broadcast). The running ring sum of each pixel
+// stays in LDS between sample groups. Intensity weighting multiplies by the caller's xM image or, when none is
+// given (the SRRF workflow), by the xM Catmull-Rom magnification computed here from the input tile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,9 +30,10 @@
 
 namespace npx {
 
-constexpr int kSrrfThreads = 256;
+constexpr int kSrrfThreads = 512;
+constexpr int kSrrfWaves = kSrrfThreads / 64;
 constexpr int kSrrfTI = 64;
-constexpr int kSrrfTJ = 32;
+constexpr int kSrrfGroup = 4;  // ring samples per row-pass group (12 = 3 groups)
 constexpr int kSE = SrrfEntry::kFloats;
 
 struct SrrfArgs {
@@ -42,38 +45,40 @@ struct SrrfArgs {
     const int* tile_r0;   // [row bands]
     const int* tile_c0;   // [column tiles]
     int u16, N, H, W, M, HM, WM, fpc;
-    int TH, TW;
+    int TJ, TH, TW;
     int j_lo, j_hi, i_lo, i_hi;
     int positivity, iw;
-    float R;
-    float xr[12], yr[12];
 };
 
-__host__ __device__ inline int srrf_align4(int n) { return (n + 3) & ~3; }
-
-// LDS layout (floats): input tile (TH + 2) x (TW + 2) | gradient tile TH x TW float2 | row table TJ x kSE
-__host__ __device__ inline int srrf_in_floats(int TH, int TW) { return srrf_align4((TH + 2) * (TW + 2)); }
-inline size_t srrf_lds_bytes(int TH, int TW) {
-    return sizeof(float) * (static_cast<size_t>(srrf_in_floats(TH, TW)) + 2 * TH * TW + kSrrfTJ * kSE);
+// LDS layout (floats): input tile (TH + 2) x (TW + 2) | gradient tile TH x TW float2 | RX kSrrfGroup x TH x 64
+// float2 | row table TJ x kSE | ring sums TJ x 64
+__host__ __device__ inline int srrf_in_floats(int TH, int TW) { return ((TH + 2) * (TW + 2) + 3) & ~3; }
+__host__ __device__ inline int srrf_g_floats(int TH, int TW) { return (2 * TH * TW + 3) & ~3; }
+inline size_t srrf_lds_bytes(int TJ, int TH, int TW) {
+    return sizeof(float) * (static_cast<size_t>(srrf_in_floats(TH, TW)) + srrf_g_floats(TH, TW) +
+                            2 * kSrrfGroup * TH * 64 + TJ * kSE + TJ * 64);
 }
 
 template <bool U16, bool INTERP>
 __global__ __launch_bounds__(kSrrfThreads) void srrf_radiality(SrrfArgs a) {
     extern __shared__ float4 smem4[];
+    const int TJ = a.TJ, TH = a.TH, TW = a.TW;
     float* const s_in = reinterpret_cast<float*>(smem4);
-    float2* const s_g = reinterpret_cast<float2*>(s_in + srrf_in_floats(a.TH, a.TW));
-    float* const s_row = reinterpret_cast<float*>(s_g + a.TH * a.TW);
+    float2* const s_g = reinterpret_cast<float2*>(s_in + srrf_in_floats(TH, TW));
+    float2* const s_rx = reinterpret_cast<float2*>(reinterpret_cast<float*>(s_g) + srrf_g_floats(TH, TW));
+    float* const s_row = reinterpret_cast<float*>(s_rx + kSrrfGroup * TH * 64);
+    float* const s_div = s_row + TJ * kSE;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int bx = blockIdx.x, by = blockIdx.y;
     const int i = bx * kSrrfTI + lane;
     const int r0 = a.tile_r0[by], c0 = a.tile_c0[bx];
-    const int IW = a.TW + 2, IH = a.TH + 2;
+    const int IW = TW + 2, IH = TH + 2;
     const int f_begin = blockIdx.z * a.fpc;
     const int f_end = min(a.N, f_begin + a.fpc);
 
     // row table of the band
-    for (int t = threadIdx.x; t < kSrrfTJ * kSE; t += kSrrfThreads) {
-        const int j = by * kSrrfTJ + t / kSE;
+    for (int t = threadIdx.x; t < TJ * kSE; t += kSrrfThreads) {
+        const int j = by * TJ + t / kSE;
         s_row[t] = j < a.HM ? a.row[static_cast<size_t>(j) * kSE + t % kSE] : 0.f;
     }
     // column table of this lane's column, in VGPRs
@@ -92,7 +97,6 @@ __global__ __launch_bounds__(kSrrfThreads) void srrf_radiality(SrrfArgs a) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) wci[u] = ce[SrrfEntry::kIW + u];
     const bool col_in = i >= a.i_lo && i < a.i_hi;
-    const float invR = 1.0f / a.R;
 
     const size_t plane = static_cast<size_t>(a.H) * a.W, planeM = static_cast<size_t>(a.HM) * a.WM;
     for (int f = f_begin; f < f_end; ++f) {
@@ -110,8 +114,8 @@ __global__ __launch_bounds__(kSrrfThreads) void srrf_radiality(SrrfArgs a) {
         }
         __syncthreads();
         // (2) gradients (_c_gradients.c:3-25): interior pixels only, 0 on the image border and outside
-        for (int t = threadIdx.x; t < a.TH * a.TW; t += kSrrfThreads) {
-            const int r = t / a.TW, c = t % a.TW;
+        for (int t = threadIdx.x; t < TH * TW; t += kSrrfThreads) {
+            const int r = t / TW, c = t % TW;
             const int y = r0 + r, x = c0 + c;
             float2 g = make_float2(0.f, 0.f);
             if (y >= 1 && y < a.H - 1 && x >= 1 && x < a.W - 1) {
@@ -121,70 +125,104 @@ __global__ __launch_bounds__(kSrrfThreads) void srrf_radiality(SrrfArgs a) {
             }
             s_g[t] = g;
         }
-        __syncthreads();
-        // (3) radiality of this wave's rows
-        for (int jj = wv; jj < kSrrfTJ; jj += kSrrfThreads / 64) {
-            const int j = by * kSrrfTJ + jj;
-            if (j >= a.HM) break;
-            float v = 0.f;
-            if (col_in && j >= a.j_lo && j < a.j_hi) {
-                const float* re = s_row + jj * kSE;
-                float div = 0.f;
 #pragma unroll
-                for (int k = 0; k < 12; ++k) {
-                    const int rb = __float_as_int(re[SrrfEntry::kBase + k]);
-                    const float4 wy = *reinterpret_cast<const float4*>(re + SrrfEntry::kW + 4 * k);
-                    const float dy = re[SrrfEntry::kD + k];
-                    const float2* g = s_g + rb * a.TW + cb[k];
-                    // _c_interpolate order: columns outer, rows inner
-                    float qx = 0.f, qy = 0.f;
+        for (int grp = 0; grp < 12 / kSrrfGroup; ++grp) {
+            __syncthreads();  // gradient tile ready / the previous group's column pass is done with RX
+            // (3) row pass: RX[kk][r][lane] = sum_u wx[k][u] G[r][cb[k] + u] for the group's samples
+            for (int r = wv; r < TH; r += kSrrfWaves) {
+                const float2* gr = s_g + r * TW;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const float2 t0 = g[u], t1 = g[a.TW + u], t2 = g[2 * a.TW + u], t3 = g[3 * a.TW + u];
-                        float px = t0.x * wy.x;
-                        float py = t0.y * wy.x;
-                        px = fmaf(t1.x, wy.y, px);
-                        py = fmaf(t1.y, wy.y, py);
-                        px = fmaf(t2.x, wy.z, px);
-                        py = fmaf(t2.y, wy.z, py);
-                        px = fmaf(t3.x, wy.w, px);
-                        py = fmaf(t3.y, wy.w, py);
-                        qx = fmaf(px, wx[k][u], qx);
-                        qy = fmaf(py, wx[k][u], qy);
-                    }
-                    // Dk = (1 - |G x (xc - x0, yc - y0)| / (|G| R))^2, 0 where |G| = 0 (_c_sr_radiality.c:6-13)
-                    const float g2 = fmaf(qx, qx, qy * qy);
-                    const float cr = fabsf(fmaf(qy, dx[k], -qx * dy));
-                    float dk = fmaf(-cr * __builtin_amdgcn_rsqf(g2), invR, 1.f);
-                    dk = g2 > 0.f ? dk * dk : 0.f;
-                    const float dot = fmaf(qx, a.xr[k], qy * a.yr[k]);
-                    div += dot > 0.f ? -dk : dk;
-                }
-                div = div / 12.f;
-                v = a.positivity ? fmaxf(div, 0.f) : div;
-                if (a.iw) {
-                    float I;
-                    if (INTERP) {
-                        I = a.interp[static_cast<size_t>(f) * planeM + static_cast<size_t>(j) * a.WM + i];
-                    } else {
-                        // xM Catmull-Rom of the input at (j / M, i / M), columns outer, rows inner
-                        const int rbi = __float_as_int(re[SrrfEntry::kIBase]);
-                        const float4 wr = *reinterpret_cast<const float4*>(re + SrrfEntry::kIW);
-                        const float* p = s_in + rbi * IW + cbi;
-                        I = 0.f;
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            float s = p[u] * wr.x;
-                            s = fmaf(p[IW + u], wr.y, s);
-                            s = fmaf(p[2 * IW + u], wr.z, s);
-                            s = fmaf(p[3 * IW + u], wr.w, s);
-                            I = fmaf(s, wci[u], I);
-                        }
-                    }
-                    v *= I;
+                for (int kk = 0; kk < kSrrfGroup; ++kk) {
+                    const int k = grp * kSrrfGroup + kk;
+                    const float2* gp = gr + cb[k];
+                    const float2 t0 = gp[0], t1 = gp[1], t2 = gp[2], t3 = gp[3];
+                    float2 v;
+                    v.x = t0.x * wx[k][0];
+                    v.y = t0.y * wx[k][0];
+                    v.x = fmaf(t1.x, wx[k][1], v.x);
+                    v.y = fmaf(t1.y, wx[k][1], v.y);
+                    v.x = fmaf(t2.x, wx[k][2], v.x);
+                    v.y = fmaf(t2.y, wx[k][2], v.y);
+                    v.x = fmaf(t3.x, wx[k][3], v.x);
+                    v.y = fmaf(t3.y, wx[k][3], v.y);
+                    s_rx[(kk * TH + r) * 64 + lane] = v;
                 }
             }
-            if (i < a.WM) a.out[static_cast<size_t>(f) * planeM + static_cast<size_t>(j) * a.WM + i] = v;
+            __syncthreads();
+            // (4) column pass: the group's samples of this wave's rows, added to the pixel's running ring sum
+            for (int jj = wv; jj < TJ; jj += kSrrfWaves) {
+                const int j = by * TJ + jj;
+                if (j >= a.HM) break;
+                const bool in = col_in && j >= a.j_lo && j < a.j_hi;
+                float div = grp == 0 ? 0.f : s_div[jj * 64 + lane];
+                if (in) {
+                    const float* re = s_row + jj * kSE;
+                    // the group's 4 bases, 16 weights and 4 offsets of this row: 6 aligned 16-byte broadcast reads
+                    const float4 rb4 = *reinterpret_cast<const float4*>(re + SrrfEntry::kBase + grp * kSrrfGroup);
+                    const float4 dy4 = *reinterpret_cast<const float4*>(re + SrrfEntry::kD + grp * kSrrfGroup);
+                    const float rbs[4] = {rb4.x, rb4.y, rb4.z, rb4.w}, dys[4] = {dy4.x, dy4.y, dy4.z, dy4.w};
+#pragma unroll
+                    for (int kk = 0; kk < kSrrfGroup; ++kk) {
+                        const int k = grp * kSrrfGroup + kk;
+                        const int rb = __float_as_int(rbs[kk]);
+                        const float4 wy = *reinterpret_cast<const float4*>(re + SrrfEntry::kW + 4 * k);
+                        const float dy = dys[kk];
+                        const float2* rp = s_rx + (kk * TH + rb) * 64 + lane;
+                        const float2 t0 = rp[0], t1 = rp[64], t2 = rp[128], t3 = rp[192];
+                        float qx = t0.x * wy.x;
+                        float qy = t0.y * wy.x;
+                        qx = fmaf(t1.x, wy.y, qx);
+                        qy = fmaf(t1.y, wy.y, qy);
+                        qx = fmaf(t2.x, wy.z, qx);
+                        qy = fmaf(t2.y, wy.z, qy);
+                        qx = fmaf(t3.x, wy.w, qx);
+                        qy = fmaf(t3.y, wy.w, qy);
+                        // Dk = (1 - |G x (xc - x0, yc - y0)| / (|G| R))^2, 0 where |G| = 0 (_c_sr_radiality.c:6-13);
+                        // dx, dy are (xc - x0) / R and (yc - y0) / R. The sample counts -Dk where G points along the
+                        // ring vector (x0 - xc, y0 - yc), i.e. where G . (dx, dy) < 0 (the reference tests G . ring
+                        // with the unrounded ring coordinates; the two differ only where G . ring ~ 0, where Dk ~ 0)
+                        const float g2 = fmaf(qx, qx, qy * qy);
+                        const float cr = fabsf(fmaf(qy, dx[k], -qx * dy));
+                        float dk = fmaf(-cr, __builtin_amdgcn_rsqf(g2), 1.f);
+                        dk = g2 > 0.f ? dk * dk : 0.f;
+                        const float dot = fmaf(qx, dx[k], qy * dy);
+                        div += dot < 0.f ? -dk : dk;
+                    }
+                }
+                if (grp + 1 < 12 / kSrrfGroup) {
+                    s_div[jj * 64 + lane] = div;
+                    continue;
+                }
+                // (5) last group: ring mean, positivity, intensity weight, store
+                float v = 0.f;
+                if (in) {
+                    div = div / 12.f;
+                    v = a.positivity ? fmaxf(div, 0.f) : div;
+                    if (a.iw) {
+                        float I;
+                        if (INTERP) {
+                            I = a.interp[static_cast<size_t>(f) * planeM + static_cast<size_t>(j) * a.WM + i];
+                        } else {
+                            // xM Catmull-Rom of the input at (j / M, i / M)
+                            const float* re = s_row + jj * kSE;
+                            const int rbi = __float_as_int(re[SrrfEntry::kIBase]);
+                            const float4 wr = *reinterpret_cast<const float4*>(re + SrrfEntry::kIW);
+                            const float* p = s_in + rbi * IW + cbi;
+                            I = 0.f;
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                float s = p[u] * wr.x;
+                                s = fmaf(p[IW + u], wr.y, s);
+                                s = fmaf(p[2 * IW + u], wr.z, s);
+                                s = fmaf(p[3 * IW + u], wr.w, s);
+                                I = fmaf(s, wci[u], I);
+                            }
+                        }
+                        v *= I;
+                    }
+                }
+                if (i < a.WM) a.out[static_cast<size_t>(f) * planeM + static_cast<size_t>(j) * a.WM + i] = v;
+            }
         }
     }
 }
@@ -213,6 +251,19 @@ using SKey = std::tuple<int, int, int, uint64_t, int>;
 std::mutex s_mu;
 std::map<SKey, std::unique_ptr<SrrfCached>> s_plans;
 
+// Plan with the tallest block (64, 32 or 16 output rows) whose LDS fits two blocks per CU, else one. Returns
+// NPX_OK, NPX_ERR_ARG (bad parameters) or NPX_ERR_UNSUPPORTED (the ring does not fit a tile in LDS).
+int build_srrf_plan_fit(int H, int W, int M, double ringRadius, int border, SrrfPlan& p, std::string* err) {
+    for (size_t limit : {static_cast<size_t>(80 * 1024), static_cast<size_t>(160 * 1024)}) {
+        for (int TJ : {64, 32, 16}) {
+            if (!build_srrf_plan(H, W, M, ringRadius, border, TJ, kSrrfTI, p, err)) return NPX_ERR_ARG;
+            if (srrf_lds_bytes(p.TJ, p.TH, p.TW) <= limit) return NPX_OK;
+        }
+    }
+    if (err) *err = "ring radius too large for the radiality tile (LDS)";
+    return NPX_ERR_UNSUPPORTED;
+}
+
 uint64_t dbits(double v) {
     uint64_t b;
     std::memcpy(&b, &v, sizeof b);
@@ -228,10 +279,8 @@ int srrf_plan(int H, int W, int M, double ringRadius, int border, hipStream_t st
     if (it == s_plans.end()) {
         auto p = std::make_unique<SrrfCached>();
         std::string err;
-        if (!build_srrf_plan(H, W, M, ringRadius, border, kSrrfTJ, kSrrfTI, p->host, &err))
-            return fail(NPX_ERR_ARG, err);
-        if (srrf_lds_bytes(p->host.TH, p->host.TW) > 160 * 1024)
-            return fail(NPX_ERR_UNSUPPORTED, "ring radius too large for the radiality tile");
+        const int rc = build_srrf_plan_fit(H, W, M, ringRadius, border, p->host, &err);
+        if (rc != NPX_OK) return fail(rc, err);
         it = s_plans.emplace(key, std::move(p)).first;
     }
     SrrfCached* c = it->second.get();
@@ -306,6 +355,7 @@ int npx_radiality(const void* d_img, int dtype, const float* d_interp, int N, in
     a.M = M;
     a.HM = h->HM;
     a.WM = h->WM;
+    a.TJ = h->TJ;
     a.TH = h->TH;
     a.TW = h->TW;
     a.j_lo = h->j_lo;
@@ -314,17 +364,12 @@ int npx_radiality(const void* d_img, int dtype, const float* d_interp, int N, in
     a.i_hi = h->i_hi;
     a.positivity = positivity ? 1 : 0;
     a.iw = iw ? 1 : 0;
-    a.R = h->R;
-    for (int k = 0; k < 12; ++k) {
-        a.xr[k] = h->xr[k];
-        a.yr[k] = h->yr[k];
-    }
     const int ntx = static_cast<int>(h->tile_c0.size()), nty = static_cast<int>(h->tile_r0.size());
     const int tiles = ntx * nty;
     int chunks = std::max(1, std::min(N, (g_srrf_target_blocks + tiles - 1) / tiles));
     a.fpc = (N + chunks - 1) / chunks;
     chunks = (N + a.fpc - 1) / a.fpc;
-    const size_t lds = srrf_lds_bytes(h->TH, h->TW);
+    const size_t lds = srrf_lds_bytes(h->TJ, h->TH, h->TW);
     const dim3 grid(ntx, nty, chunks), block(kSrrfThreads);
     if (a.u16) {
         if (d_interp) hipLaunchKernelGGL((srrf_radiality<true, true>), grid, block, lds, st, a);
@@ -343,8 +388,9 @@ int npx_radiality_info(int H, int W, int M, double ringRadius, int border, int* 
     if (!info || !ring) return host::fail(NPX_ERR_ARG, "NULL output");
     SrrfPlan p;
     std::string err;
-    if (!build_srrf_plan(H, W, M, ringRadius, border, kSrrfTJ, kSrrfTI, p, &err)) return host::fail(NPX_ERR_ARG, err);
-    const int v[12] = {p.HM, p.WM, p.j_lo, p.j_hi, p.i_lo, p.i_hi, p.TH, p.TW, kSrrfTJ, kSrrfTI,
+    const int rc = build_srrf_plan_fit(H, W, M, ringRadius, border, p, &err);
+    if (rc != NPX_OK) return host::fail(rc, err);
+    const int v[12] = {p.HM, p.WM, p.j_lo, p.j_hi, p.i_lo, p.i_hi, p.TH, p.TW, p.TJ, kSrrfTI,
                        static_cast<int>(p.tile_r0.size()), static_cast<int>(p.tile_c0.size())};
     std::memcpy(info, v, sizeof v);
     ring[0] = p.R;
@@ -361,7 +407,8 @@ int npx_radiality_tables(int H, int W, int M, double ringRadius, int border, flo
     if (!row || !col || !tile_r0 || !tile_c0) return host::fail(NPX_ERR_ARG, "NULL output");
     SrrfPlan p;
     std::string err;
-    if (!build_srrf_plan(H, W, M, ringRadius, border, kSrrfTJ, kSrrfTI, p, &err)) return host::fail(NPX_ERR_ARG, err);
+    const int rc = build_srrf_plan_fit(H, W, M, ringRadius, border, p, &err);
+    if (rc != NPX_OK) return host::fail(rc, err);
     std::memcpy(row, p.row.data(), p.row.size() * sizeof(float));
     std::memcpy(col, p.col.data(), p.col.size() * sizeof(float));
     std::memcpy(tile_r0, p.tile_r0.data(), p.tile_r0.size() * sizeof(int));

@@ -48,6 +48,10 @@ CONFIGS = {
     "sweep_256": Scene(200, 256, 256, 3200, 1.5, 500, 0.3, 100, 0.0, 2.0, 0.0, "float32", 4,
                        dict(magnification=(2, 4), radius=(1.0, 1.5, 2.0, 2.5), sensitivity=(1, 2, 3, 4),
                             doIntensityWeighting=True, temporal=("AVG",))),
+    # SRRF radiality at the reference's SRRF() defaults on the plain_256 scene (per-frame maps, 1280 x 1280)
+    "srrf_256": Scene(100, 256, 256, 1500, 1.5, 1000, 0.1, 100, 0.0, 0.0, 0.0, "float32", 6,
+                      dict(magnification=5, ringRadius=0.5, border=0, radialityPositivityConstraint=True,
+                           doIntensityWeighting=True)),
 }
 
 

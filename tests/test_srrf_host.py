@@ -116,11 +116,11 @@ def _emulate(x, interp, M, R, border, pos, iw):
                     qx += w * gxp[rr, cc]
                     qy += w * gyp[rr, cc]
             g = np.sqrt(qx * qx + qy * qy)
-            cr = np.abs(qy * dx[I, k] - qx * dy[J, k])
+            cr = np.abs(qy * dx[I, k] - qx * dy[J, k])       # offsets are stored over the ring radius R
             with np.errstate(divide="ignore", invalid="ignore"):
-                dk = np.where(g > 0, (1 - cr / g / d["R"]) ** 2, 0.0)
-            dot = qx * d["xRing"][k] + qy * d["yRing"][k]
-            div += np.where(dot > 0, -dk, dk)
+                dk = np.where(g > 0, (1 - cr / g) ** 2, 0.0)
+            dot = qx * dx[I, k] + qy * dy[J, k]
+            div += np.where(dot < 0, -dk, dk)
         div /= 12
         v = np.maximum(div, 0) if pos else div
         if iw:
