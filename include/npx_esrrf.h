@@ -177,6 +177,24 @@ int npx_radiality_tables(int H, int W, int M, double ringRadius, int border, flo
                          int *tile_c0);
 
 /*
+ * eSRRF 3D of a device (F, S, R, C) stack (dtype NPX_DTYPE_F32 or NPX_DTYPE_U16) into d_out (S*Mz, R*Mxy, C*Mxy)
+ * float32: per frame the xMxy Catmull-Rom of the intensity and of the 3D gradients (z slices repeated Mz times), the
+ * radial gradient convergence of every voxel of [int(radius_z*Mz), S*Mz - int(radius_z*Mz)) x [int(radius*Mxy), ...)^2
+ * with the sensitivity power and, with iw, the intensity weight; frames folded into their running mean (mode 0,
+ * "average") or standard deviation (mode 1, "std"). d_work is npx_esrrf3d_workspace() bytes of device scratch.
+ * Replaces eSRRF3D._run_threaded / _run_opencl, src/nanopyx/core/transform/_le_esrrf3d.pyx:73-175 (per-voxel kernel
+ * src/include/_c_sr_radial_gradient_convergence.c:132-204, gradients src/include/_c_gradients.c:75-116,
+ * interpolate_3d_zlinear src/nanopyx/core/transform/_interpolation.pyx:54-103).
+ */
+size_t npx_esrrf3d_workspace(int F, int S, int R, int C, int Mxy, int Mz, int mode);
+/* Workspace bytes per chunk of frames that later npx_esrrf3d calls size their chunks by (0 leaves it); returns the
+ * previous value. No reference counterpart: the chunking is this library's. */
+size_t npx_esrrf3d_chunk_bytes(size_t bytes);
+int npx_esrrf3d(const void *d_img, int dtype, int F, int S, int R, int C, int Mxy, int Mz, double radius,
+                double radius_z, double sensitivity, int mode, int iw, float *d_out, void *d_work, size_t work_bytes,
+                void *stream);
+
+/*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
  */

@@ -7,17 +7,19 @@ Drop-in entry points with the reference's signatures:
   * ``calculate_eSRRF_temporal_correlations``, ``macro_pixel_corrector``
   * ``nanopyx_amd.SRRF(...)`` / ``core.transform.Radiality`` — SRRF radiality (src/nanopyx/methods/srrf/
     SRRF_workflow.py:9-76, src/nanopyx/core/transform/_le_radiality.pyx:27-363) on one fused kernel
+  * ``nanopyx_amd.eSRRF3D(...)`` / ``core.transform.eSRRF3D_ST`` — eSRRF 3D (src/nanopyx/methods/esrrf_3d/
+    eSRRF3D_workflow.py:8-119, src/nanopyx/core/transform/_le_esrrf3d.pyx:26-687)
   * ``run_esrrf_parameter_sweep`` / ``core.analysis.ParameterSweep`` — every (sensitivity, radius) reconstruction
     in one batched launch, scored by QnR on the host (src/nanopyx/core/analysis/parameter_sweep.py:11-139)
   * ``calculate_frc``, ``calculate_decorr_analysis``, ``calculate_error_map`` — the metrics the sweep scores with
 """
-from .methods import (SRRF, eSRRF, Workflow, calculate_decorr_analysis, calculate_error_map, calculate_frc,
+from .methods import (SRRF, eSRRF, eSRRF3D, Workflow, calculate_decorr_analysis, calculate_error_map, calculate_frc,
                       run_esrrf_parameter_sweep)
 from .core.transform import (Radiality, RadialGradientConvergence, calculate_eSRRF_temporal_correlations, eSRRF_ST,
                              macro_pixel_corrector)
 
 __version__ = "0.1.0"
 
-__all__ = ["eSRRF", "SRRF", "Workflow", "run_esrrf_parameter_sweep", "eSRRF_ST", "RadialGradientConvergence",
+__all__ = ["eSRRF", "eSRRF3D", "SRRF", "Workflow", "run_esrrf_parameter_sweep", "eSRRF_ST", "RadialGradientConvergence",
            "Radiality", "calculate_eSRRF_temporal_correlations", "macro_pixel_corrector", "calculate_frc",
            "calculate_decorr_analysis", "calculate_error_map"]

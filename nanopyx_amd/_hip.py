@@ -96,6 +96,12 @@ def lib():
                                     _c_int, _c_int, _c_void_p, _c_void_p]
         L.npx_radiality_info.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _ip, _fp]
         L.npx_radiality_tables.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _fp, _fp, _ip, _ip]
+        L.npx_esrrf3d_chunk_bytes.argtypes = [_c_size_t]
+        L.npx_esrrf3d_chunk_bytes.restype = _c_size_t
+        L.npx_esrrf3d_workspace.argtypes = [_c_int] * 7
+        L.npx_esrrf3d_workspace.restype = _c_size_t
+        L.npx_esrrf3d.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double,
+                                  _c_double, _c_double, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -412,6 +418,47 @@ def radiality(stack, interp, magnification, ringRadius, border, positivity, iw, 
     _check(lib().npx_radiality(x.data_ptr(), _dtype_code(x), ip, N, H, W, M, float(ringRadius), int(border),
                                int(bool(positivity)), int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
     return out
+
+
+ESRRF3D_MODES = {"average": 0, "std": 1}
+
+
+def esrrf3d(stack, magnification_xy, magnification_z, radius, radius_z, sensitivity, mode, iw, stream=None):
+    """eSRRF 3D of an (F, S, R, C) stack (float32 or uint16): the (S*Mz, R*Mxy, C*Mxy) running mean ("average") or
+    standard deviation ("std") over frames, as a device tensor."""
+    torch = _torch()
+    if isinstance(stack, torch.Tensor):
+        x = stack if stack.dtype in (torch.float32, torch.uint16) else stack.to(torch.float32)
+        x = x.cuda() if not x.is_cuda else x
+    else:
+        a = np.asarray(stack)
+        if a.dtype != np.uint16 and a.dtype != np.float32:
+            a = a.astype(np.float32)
+        x = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if x.dim() == 3:
+        x = x.unsqueeze(0)
+    if x.dim() != 4:
+        raise ValueError("image must either be 3D or 4D")
+    x = x.contiguous()
+    if mode not in ESRRF3D_MODES:
+        raise ValueError("mode must be 'average' or 'std'")
+    F, S, R, C = x.shape
+    Mxy, Mz = int(magnification_xy), int(magnification_z)
+    m = ESRRF3D_MODES[mode]
+    need = int(lib().npx_esrrf3d_workspace(F, S, R, C, Mxy, Mz, m))
+    if need == 0:
+        raise ValueError("bad eSRRF 3D shape")
+    work = torch.empty(need, dtype=torch.uint8, device=x.device)
+    out = torch.empty((S * Mz, R * Mxy, C * Mxy), dtype=torch.float32, device=x.device)
+    _check(lib().npx_esrrf3d(x.data_ptr(), _dtype_code(x), F, S, R, C, Mxy, Mz, float(radius), float(radius_z),
+                             float(sensitivity), m, int(bool(iw)), out.data_ptr(), work.data_ptr(), need,
+                             _stream(torch, stream)))
+    return out
+
+
+def esrrf3d_chunk_bytes(nbytes=0) -> int:
+    """Sets the workspace bytes per frame chunk of later eSRRF 3D calls (0 keeps it); returns the previous value."""
+    return int(lib().npx_esrrf3d_chunk_bytes(int(nbytes)))
 
 
 def radiality_info(H, W, M, ringRadius, border):

@@ -1,0 +1,376 @@
+// eSRRF 3D on gfx950 (eSRRF3D, src/nanopyx/core/transform/_le_esrrf3d.pyx:73-175; per-voxel kernel
+// src/include/_c_sr_radial_gradient_convergence.c:92-204; gradients src/include/_c_gradients.c:75-116;
+// interpolation src/nanopyx/core/transform/_interpolation.pyx:54-103), and its C ABI entry points.
+//
+// Per chunk of frames: (1) grad3d: x and y central differences / 2 and the z backward difference / 2 of every
+// voxel (neighbours clamped); (2) magnify_xy: Catmull-Rom xMxy of the intensity and the three gradients of every
+// slice (identity at Mxy = 1), stored interleaved as float4 (Gx, Gy, Gz, I) per magnified pixel of each input slice;
+// the reference's z "linear" step repeats each slice Mz times (its slice position is an integer), so no z copy is
+// made: magnified slice zM reads input slice zM / Mz; (3) rgc3d: one thread per output voxel walks the chunk's
+// frames, sums the window samples (one float4 gather each), applies the sensitivity power and the intensity weight,
+// and folds the value into the running mean (or Welford M2 and mean) kept in registers, reading and writing the
+// caller's state once per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/npx_esrrf.h"
+#include "npx_host.h"
+#include "npx_plan.h"
+
+namespace npx {
+
+struct E3Args {
+    const void* in;       // (F, S, R, C) float32 or uint16
+    float4* mag;          // (chunk, S, RM, CM) float4: Gx, Gy, Gz, intensity
+    float* grad;          // (chunk, 3, S, R, C)
+    float* out;           // (SM, RM, CM): running mean (average) or M2 (std)
+    float* mean;          // (SM, RM, CM) running mean of the std mode
+    const int* crb_r;     // [RM] first tap row,    crw_r [RM] 4 weights
+    const float4* crw_r;
+    const int* crb_c;     // [CM]
+    const float4* crw_c;
+    int u16, F, S, R, C, Mxy, Mz, SM, RM, CM;
+    int f0, nf;           // first frame of the chunk, frames in the chunk
+    int margin, margin_z;
+    int j0, j1, k0, k1;   // window: x/y offsets [j0, j1], z offsets [k0, k1]
+    float tSO, tSO_z, tSS, tSS_z, S_pow;
+    int pow_on, mode, iw, last;  // last: this chunk ends the stack (std: take sqrt(M2 / F))
+};
+
+__device__ inline float e3_load(const E3Args& a, size_t o) {
+    return a.u16 ? static_cast<float>(static_cast<const uint16_t*>(a.in)[o]) : static_cast<const float*>(a.in)[o];
+}
+
+// (1) gradients of every voxel of the chunk's frames
+__global__ __launch_bounds__(256) void e3_grad3d(E3Args a) {
+    const size_t vol = static_cast<size_t>(a.S) * a.R * a.C;
+    const size_t n = vol * a.nf;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t fl = t / vol, v = t % vol;
+        const int x = static_cast<int>(v % a.C), y = static_cast<int>((v / a.C) % a.R),
+                  z = static_cast<int>(v / (static_cast<size_t>(a.R) * a.C));
+        const size_t base = (a.f0 + fl) * vol, plane = static_cast<size_t>(a.R) * a.C;
+        const int zm = z <= 0 ? 0 : z - 1;
+        const int yp = y >= a.R - 1 ? a.R - 1 : y + 1, ym = y <= 0 ? 0 : y - 1;
+        const int xp = x >= a.C - 1 ? a.C - 1 : x + 1, xm = x <= 0 ? 0 : x - 1;
+        const size_t row = base + z * plane + static_cast<size_t>(y) * a.C;
+        const float c = e3_load(a, row + x);
+        const float gx = (e3_load(a, row + xp) - e3_load(a, row + xm)) / 2;
+        const float gy = (e3_load(a, base + z * plane + static_cast<size_t>(yp) * a.C + x) -
+                          e3_load(a, base + z * plane + static_cast<size_t>(ym) * a.C + x)) / 2;
+        const float gz = (c - e3_load(a, base + zm * plane + static_cast<size_t>(y) * a.C + x)) / 2;
+        float* g = a.grad + fl * 3 * vol;
+        g[v] = gx;
+        g[vol + v] = gy;
+        g[2 * vol + v] = gz;
+    }
+}
+
+// (2) Catmull-Rom xMxy of (Gx, Gy, Gz, intensity) of every slice: columns outer, rows inner, as _c_interpolate
+__global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
+    const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, vol = static_cast<size_t>(a.S) * a.R * a.C;
+    const size_t n = volM * a.nf, plane = static_cast<size_t>(a.R) * a.C;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t fl = t / volM, v = t % volM;
+        const int cM = static_cast<int>(v % a.CM), rM = static_cast<int>((v / a.CM) % a.RM),
+                  s = static_cast<int>(v / (static_cast<size_t>(a.RM) * a.CM));
+        const float* g = a.grad + fl * 3 * vol + s * plane;
+        const size_t ibase = (a.f0 + fl) * vol + s * plane;
+        float4 r;
+        if (a.Mxy == 1) {
+            const size_t o = static_cast<size_t>(rM) * a.C + cM;
+            r = make_float4(g[o], g[vol + o], g[2 * vol + o], e3_load(a, ibase + o));
+        } else {
+            const int rb = a.crb_r[rM], cb = a.crb_c[cM];
+            const float4 wr = a.crw_r[rM], wc = a.crw_c[cM];
+            const float wrs[4] = {wr.x, wr.y, wr.z, wr.w}, wcs[4] = {wc.x, wc.y, wc.z, wc.w};
+            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int cn = cb + u;
+                if (wcs[u] == 0.f || cn < 0 || cn >= a.C) continue;
+                float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const int rn = rb + w;
+                    if (wrs[w] == 0.f || rn < 0 || rn >= a.R) continue;
+                    const size_t o = static_cast<size_t>(rn) * a.C + cn;
+                    p0 = fmaf(g[o], wrs[w], p0);
+                    p1 = fmaf(g[vol + o], wrs[w], p1);
+                    p2 = fmaf(g[2 * vol + o], wrs[w], p2);
+                    p3 = fmaf(e3_load(a, ibase + o), wrs[w], p3);
+                }
+                q0 = fmaf(p0, wcs[u], q0);
+                q1 = fmaf(p1, wcs[u], q1);
+                q2 = fmaf(p2, wcs[u], q2);
+                q3 = fmaf(p3, wcs[u], q3);
+            }
+            r = make_float4(q0, q1, q2, q3);
+        }
+        a.mag[t] = r;
+    }
+}
+
+// (3) RGC of every output voxel over the chunk's frames, folded into the running statistics
+__global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
+    const size_t nvox = static_cast<size_t>(a.SM) * a.RM * a.CM;
+    const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, planeM = static_cast<size_t>(a.RM) * a.CM;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < nvox;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int cM = static_cast<int>(t % a.CM), rM = static_cast<int>((t / a.CM) % a.RM),
+                  sM = static_cast<int>(t / planeM);
+        const bool in = sM >= a.margin_z && sM < a.SM - a.margin_z && rM >= a.margin && rM < a.RM - a.margin &&
+                        cM >= a.margin && cM < a.CM - a.margin;
+        float acc = a.f0 == 0 ? 0.f : a.out[t];
+        float mu = (a.mode == 1 && a.f0 > 0) ? a.mean[t] : 0.f;
+        if (in) {
+            // window coordinates exactly as the reference forms them (float32)
+            const float xc = static_cast<float>(cM) / a.Mxy, yc = static_cast<float>(rM) / a.Mxy;
+            const float zc = static_cast<float>(sM) / a.Mz;
+            const float ylim = static_cast<float>(a.RM / a.Mxy - 1), xlim = static_cast<float>(a.CM / a.Mxy - 1);
+            const float zlim = static_cast<float>(a.SM / a.Mz - 1);
+            const size_t self = static_cast<size_t>(sM / a.Mz) * planeM + static_cast<size_t>(rM) * a.CM + cM;
+            for (int fl = 0; fl < a.nf; ++fl) {
+                const float4* m = a.mag + fl * volM;
+                float rgc = 0.f, wsum = 0.f;
+                for (int j = a.j0; j <= a.j1; ++j) {
+                    const float vy = yc + j;
+                    if (!(0.f < vy && vy <= ylim)) continue;
+                    const float dy = vy - yc;
+                    const size_t oy = static_cast<size_t>(static_cast<int>(a.Mxy * vy)) * a.CM;
+                    for (int i = a.j0; i <= a.j1; ++i) {
+                        const float vx = xc + i;
+                        if (!(0.f < vx && vx <= xlim)) continue;
+                        const float dx = vx - xc;
+                        const float dxy2 = dx * dx + dy * dy;
+                        const float dxy = sqrtf(dxy2);
+                        if (!(dxy <= a.tSO)) continue;
+                        const size_t oyx = oy + static_cast<int>(a.Mxy * vx);
+                        const float wxy = __expf(-dxy2 / a.tSS);
+                        for (int k = a.k0; k <= a.k1; ++k) {
+                            const float vz = zc + k;
+                            if (!(0.f < vz && vz <= zlim)) continue;
+                            const float dz = vz - zc;
+                            const float d2 = dxy2 + dz * dz;
+                            if (d2 == 0.f || !(dz <= a.tSO_z)) continue;
+                            const float dist = sqrtf(d2);
+                            const int zs = static_cast<int>(vz * a.Mz) / a.Mz;  // source slice of the repeated volume
+                            const float4 G = m[static_cast<size_t>(zs) * planeM + oyx];
+                            float w = dist * (wxy * __expf(-dz * dz / a.tSS_z));
+                            w = (w * w) * (w * w);
+                            wsum += w;
+                            const float gdr = fmaf(G.x, dx, fmaf(G.y, dy, G.z * dz));
+                            if (gdr < 0.f) {
+                                const float c1 = fmaf(G.y, dz, -G.z * dy), c2 = fmaf(G.z, dx, -G.x * dz),
+                                            c3 = fmaf(G.x, dy, -G.y * dx);
+                                const float cr2 = fmaf(c1, c1, fmaf(c2, c2, c3 * c3));
+                                const float g2 = fmaf(G.x, G.x, fmaf(G.y, G.y, G.z * G.z));
+                                const float dk = 1.f - sqrtf(cr2 / g2) / dist;
+                                rgc = fmaf(dk, w, rgc);
+                            }
+                        }
+                    }
+                }
+                rgc = rgc / wsum;
+                if (rgc >= 0.f && a.pow_on) rgc = powf(rgc, a.S_pow);
+                else if (rgc < 0.f) rgc = 0.f;
+                if (a.iw) rgc *= m[self].w;
+                const float fn = static_cast<float>(a.f0 + fl + 1);
+                if (a.mode == 0) {
+                    acc = acc + (rgc - acc) / fn;
+                } else {
+                    const float delta = rgc - mu;
+                    mu += delta / fn;
+                    acc += delta * (rgc - mu);
+                }
+            }
+        }
+        if (a.mode == 1 && !a.last) a.mean[t] = mu;
+        if (a.mode == 1 && a.last) acc = sqrtf(acc / static_cast<float>(a.F));
+        a.out[t] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------------ host side
+
+namespace {
+
+using npx::host::fail;
+using npx::host::hip_fail;
+
+struct E3Dev {
+    void* mem = nullptr;
+    const int *crb_r = nullptr, *crb_c = nullptr;
+    const float4 *crw_r = nullptr, *crw_c = nullptr;
+};
+
+std::mutex e3_mu;
+std::map<std::tuple<int, int, int, int>, E3Dev> e3_tables;  // (device, R, C, Mxy)
+
+int e3_tables_for(int R, int C, int Mxy, hipStream_t st, const E3Dev** out) {
+    std::lock_guard<std::mutex> lk(e3_mu);
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    const auto key = std::make_tuple(d, R, C, Mxy);
+    auto it = e3_tables.find(key);
+    if (it == e3_tables.end()) {
+        if (npx::host::capturing(st)) return fail(NPX_ERR_ARG, "first use of an eSRRF 3D plan under capture");
+        const int RM = R * Mxy, CM = C * Mxy;
+        std::vector<int> br(RM), bc(CM);
+        std::vector<float4> wr(RM), wc(CM);
+        const float mag = static_cast<float>(Mxy);
+        for (int r = 0; r < RM; ++r) {
+            const CrTap t = cr_taps(r, mag, R);
+            br[r] = t.base;
+            wr[r] = make_float4(t.w[0], t.w[1], t.w[2], t.w[3]);
+        }
+        for (int c = 0; c < CM; ++c) {
+            const CrTap t = cr_taps(c, mag, C);
+            bc[c] = t.base;
+            wc[c] = make_float4(t.w[0], t.w[1], t.w[2], t.w[3]);
+        }
+        const size_t bwr = RM * sizeof(float4), bwc = CM * sizeof(float4), bbr = RM * sizeof(int);
+        const size_t bbc = CM * sizeof(int);
+        std::vector<uint8_t> blob(bwr + bwc + bbr + bbc);
+        std::memcpy(blob.data(), wr.data(), bwr);
+        std::memcpy(blob.data() + bwr, wc.data(), bwc);
+        std::memcpy(blob.data() + bwr + bwc, br.data(), bbr);
+        std::memcpy(blob.data() + bwr + bwc + bbr, bc.data(), bbc);
+        void* mem = nullptr;
+        e = hipMalloc(&mem, blob.size());
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(eSRRF 3D tables)");
+        e = hipMemcpy(mem, blob.data(), blob.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            (void)hipFree(mem);
+            return hip_fail(e, "upload of eSRRF 3D tables");
+        }
+        auto* b = static_cast<uint8_t*>(mem);
+        E3Dev t;
+        t.mem = mem;
+        t.crw_r = reinterpret_cast<const float4*>(b);
+        t.crw_c = reinterpret_cast<const float4*>(b + bwr);
+        t.crb_r = reinterpret_cast<const int*>(b + bwr + bwc);
+        t.crb_c = reinterpret_cast<const int*>(b + bwr + bwc + bbr);
+        it = e3_tables.emplace(key, t).first;
+    }
+    *out = &it->second;
+    return NPX_OK;
+}
+
+size_t g_e3_chunk_bytes = size_t(1) << 30;  // workspace per chunk of frames (magnified volumes + gradients)
+
+int e3_chunk(int F, int S, int R, int C, int Mxy) {
+    const size_t per = static_cast<size_t>(S) * R * Mxy * C * Mxy * sizeof(float4) +
+                       static_cast<size_t>(3) * S * R * C * sizeof(float);
+    return static_cast<int>(std::max<size_t>(1, std::min<size_t>(static_cast<size_t>(F), g_e3_chunk_bytes / per)));
+}
+
+int e3_grid(size_t n) { return static_cast<int>(std::min<size_t>((n + 255) / 256, size_t(65536))); }
+
+}  // namespace
+}  // namespace npx
+
+extern "C" {
+
+size_t npx_esrrf3d_chunk_bytes(size_t bytes) {
+    const size_t prev = npx::g_e3_chunk_bytes;
+    if (bytes) npx::g_e3_chunk_bytes = bytes;
+    return prev;
+}
+
+size_t npx_esrrf3d_workspace(int F, int S, int R, int C, int Mxy, int Mz, int mode) {
+    if (F < 1 || S < 1 || R < 1 || C < 1 || Mxy < 1 || Mz < 1) return 0;
+    const int ch = npx::e3_chunk(F, S, R, C, Mxy);
+    const size_t mag = static_cast<size_t>(ch) * S * R * Mxy * C * Mxy * sizeof(float4);
+    const size_t grad = static_cast<size_t>(ch) * 3 * S * R * C * sizeof(float);
+    const size_t mean = mode == 1 ? static_cast<size_t>(S) * Mz * R * Mxy * C * Mxy * sizeof(float) : 0;
+    return ((mag + 255) & ~size_t(255)) + ((grad + 255) & ~size_t(255)) + mean;
+}
+
+int npx_esrrf3d(const void* d_img, int dtype, int F, int S, int R, int C, int Mxy, int Mz, double radius,
+                double radius_z, double sensitivity, int mode, int iw, float* d_out, void* d_work, size_t work_bytes,
+                void* stream) {
+    using namespace npx;
+    if (!d_img || !d_out || !d_work) return host::fail(NPX_ERR_ARG, "NULL array pointer");
+    if (dtype != NPX_DTYPE_F32 && dtype != NPX_DTYPE_U16) return host::fail(NPX_ERR_ARG, "dtype must be f32 or u16");
+    if (F < 1 || S < 1 || R < 1 || C < 1 || Mxy < 1 || Mz < 1) return host::fail(NPX_ERR_ARG, "bad shape");
+    if (mode != 0 && mode != 1) return host::fail(NPX_ERR_ARG, "mode must be average (0) or std (1)");
+    if (!(radius >= 0) || !(radius_z >= 0)) return host::fail(NPX_ERR_ARG, "radii must be >= 0");
+    const size_t need = npx_esrrf3d_workspace(F, S, R, C, Mxy, Mz, mode);
+    if (work_bytes < need) return host::fail(NPX_ERR_WORKSPACE, "workspace smaller than npx_esrrf3d_workspace()");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const E3Dev* tb = nullptr;
+    int rc = e3_tables_for(R, C, Mxy, st, &tb);
+    if (rc != NPX_OK) return rc;
+    // parameters as src/nanopyx/core/transform/_le_esrrf3d.pyx:80-92 derives them
+    const float sigma = static_cast<float>(radius / 2.355);
+    const float sigma_z = static_cast<float>(radius_z / 2.355);
+    const float fwhm = static_cast<float>(radius), fwhm_z = static_cast<float>(radius_z);
+    E3Args a{};
+    a.in = d_img;
+    a.u16 = dtype == NPX_DTYPE_U16;
+    a.F = F;
+    a.S = S;
+    a.R = R;
+    a.C = C;
+    a.Mxy = Mxy;
+    a.Mz = Mz;
+    a.SM = S * Mz;
+    a.RM = R * Mxy;
+    a.CM = C * Mxy;
+    a.margin = static_cast<int>(radius * Mxy);
+    a.margin_z = static_cast<int>(radius_z * Mz);
+    a.tSS = 2 * sigma * sigma;
+    a.tSO = 2 * sigma + 1;
+    a.tSS_z = 2 * sigma_z * sigma_z;
+    a.tSO_z = 2 * sigma_z + 1;
+    a.j0 = -static_cast<int>(fwhm);
+    a.j1 = static_cast<int>(fwhm + 1);
+    a.k0 = -static_cast<int>(fwhm_z);
+    a.k1 = static_cast<int>(fwhm_z + 1);
+    a.S_pow = static_cast<float>(sensitivity);
+    a.pow_on = a.S_pow > 1.0f;
+    a.mode = mode;
+    a.iw = iw ? 1 : 0;
+    a.out = d_out;
+    a.crb_r = tb->crb_r;
+    a.crw_r = tb->crw_r;
+    a.crb_c = tb->crb_c;
+    a.crw_c = tb->crw_c;
+    const int ch = e3_chunk(F, S, R, C, Mxy);
+    const size_t mag = static_cast<size_t>(ch) * S * a.RM * a.CM * sizeof(float4);
+    const size_t grad = static_cast<size_t>(ch) * 3 * S * R * C * sizeof(float);
+    auto* w = static_cast<uint8_t*>(d_work);
+    a.mag = reinterpret_cast<float4*>(w);
+    a.grad = reinterpret_cast<float*>(w + ((mag + 255) & ~size_t(255)));
+    a.mean = mode == 1 ? reinterpret_cast<float*>(w + ((mag + 255) & ~size_t(255)) + ((grad + 255) & ~size_t(255)))
+                       : nullptr;
+    const size_t nvox = static_cast<size_t>(a.SM) * a.RM * a.CM;
+    for (int f0 = 0; f0 < F; f0 += ch) {
+        a.f0 = f0;
+        a.nf = std::min(ch, F - f0);
+        a.last = f0 + a.nf >= F;
+        const size_t ng = static_cast<size_t>(a.nf) * S * R * C, nm = static_cast<size_t>(a.nf) * S * a.RM * a.CM;
+        hipLaunchKernelGGL(e3_grad3d, dim3(e3_grid(ng)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(e3_magnify_xy, dim3(e3_grid(nm)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(e3_rgc3d, dim3(e3_grid(nvox)), dim3(256), 0, st, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return host::hip_fail(e, "eSRRF 3D launch");
+    }
+    return NPX_OK;
+}
+
+}  // extern "C"

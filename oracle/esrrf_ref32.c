@@ -391,3 +391,191 @@ void ora_radiality(const float *image, const float *image_interp, int nFrames, i
     free(gx);
     free(gy);
 }
+
+/* ---------------------------------------------------------------- eSRRF 3D (restated) */
+
+/* src/include/_c_gradients.c:75-116 — per voxel: x and y central differences / 2, z backward difference / 2,
+ * neighbours clamped to the volume. */
+static void ora_gradient_3d(float *image, float *imGc, float *imGr, float *imGs, int slices, int rows, int cols) {
+    for (int z = 0; z < slices; z++) {
+        for (int y = 0; y < rows; y++) {
+            for (int x = 0; x < cols; x++) {
+                int zm = z <= 0 ? 0 : z - 1;
+                int yp = y >= rows - 1 ? rows - 1 : y + 1, ym = y <= 0 ? 0 : y - 1;
+                int xp = x >= cols - 1 ? cols - 1 : x + 1, xm = x <= 0 ? 0 : x - 1;
+                size_t plane = (size_t)rows * cols, o = z * plane + (size_t)y * cols + x;
+                float c = image[o];
+                float yplus = image[z * plane + (size_t)yp * cols + x];
+                float xplus = image[z * plane + (size_t)y * cols + xp];
+                float yminus = image[z * plane + (size_t)ym * cols + x];
+                float xminus = image[z * plane + (size_t)y * cols + xm];
+                float zminus = image[zm * plane + (size_t)y * cols + x];
+                imGc[o] = (xplus - xminus) / 2;
+                imGr[o] = (yplus - yminus) / 2;
+                imGs[o] = (c - zminus) / 2;
+            }
+        }
+    }
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:96-102 — the factors are float, the power is double. */
+static double ora_dw3d(double distance, double distance_xy, double distance_z, double tSS, double tSS_z) {
+    float wxy, wz, w;
+    wxy = (exp((-distance_xy * distance_xy) / tSS));
+    wz = (exp((-distance_z * distance_z) / tSS_z));
+    w = distance * (wxy * wz);
+    return pow(w, 4);
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:112-119 — 1 - |G x d| / (|G| |d|); NaN (G = 0) gives 0. */
+static double ora_dk3d(float Gx, float Gy, float Gz, float dx, float dy, float dz, float distance) {
+    float Dk = sqrt((Gy * dz - Gz * dy) * (Gy * dz - Gz * dy) + (Gz * dx - Gx * dz) * (Gz * dx - Gx * dz) +
+                    (Gx * dy - Gy * dx) * (Gx * dy - Gy * dx)) / sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+    if (isnan(Dk)) Dk = distance;
+    Dk = 1 - Dk / distance;
+    return Dk;
+}
+
+/* src/include/_c_sr_radial_gradient_convergence.c:132-204 — window [-int(fwhm), int(fwhm + 1)] in x and y,
+ * [-int(fwhm_z), int(fwhm_z + 1)] in z around (xM/Mxy, yM/Mxy, sliceM/Mz); samples inside the volume (strictly
+ * positive coordinates), d != 0, d_xy <= tSO and (signed) d_z <= tSO_z; gradients read from the magnified volumes at
+ * (int(vz Mz), int(Mxy vy), int(Mxy vx)); RGC = sum(Dk w over G.d < 0) / sum(w), then ^S when S > 1, 0 if < 0. */
+static float ora_rgc3d(int xM, int yM, int sliceM, float *imIntGx, float *imIntGy, float *imIntGz, int colsM,
+                       int rowsM, int slicesM, int magnification_xy, int magnification_z, float voxel_ratio,
+                       float fwhm, float fwhm_z, float tSO, float tSO_z, float tSS, float tSS_z, float sensitivity) {
+    float vx, vy, vz, Gx, Gy, Gz, dx, dy, dz, distance, distance_xy, w, GdotR, Dk;
+    (void)voxel_ratio;
+    float xc = (float)(xM) / magnification_xy;
+    float yc = (float)(yM) / magnification_xy;
+    float zc = (float)(sliceM) / magnification_z;
+    float RGC = 0;
+    float wsum = 0;
+    int j0 = -(int)(fwhm), j1 = (int)(fwhm + 1);
+    int k0 = -(int)(fwhm_z), k1 = (int)(fwhm_z + 1);
+    for (int j = j0; j <= j1; j++) {
+        vy = yc + j;
+        if (!(0 < vy && vy <= (rowsM / magnification_xy) - 1)) continue;
+        for (int i = j0; i <= j1; i++) {
+            vx = xc + i;
+            if (!(0 < vx && vx <= (colsM / magnification_xy) - 1)) continue;
+            for (int k = k0; k <= k1; k++) {
+                vz = zc + k;
+                if (!(0 < vz && vz <= (slicesM / magnification_z) - 1)) continue;
+                dx = vx - xc;
+                dy = vy - yc;
+                dz = vz - zc;
+                distance = sqrt(dx * dx + dy * dy + dz * dz);
+                distance_xy = sqrt(dx * dx + dy * dy);
+                if (distance != 0 && distance_xy <= tSO && dz <= tSO_z) {
+                    int idx = (int)(vz * magnification_z) * rowsM * colsM + (int)(magnification_xy * vy) * colsM +
+                              (int)(magnification_xy * vx);
+                    Gx = imIntGx[idx];
+                    Gy = imIntGy[idx];
+                    Gz = imIntGz[idx];
+                    w = ora_dw3d(distance, distance_xy, dz, tSS, tSS_z);
+                    wsum += w;
+                    GdotR = Gx * dx + Gy * dy + Gz * dz;
+                    if (GdotR < 0) {
+                        Dk = ora_dk3d(Gx, Gy, Gz, dx, dy, dz, distance);
+                        RGC += (Dk * w);
+                    }
+                }
+            }
+        }
+    }
+    RGC /= wsum;
+    if (RGC >= 0 && sensitivity > 1) {
+        RGC = pow(RGC, sensitivity);
+    } else if (RGC < 0) {
+        RGC = 0;
+    }
+    return RGC;
+}
+
+typedef float (*rgc3d_fn_t)(int, int, int, float *, float *, float *, int, int, int, int, int, float, float, float,
+                            float, float, float, float, float);
+typedef void (*grad3d_fn_t)(float *, float *, float *, float *, int, int, int);
+static rgc3d_fn_t k_rgc3d = ora_rgc3d;
+static grad3d_fn_t k_grad3d = ora_gradient_3d;
+
+/* Swap the 3D kernels for the reference's _c_calculate_rgc3D / _c_gradient_3d (NULL restores the restatement). */
+void ora_use_3d_kernels(void *rgc3d, void *grad3d) {
+    k_rgc3d = rgc3d ? (rgc3d_fn_t)rgc3d : ora_rgc3d;
+    k_grad3d = grad3d ? (grad3d_fn_t)grad3d : ora_gradient_3d;
+}
+
+/* interpolate_3d_zlinear (src/nanopyx/core/transform/_interpolation.pyx:91-103, :54-88): Catmull-Rom xMxy of every
+ * slice (skipped when Mxy = 1), then the z step, whose integer slice position makes it a repetition of each slice
+ * Mz times (weight1 = slc - slice0 = 0). */
+static void ora_interpolate_3d_zlinear(const float *vol, int slices, int rows, int cols, int Mxy, int Mz,
+                                       float *xy_scratch, float *out) {
+    int rM = rows * Mxy, cM = cols * Mxy;
+    size_t pM = (size_t)rM * cM;
+    if (Mxy > 1)
+        ora_shift_magnify(vol, slices, rows, cols, 0, 0, (float)Mxy, (float)Mxy, xy_scratch);
+    else
+        memcpy(xy_scratch, vol, sizeof(float) * pM * slices);
+    for (int sM = 0; sM < slices * Mz; sM++) memcpy(out + sM * pM, xy_scratch + (size_t)(sM / Mz) * pM, sizeof(float) * pM);
+}
+
+/* eSRRF3D._run_threaded: src/nanopyx/core/transform/_le_esrrf3d.pyx:73-175. Parameters derived as there (float32
+ * sigma, tSS, tSO; margins int(radius * Mxy), int(radius_z * Mz)); per frame: magnified intensity, 3D gradients,
+ * magnified gradients, RGC over [margin_z, SM - margin_z) x [margin, RM - margin) x [margin, CM - margin), times the
+ * magnified intensity, folded into a running mean (mode 0) or Welford M2 (mode 1, then sqrt(M2 / n)). */
+int ora_esrrf3d(const float *in, int nFrames, int slices, int rows, int cols, int Mxy, int Mz, double radius,
+                double radius_z, double voxel_ratio, double sensitivity, int mode, int doIntensityWeighting,
+                float *out) {
+    float sigma = (float)(radius / 2.355);
+    int margin = (int)(radius * Mxy);
+    float tSS = 2 * sigma * sigma;
+    float tSO = 2 * sigma + 1;
+    float sigma_z = (float)(radius_z / 2.355);
+    int margin_z = (int)(radius_z * Mz);
+    float tSS_z = 2 * sigma_z * sigma_z;
+    float tSO_z = 2 * sigma_z + 1;
+    float fwhm = (float)radius, fwhm_z = (float)radius_z, vr = (float)voxel_ratio, S = (float)sensitivity;
+    int SM = slices * Mz, RM = rows * Mxy, CM = cols * Mxy;
+    size_t p = (size_t)slices * rows * cols, pxy = (size_t)slices * RM * CM, pM = (size_t)SM * RM * CM;
+    float *scratch = (float *)malloc(sizeof(float) * pxy);
+    float *interp = (float *)malloc(sizeof(float) * pM);
+    float *g[3], *gm[3];
+    int ok = scratch && interp;
+    for (int c = 0; c < 3; c++) {
+        g[c] = (float *)calloc(p, sizeof(float));
+        gm[c] = (float *)malloc(sizeof(float) * pM);
+        ok = ok && g[c] && gm[c];
+    }
+    float *mean = mode == 1 ? (float *)calloc(pM, sizeof(float)) : NULL;
+    if (!ok || (mode == 1 && !mean)) return -1;
+    memset(out, 0, sizeof(float) * pM);
+    for (int f = 0; f < nFrames; f++) {
+        const float *vol = in + f * p;
+        ora_interpolate_3d_zlinear(vol, slices, rows, cols, Mxy, Mz, scratch, interp);
+        k_grad3d((float *)vol, g[0], g[1], g[2], slices, rows, cols);
+        for (int c = 0; c < 3; c++) ora_interpolate_3d_zlinear(g[c], slices, rows, cols, Mxy, Mz, scratch, gm[c]);
+        for (int sM = margin_z; sM < SM - margin_z; sM++) {
+#pragma omp parallel for schedule(static)
+            for (int rM = margin; rM < RM - margin; rM++) {
+                for (int cM = margin; cM < CM - margin; cM++) {
+                    float v = k_rgc3d(cM, rM, sM, gm[0], gm[1], gm[2], CM, RM, SM, Mxy, Mz, vr, fwhm, fwhm_z, tSO,
+                                      tSO_z, tSS, tSS_z, S);
+                    size_t o = (size_t)sM * RM * CM + (size_t)rM * CM + cM;
+                    if (doIntensityWeighting) v = v * interp[o];
+                    if (mode == 0) {
+                        out[o] = out[o] + (v - out[o]) / (f + 1);
+                    } else {
+                        float delta = v - mean[o];
+                        mean[o] += delta / (f + 1);
+                        float delta2 = v - mean[o];
+                        out[o] += delta * delta2;
+                    }
+                }
+            }
+        }
+    }
+    if (mode == 1)
+        for (size_t o = 0; o < pM; o++) out[o] = sqrtf(out[o] / (float)nFrames);
+    free(scratch); free(interp); free(mean);
+    for (int c = 0; c < 3; c++) { free(g[c]); free(gm[c]); }
+    return 0;
+}

@@ -57,6 +57,11 @@ def lib():
         L.ora_px_rgc.restype = ctypes.c_float
         L.ora_px_roberts.argtypes = [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.ora_num_threads.restype = ctypes.c_int
+        L.ora_use_3d_kernels.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ora_esrrf3d.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_int, _fp]
+        L.ora_esrrf3d.restype = ctypes.c_int
         L.ora_use_radiality_kernels.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.ora_px_radiality.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, _fp, ctypes.c_int, ctypes.c_float,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -98,6 +103,20 @@ def use_reference_radiality(on: bool) -> None:
         _ref_srrf = ctypes.CDLL(_REF_SRRF_PATH)
     addr = lambda name: ctypes.cast(getattr(_ref_srrf, name), ctypes.c_void_p).value
     L.ora_use_radiality_kernels(addr("_c_calculate_radiality_per_subpixel"), addr("_c_gradient_radiality"))
+
+
+def use_reference_3d(on: bool) -> None:
+    """Route the eSRRF 3D kernels to the reference's compiled _c_calculate_rgc3D / _c_gradient_3d (True) or to the
+    restatement (False); the xy Catmull-Rom follows use_reference_kernels."""
+    global _ref
+    L = lib()
+    if not on:
+        L.ora_use_3d_kernels(None, None)
+        return
+    if _ref is None:
+        _ref = ctypes.CDLL(_REF_PATH)
+    addr = lambda name: ctypes.cast(getattr(_ref, name), ctypes.c_void_p).value
+    L.ora_use_3d_kernels(addr("_c_calculate_rgc3D"), addr("_c_gradient_3d"))
 
 
 def reference_srrf_available() -> bool:
@@ -218,6 +237,25 @@ def srrf_workflow(image, magnification=5, ringRadius=0.5, border=0, radialityPos
     interp = shift_magnify(image, M, M)
     rad = radiality(image, interp, M, ringRadius, border, radialityPositivityConstraint, doIntensityWeighting)
     return macro_pixel(rad, M) if macro_pixel_correction else rad
+
+
+_MODES3D = {"average": 0, "std": 1}
+
+
+def esrrf3d(image, magnification_xy=2, magnification_z=2, radius=1.5, PSF_ratio=2.8, voxel_ratio=4.0,
+            sensitivity=1, mode="average", doIntensityWeighting=True):
+    """eSRRF3D.run on the CPU ("Threaded"): (slices*Mz, rows*Mxy, cols*Mxy) float32 for an (F, S, R, C) or (S, R, C)
+    stack; radius_z = radius * PSF_ratio / voxel_ratio as in src/nanopyx/core/transform/_le_esrrf3d.pyx:40."""
+    image = _f32(image if np.ndim(image) == 4 else np.asarray(image)[None])
+    F, S, R, C = image.shape
+    radius_z = radius * PSF_ratio / voxel_ratio
+    out = np.zeros((S * int(magnification_z), R * int(magnification_xy), C * int(magnification_xy)), np.float32)
+    rc = lib().ora_esrrf3d(_p(image), F, S, R, C, int(magnification_xy), int(magnification_z), float(radius),
+                           float(radius_z), float(voxel_ratio), float(sensitivity), _MODES3D[mode],
+                           int(bool(doIntensityWeighting)), _p(out))
+    if rc != 0:
+        raise MemoryError("oracle eSRRF 3D could not allocate its intermediates")
+    return out
 
 
 def num_threads() -> int:
