@@ -163,6 +163,71 @@ def srrf_bench(reps, cpu_frames=2, workload="srrf_256"):
     return res
 
 
+def esrrf3d_samples_mean(Mxy, Mz, radius, radius_z):
+    """Mean count of window samples that pass the distance conditions, over the Mz x Mxy x Mxy output phases (interior
+    voxels)."""
+    sigma, sigma_z = np.float32(radius / 2.355), np.float32(radius_z / 2.355)
+    tSO, tSO_z = 2 * sigma + 1, 2 * sigma_z + 1
+    j = np.arange(-int(radius), int(np.float32(radius) + 1) + 1)
+    k = np.arange(-int(radius_z), int(np.float32(radius_z) + 1) + 1)
+    J, I, K = np.meshgrid(j, j, k, indexing="ij")
+    dxy = np.sqrt(I ** 2 + J ** 2)
+    d = np.sqrt(I ** 2 + J ** 2 + K ** 2)
+    return float(((d != 0) & (dxy <= tSO) & (K <= tSO_z)).sum())
+
+
+def esrrf3d_bench(reps, cpu_frames=1):
+    """eSRRF 3D (npx_esrrf3d: gradients, xMxy Catmull-Rom of intensity and gradients, per-voxel convergence folded
+    into the running mean) of 10 frames x 32 slices x 128 x 128 at Mxy 2, Mz 2, radius 1.5 (radius_z 1.05),
+    sensitivity 1, intensity weighting, 'average'. Times from HIP events on the launch stream; algorithmic flops per
+    output voxel-frame: 43 per window sample (distances 7, weight 6, G.d 5, cross product and norms 20, Dk and
+    accumulate 5) x the mean sample count, + 6 per voxel, + 160 / Mz for the xy magnification of 4 fields."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    F, S, R, C, Mxy, Mz, rad = 10, 32, 128, 128, 2, 2, 1.5
+    rad_z = rad * 2.8 / 4.0
+    x_host = synthetic.generate_3d(F, S, R, C, emitters=400)
+    x = torch.from_numpy(x_host).cuda()
+
+    def launch():
+        return _hip.esrrf3d(x, Mxy, Mz, rad, rad_z, 1.0, "average", True)
+
+    for _ in range(2):
+        launch()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        out = launch()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    U = F * S * Mz * R * Mxy * C * Mxy
+    s_bar = esrrf3d_samples_mean(Mxy, Mz, rad, rad_z)
+    flops = U * (43.0 * s_bar + 6.0 + 160.0 / Mz)
+    achieved = flops / (ms / 1e3) / 1e12
+    res = {"workload": f"esrrf3d: {F} frames x {S}x{R}x{C} float32, Mxy {Mxy} Mz {Mz}, radius {rad} "
+                       f"(radius_z {rad_z:.3f}), sensitivity 1, average -> {S * Mz}x{R * Mxy}x{C * Mxy}",
+           "ms_per_call": round(ms, 4), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mvoxel·frames/s",
+           "roofline": {"bound": "mfma", "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                        "flops_per_call": flops, "samples_mean": s_bar}}
+    if cpu_frames:
+        from oracle import ref32
+        ref32.build()
+        t0 = time.perf_counter()
+        ref = ref32.esrrf3d(x_host[:cpu_frames], Mxy, Mz, rad, 2.8, 4.0, 1, "average", True)
+        dt = time.perf_counter() - t0
+        Us = cpu_frames * S * Mz * R * Mxy * C * Mxy
+        res["cpu_baseline"] = {"value": round(Us / 1e6 / dt, 3), "unit": "Mvoxel·frames/s",
+                               "cores": ref32.num_threads(), "kind": "port",
+                               "sample": f"first {cpu_frames} frame(s) of the same stack"}
+        got = _hip.esrrf3d(x[:cpu_frames], Mxy, Mz, rad, rad_z, 1.0, "average", True).cpu().numpy()
+        res["parity_max_rel_err"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    return res
+
+
 def serving_bench(n, rate_hz, seed=5):
     """The served stream: n seeded requests (frames log-uniform 50-1000, H and W from {64..1024}, M from
     {2, 3, 4, 5, 8}, R uniform on [1, 2.5], S from {1, 2, 3}, one of AVG / VAR / TAC2, float32 or uint16) arriving as
@@ -292,6 +357,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-srrf", action="store_true")
     ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
+    ap.add_argument("--no-esrrf3d", action="store_true")
+    ap.add_argument("--esrrf3d-only", action="store_true", help="run only the eSRRF 3D leg and print its JSON")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=50.0)
     args = ap.parse_args()
@@ -302,6 +369,9 @@ def main():
 
     if args.srrf_only:
         print(json.dumps(srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)), flush=True)
+        return
+    if args.esrrf3d_only:
+        print(json.dumps(esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)), flush=True)
         return
 
     import torch
@@ -388,6 +458,7 @@ def main():
         "sweep": None,
         "serving": None,
         "srrf": None,
+        "esrrf3d": None,
     }
     if not args.no_sweep:
         log("parameter sweep ...")
@@ -397,6 +468,10 @@ def main():
         log("SRRF radiality ...")
         result["srrf"] = srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)
         log(f"srrf: {result['srrf']}")
+    if not args.no_esrrf3d:
+        log("eSRRF 3D ...")
+        result["esrrf3d"] = esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)
+        log(f"esrrf3d: {result['esrrf3d']}")
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)

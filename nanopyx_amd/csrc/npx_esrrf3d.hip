@@ -122,7 +122,11 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
     }
 }
 
-// (3) RGC of every output voxel over the chunk's frames, folded into the running statistics
+// (3) RGC of every output voxel over the chunk's frames, folded into the running statistics. The window geometry
+// (validity, offsets, distance, weight and the weight sum) does not depend on the frame, so it is formed once per
+// batch of kE3Batch frames and the batch's ring sums are kept in registers.
+constexpr int kE3Batch = 8;
+
 __global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
     const size_t nvox = static_cast<size_t>(a.SM) * a.RM * a.CM;
     const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, planeM = static_cast<size_t>(a.RM) * a.CM;
@@ -141,9 +145,15 @@ __global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
             const float ylim = static_cast<float>(a.RM / a.Mxy - 1), xlim = static_cast<float>(a.CM / a.Mxy - 1);
             const float zlim = static_cast<float>(a.SM / a.Mz - 1);
             const size_t self = static_cast<size_t>(sM / a.Mz) * planeM + static_cast<size_t>(rM) * a.CM + cM;
-            for (int fl = 0; fl < a.nf; ++fl) {
-                const float4* m = a.mag + fl * volM;
-                float rgc = 0.f, wsum = 0.f;
+            // batches of equal size (at most kE3Batch frames)
+            const int nbat = (a.nf + kE3Batch - 1) / kE3Batch, bsz = (a.nf + nbat - 1) / nbat;
+            for (int fb = 0; fb < a.nf; fb += bsz) {
+                const int nb = min(bsz, a.nf - fb);
+                const float4* m = a.mag + fb * volM;
+                float rg[kE3Batch];
+#pragma unroll
+                for (int q = 0; q < kE3Batch; ++q) rg[q] = 0.f;
+                float wsum = 0.f;
                 for (int j = a.j0; j <= a.j1; ++j) {
                     const float vy = yc + j;
                     if (!(0.f < vy && vy <= ylim)) continue;
@@ -154,8 +164,7 @@ __global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
                         if (!(0.f < vx && vx <= xlim)) continue;
                         const float dx = vx - xc;
                         const float dxy2 = dx * dx + dy * dy;
-                        const float dxy = sqrtf(dxy2);
-                        if (!(dxy <= a.tSO)) continue;
+                        if (!(sqrtf(dxy2) <= a.tSO)) continue;
                         const size_t oyx = oy + static_cast<int>(a.Mxy * vx);
                         const float wxy = __expf(-dxy2 / a.tSS);
                         for (int k = a.k0; k <= a.k1; ++k) {
@@ -165,34 +174,43 @@ __global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
                             const float d2 = dxy2 + dz * dz;
                             if (d2 == 0.f || !(dz <= a.tSO_z)) continue;
                             const float dist = sqrtf(d2);
-                            const int zs = static_cast<int>(vz * a.Mz) / a.Mz;  // source slice of the repeated volume
-                            const float4 G = m[static_cast<size_t>(zs) * planeM + oyx];
                             float w = dist * (wxy * __expf(-dz * dz / a.tSS_z));
                             w = (w * w) * (w * w);
                             wsum += w;
-                            const float gdr = fmaf(G.x, dx, fmaf(G.y, dy, G.z * dz));
-                            if (gdr < 0.f) {
+                            const float wd = w * __builtin_amdgcn_rcpf(dist);  // Dk w = w - |G x d| / |G| * w / |d|
+                            const float4* g = m + static_cast<size_t>(static_cast<int>(vz * a.Mz) / a.Mz) * planeM +
+                                              oyx;  // source slice of the repeated volume
+#pragma unroll
+                            for (int q = 0; q < kE3Batch; ++q) {
+                                if (q >= nb) break;
+                                const float4 G = g[q * volM];
+                                const float gdr = fmaf(G.x, dx, fmaf(G.y, dy, G.z * dz));
                                 const float c1 = fmaf(G.y, dz, -G.z * dy), c2 = fmaf(G.z, dx, -G.x * dz),
                                             c3 = fmaf(G.x, dy, -G.y * dx);
                                 const float cr2 = fmaf(c1, c1, fmaf(c2, c2, c3 * c3));
                                 const float g2 = fmaf(G.x, G.x, fmaf(G.y, G.y, G.z * G.z));
-                                const float dk = 1.f - sqrtf(cr2 / g2) / dist;
-                                rgc = fmaf(dk, w, rgc);
+                                const float term =
+                                    fmaf(-__builtin_amdgcn_sqrtf(cr2 * __builtin_amdgcn_rcpf(g2)), wd, w);
+                                rg[q] += gdr < 0.f ? term : 0.f;
                             }
                         }
                     }
                 }
-                rgc = rgc / wsum;
-                if (rgc >= 0.f && a.pow_on) rgc = powf(rgc, a.S_pow);
-                else if (rgc < 0.f) rgc = 0.f;
-                if (a.iw) rgc *= m[self].w;
-                const float fn = static_cast<float>(a.f0 + fl + 1);
-                if (a.mode == 0) {
-                    acc = acc + (rgc - acc) / fn;
-                } else {
-                    const float delta = rgc - mu;
-                    mu += delta / fn;
-                    acc += delta * (rgc - mu);
+#pragma unroll
+                for (int q = 0; q < kE3Batch; ++q) {
+                    if (q >= nb) break;
+                    float rgc = rg[q] / wsum;
+                    if (rgc >= 0.f && a.pow_on) rgc = powf(rgc, a.S_pow);
+                    else if (rgc < 0.f) rgc = 0.f;
+                    if (a.iw) rgc *= m[q * volM + self].w;
+                    const float fn = static_cast<float>(a.f0 + fb + q + 1);
+                    if (a.mode == 0) {
+                        acc = acc + (rgc - acc) / fn;
+                    } else {
+                        const float delta = rgc - mu;
+                        mu += delta / fn;
+                        acc += delta * (rgc - mu);
+                    }
                 }
             }
         }

@@ -104,6 +104,24 @@ def generate(scene: Scene, frames: int | None = None) -> np.ndarray:
     return out
 
 
+def generate_3d(frames, slices, rows, cols, emitters, sigma_xy=1.5, sigma_z=1.2, photons=800.0, p_on=0.2,
+                background=100.0, seed=7):
+    """(F, S, R, C) float32 volumes of blinking emitters: uniform 3D positions, a separable Gaussian PSF (sigma_xy,
+    sigma_z in voxels), per-frame Bernoulli(p_on) emission, background and Poisson noise."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z, y, x = (rng.uniform(0, n, emitters) for n in (slices, rows, cols))
+    gz = np.exp(-((np.arange(slices)[None, :] + 0.5 - z[:, None]) ** 2) / (2 * sigma_z ** 2))
+    gy = np.exp(-((np.arange(rows)[None, :] + 0.5 - y[:, None]) ** 2) / (2 * sigma_xy ** 2))
+    gx = np.exp(-((np.arange(cols)[None, :] + 0.5 - x[:, None]) ** 2) / (2 * sigma_xy ** 2))
+    norm = photons / (gz.sum(1) * gy.sum(1) * gx.sum(1))
+    out = np.empty((frames, slices, rows, cols), np.float32)
+    for f in range(frames):
+        on = np.nonzero(rng.random(emitters) < p_on)[0]
+        vol = np.einsum("e,ez,ey,ex->zyx", norm[on], gz[on], gy[on], gx[on], optimize=True) + background
+        out[f] = rng.poisson(vol).astype(np.float32)
+    return out
+
+
 def request_scene(rng: np.random.Generator, seed: int) -> Scene:
     """One served-stream job: frames log-uniform on 50-1000, H and W from {64,...,1024}, density 0.02/px^2."""
     sizes = (64, 128, 256, 384, 512, 1024)
