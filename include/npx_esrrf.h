@@ -195,6 +195,25 @@ int npx_esrrf3d(const void *d_img, int dtype, int F, int S, int R, int C, int Mx
                 void *stream);
 
 /*
+ * Roberts cross gradients of a device (N, H, W) stack (f32 or u16) into d_gc, d_gr (N, H, W) float32, the previous
+ * row and column clamped at 0. Replaces GradientRobertsCross._run_threaded,
+ * src/nanopyx/core/transform/_le_roberts_cross_gradients.pyx:67-83 (per pixel src/include/_c_gradients.c:53-72).
+ */
+int npx_roberts(const void *d_img, int dtype, int N, int H, int W, float *d_gc, float *d_gr, void *stream);
+
+/*
+ * Fourier interpolation of a device (N, h, w) float32 stack by an integer factor M into d_out (N, h*M, w*M): frames
+ * larger than tile_size (0 = 256) on either side are split into tiles with `overlap` pixels on each side, each
+ * interpolated with mirror padding and blended with a Hann window; smaller frames are interpolated whole, mirror
+ * padded when `mirror`. d_work is npx_fht_workspace() bytes of device scratch. Replaces fht_space_interpolation /
+ * fht_space_interpolation_tiled, src/nanopyx/core/transform/interpolation.py:58-292 (the use_fht branch of eSRRF,
+ * src/nanopyx/core/transform/_le_esrrf.pyx:183-188).
+ */
+size_t npx_fht_workspace(int N, int h, int w, int M, int mirror, int tile_size, int overlap);
+int npx_fht_interpolate(const float *d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
+                        float *d_out, void *d_work, size_t work_bytes, void *stream);
+
+/*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
  */

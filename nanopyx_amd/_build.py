@@ -47,6 +47,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     sess_o = os.path.join(BUILD_DIR, "npx_session.o")
     srrf_o = os.path.join(BUILD_DIR, "npx_srrf.o")
     e3d_o = os.path.join(BUILD_DIR, "npx_esrrf3d.o")
+    fht_o = os.path.join(BUILD_DIR, "npx_fht.o")
     hip = [HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc]
     steps = [
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", *inc, "-c",
@@ -57,14 +58,15 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
         [*hip, "-x", "hip", "-c", os.path.join(CSRC, "npx_session.cpp"), "-o", sess_o],
         [*hip, *extra, "-c", os.path.join(CSRC, "npx_srrf.hip"), "-o", srrf_o],
         [*hip, *extra, "-c", os.path.join(CSRC, "npx_esrrf3d.hip"), "-o", e3d_o],
+        [*hip, *extra, "-c", os.path.join(CSRC, "npx_fht.hip"), "-o", fht_o],
     ]
     # the translation units are independent: compile them concurrently
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(steps)) as ex:
         for f in [ex.submit(_run, cmd, verbose) for cmd in steps]:
             f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, batch_o, capi_o, sess_o, srrf_o, e3d_o, plan_o,
-          "-o",
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, batch_o, capi_o, sess_o, srrf_o, e3d_o, fht_o,
+          plan_o, "-L", os.path.join(ROCM, "lib"), "-lhipfft", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o",
           tmp], verbose)
     os.replace(tmp, LIB)
     return LIB

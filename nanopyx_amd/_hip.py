@@ -102,6 +102,11 @@ def lib():
         L.npx_esrrf3d_workspace.restype = _c_size_t
         L.npx_esrrf3d.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double,
                                   _c_double, _c_double, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]
+        L.npx_roberts.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]
+        L.npx_fht_workspace.argtypes = [_c_int] * 7
+        L.npx_fht_workspace.restype = _c_size_t
+        L.npx_fht_interpolate.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                          _c_void_p, _c_void_p, _c_size_t, _c_void_p]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -417,6 +422,34 @@ def radiality(stack, interp, magnification, ringRadius, border, positivity, iw, 
         out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
     _check(lib().npx_radiality(x.data_ptr(), _dtype_code(x), ip, N, H, W, M, float(ringRadius), int(border),
                                int(bool(positivity)), int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+    return out
+
+
+def roberts(stack, stream=None):
+    """Roberts cross gradients (gc, gr) of an (N, H, W) stack as device tensors."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    gc = torch.empty((N, H, W), dtype=torch.float32, device=x.device)
+    gr = torch.empty_like(gc)
+    _check(lib().npx_roberts(x.data_ptr(), _dtype_code(x), N, H, W, gc.data_ptr(), gr.data_ptr(),
+                             _stream(torch, stream)))
+    return gc, gr
+
+
+def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, stream=None):
+    """Fourier interpolation of an (N, h, w) stack by an integer factor: (N, h*M, w*M) float32 device tensor."""
+    torch = _torch()
+    x = _dev_stack(stack).to(torch.float32).contiguous()
+    N, h, w = x.shape
+    M = int(magnification)
+    need = int(lib().npx_fht_workspace(N, h, w, M, int(bool(mirror)), int(tile_size), int(overlap)))
+    if need == 0:
+        raise ValueError("bad Fourier interpolation shape")
+    work = torch.empty(need, dtype=torch.uint8, device=x.device)
+    out = torch.empty((N, h * M, w * M), dtype=torch.float32, device=x.device)
+    _check(lib().npx_fht_interpolate(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size), int(overlap),
+                                     out.data_ptr(), work.data_ptr(), need, _stream(torch, stream)))
     return out
 
 

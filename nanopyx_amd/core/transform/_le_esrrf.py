@@ -24,6 +24,20 @@ def _is_tensor(x) -> bool:
         return False
 
 
+def _esrrf_fht(image, M, radius, sensitivity, iw, pad_edges):
+    """The use_fht branch (_le_esrrf.pyx:183-188): Fourier xM of the image, Roberts cross of the image, Fourier x2M
+    of both gradients (npx_fht_interpolate, npx_roberts), then RGC on those arrays with pad_edges forwarded
+    (npx_rgc); returns the (N, H*M, W*M) device tensor."""
+    x = _hip._dev_stack(image)
+    mag = _hip.fht_interpolate(x, M)
+    gc, gr = _hip.roberts(x)
+    gci = _hip.fht_interpolate(gc, 2 * M)
+    del gc
+    gri = _hip.fht_interpolate(gr, 2 * M)
+    del gr
+    return _hip.rgc(gci, gri, mag, M, radius, sensitivity, iw, pad_edges)
+
+
 class eSRRF(LiquidEngine):
     """eSRRF using the Liquid Engine and running as a single task."""
 
@@ -50,7 +64,8 @@ class eSRRF(LiquidEngine):
         @gpu
         """
         if use_fht:
-            raise ValueError("use_fht=True (Fourier interpolation) is not part of the HIP_gfx950 run type")
+            out = _esrrf_fht(image, int(magnification), radius, sensitivity, doIntensityWeighting, pad_edges)
+            return out if _is_tensor(image) else out.cpu().numpy()
         # pad_edges is accepted and, as in the reference's non-FHT branch (_le_esrrf.pyx:194), not forwarded
         out = _hip.esrrf_frames(image, int(magnification), radius, sensitivity, doIntensityWeighting)
         return out if _is_tensor(image) else out.cpu().numpy()

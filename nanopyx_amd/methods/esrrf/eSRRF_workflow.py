@@ -4,7 +4,8 @@ Same signature and output: frames are split into timepoints of ``frames_per_time
 timepoint becomes one (H*M, W*M) map of the requested temporal correlation, optionally macro-pixel corrected,
 and the (T, H*M, W*M) float32 result is squeezed. On this path each timepoint is one fused launch
 (eSRRF + AVG/VAR/TAC2, ``npx_esrrf_temporal``) followed by the macro-pixel kernel; the stack is uploaded once and
-the per-frame maps never leave the GPU.
+the per-frame maps never leave the GPU. With ``use_fht`` each timepoint runs the Fourier-interpolation branch of the
+operator (per-frame maps on the GPU), then ``npx_temporal`` and the macro-pixel kernel.
 """
 from __future__ import annotations
 
@@ -20,8 +21,6 @@ def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float
           temporal_correlation: str = "AVG", use_fht: bool = False, doIntensityWeighting: bool = True,
           macro_pixel_correction: bool = True, pad_edges: bool = False, _force_run_type=None):
     """Perform eSRRF analysis on an (N, H, W) stack; returns the squeezed (T, H*M, W*M) float32 result."""
-    if use_fht:
-        raise ValueError("use_fht=True (Fourier interpolation) is not part of the HIP_gfx950 run type")
     if temporal_correlation not in ("AVG", "VAR", "TAC2"):
         raise ValueError("Type of correlation must be AVG, VAR or TAC2")
     import torch
@@ -40,9 +39,16 @@ def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float
     for i in range(T):
         part = stack[frames_per_timepoint * i:frames_per_timepoint * (i + 1)]
         run_type = _force_run_type or Agent.get_run_type(op, (part,), {})
-        m = op.run_temporal(part, magnification=M, radius=radius, sensitivity=sensitivity,
-                            temporal_correlation=temporal_correlation, doIntensityWeighting=doIntensityWeighting,
-                            run_type=run_type)
+        if use_fht:
+            # Fourier interpolation branch: per-frame maps (eSRRF_ST.run with use_fht, pad_edges forwarded), then
+            # the temporal correlation of the maps
+            maps = op.run(part, magnification=M, radius=radius, sensitivity=sensitivity, use_fht=True,
+                          doIntensityWeighting=doIntensityWeighting, pad_edges=pad_edges, run_type=run_type)
+            m = _hip.temporal(maps, (temporal_correlation,))[temporal_correlation]
+        else:
+            m = op.run_temporal(part, magnification=M, radius=radius, sensitivity=sensitivity,
+                                temporal_correlation=temporal_correlation, doIntensityWeighting=doIntensityWeighting,
+                                run_type=run_type)
         Agent._inform(op)
         if macro_pixel_correction:
             m = _hip.macro_pixel_(m.unsqueeze(0).contiguous(), M)[0]

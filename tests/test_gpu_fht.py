@@ -1,0 +1,80 @@
+"""The use_fht branch on the GPU: npx_fht_interpolate (hipFFT) against the oracle's restatement of the reference's
+Fourier interpolation (whole frames with and without mirror padding, Hann-blended tiles), npx_roberts against the
+oracle bit for bit, and eSRRF with use_fht (operator and workflow) against the oracle's chain."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    assert _torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return _torch
+
+
+def _stack(shape, seed=0):
+    rng = np.random.default_rng(seed)
+    return (rng.poisson(50, size=shape) + rng.random(shape) * 20).astype(np.float32)
+
+
+def _report(tag, x, ref):
+    mx, mean = rel_err(x, ref)
+    print(f"{tag}: max {mx:.3e} mean {mean:.3e}", flush=True)
+    return mx
+
+
+@pytest.mark.parametrize("shape,M,mirror,tile,overlap", [
+    ((3, 40, 60), 2, True, 0, 16),
+    ((2, 33, 47), 5, True, 0, 16),
+    ((2, 31, 29), 3, False, 0, 16),     # no padding: odd working sizes
+    ((2, 24, 24), 10, True, 0, 16),     # x2M of M = 5 gradients
+    ((2, 150, 170), 2, True, 64, 8),    # Hann-blended tiles
+    ((1, 300, 280), 2, True, 0, 16),    # default 256 tiles
+])
+def test_fht_matches_oracle(torch, shape, M, mirror, tile, overlap):
+    from nanopyx_amd import _hip
+    from oracle import fht
+    print(f"fht case {shape} M{M} mirror {mirror} tile {tile}", flush=True)
+    x = _stack(shape, seed=M)
+    ref = fht.fht_space_interpolation(x, M, mirror, tile, overlap)
+    got = _hip.fht_interpolate(x, M, mirror, tile, overlap).cpu().numpy()
+    assert got.shape == ref.shape
+    assert _report(f"fht {shape} M{M}", got, ref) <= 1e-5
+
+
+def test_roberts_bit_exact(torch, oracle_built):
+    from nanopyx_amd.core.transform import GradientRobertsCross
+    x = _stack((3, 21, 34), seed=4)
+    gc, gr = GradientRobertsCross(verbose=False).run(x, run_type="HIP_gfx950")
+    rc, rr = oracle_built.roberts(x)
+    assert np.array_equal(gc, rc) and np.array_equal(gr, rr)
+
+
+@pytest.mark.parametrize("M,R,S,pad_edges", [(2, 1.5, 1, False), (3, 1.0, 2, True), (5, 1.5, 1, False)])
+def test_esrrf_use_fht(torch, oracle_built, M, R, S, pad_edges):
+    from nanopyx_amd.core.transform import eSRRF_ST
+    from oracle import fht
+    x = _stack((3, 20, 26), seed=M)
+    ref = fht.esrrf_fht(x, M, R, S, True, pad_edges)
+    got = eSRRF_ST(verbose=False).run(x, magnification=M, radius=R, sensitivity=S, use_fht=True,
+                                      pad_edges=pad_edges, run_type="HIP_gfx950")
+    assert got.shape == ref.shape
+    assert _report(f"eSRRF use_fht M{M} R{R} S{S} pad {pad_edges}", got, ref) <= TOL
+
+
+def test_esrrf_workflow_use_fht(torch, oracle_built):
+    from nanopyx_amd import eSRRF
+    from oracle import fht
+    x = _stack((6, 18, 22), seed=8)
+    got = eSRRF(x, magnification=2, radius=1.5, frames_per_timepoint=3, temporal_correlation="VAR", use_fht=True)
+    assert got.shape == (2, 36, 44)
+    for t in range(2):
+        maps = fht.esrrf_fht(x[3 * t:3 * t + 3], 2, 1.5, 1, True, False)
+        ref = oracle_built.macro_pixel(oracle_built.temporal(maps, "VAR")[None], 2)[0]
+        assert _report(f"eSRRF(use_fht) timepoint {t}", got[t], ref) <= TOL
