@@ -228,6 +228,58 @@ def esrrf3d_bench(reps, cpu_frames=1):
     return res
 
 
+def fht_bench(reps, cpu_frames=1, frames=20):
+    """eSRRF with use_fht=True (Fourier interpolation branch) on the first `frames` frames of the srrf_256 scene
+    (256 x 256 float32), magnification 5, radius 1.5, sensitivity 1, intensity weighting, per-frame maps: Fourier x5
+    of the image, Roberts cross, Fourier x10 of both gradients (2560 x 2560 float64 spectra per frame), RGC on those
+    arrays. HIP events on the launch stream around the whole chain. HBM lower bound of the Fourier steps: every
+    enlarged complex grid written once by the spectrum placement and read and written once by the inverse FFT
+    (48 B per enlarged point)."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    from nanopyx_amd.core.transform._le_esrrf import _esrrf_fht
+    sc = synthetic.CONFIGS["srrf_256"]
+    x_host = synthetic.generate(sc, frames=frames)
+    x = torch.from_numpy(x_host).cuda()
+    N, H, W = x.shape
+    M, R = 5, 1.5
+
+    def launch():
+        return _esrrf_fht(x, M, R, 1.0, True, False)
+
+    launch()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        out = launch()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    U = N * H * M * W * M
+    big_points = N * ((256 * M) ** 2 + 2 * (256 * 2 * M) ** 2)
+    hbm_min = big_points * 48.0
+    res = {"workload": f"fht_256: {N} frames x {H}x{W} float32, eSRRF use_fht M{M} R{R} S1, per-frame maps "
+                       f"{H * M}x{W * M}",
+           "ms_per_call": round(ms, 3), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mpx·frames/s",
+           "roofline": {"bound": "hbm", "achieved": round(hbm_min / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(hbm_min / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": None, "bytes_min_per_call": hbm_min}}
+    if cpu_frames:
+        from oracle import fht, ref32
+        ref32.build()
+        t0 = time.perf_counter()
+        ref = fht.esrrf_fht(x_host[:cpu_frames], M, R, 1, True, False)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(cpu_frames * H * M * W * M / 1e6 / dt, 3), "unit": "Mpx·frames/s",
+                               "cores": ref32.num_threads(), "kind": "port",
+                               "sample": f"first {cpu_frames} frame(s): NumPy Fourier interpolation + oracle RGC"}
+        got = out[:cpu_frames].cpu().numpy()
+        res["parity_max_rel_err"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    return res
+
+
 def serving_bench(n, rate_hz, seed=5):
     """The served stream: n seeded requests (frames log-uniform 50-1000, H and W from {64..1024}, M from
     {2, 3, 4, 5, 8}, R uniform on [1, 2.5], S from {1, 2, 3}, one of AVG / VAR / TAC2, float32 or uint16) arriving as
@@ -358,6 +410,8 @@ def main():
     ap.add_argument("--no-srrf", action="store_true")
     ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
     ap.add_argument("--no-esrrf3d", action="store_true")
+    ap.add_argument("--no-fht", action="store_true")
+    ap.add_argument("--fht-only", action="store_true", help="run only the use_fht leg and print its JSON")
     ap.add_argument("--esrrf3d-only", action="store_true", help="run only the eSRRF 3D leg and print its JSON")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=50.0)
@@ -369,6 +423,9 @@ def main():
 
     if args.srrf_only:
         print(json.dumps(srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)), flush=True)
+        return
+    if args.fht_only:
+        print(json.dumps(fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)), flush=True)
         return
     if args.esrrf3d_only:
         print(json.dumps(esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)), flush=True)
@@ -459,6 +516,7 @@ def main():
         "serving": None,
         "srrf": None,
         "esrrf3d": None,
+        "use_fht": None,
     }
     if not args.no_sweep:
         log("parameter sweep ...")
@@ -472,6 +530,10 @@ def main():
         log("eSRRF 3D ...")
         result["esrrf3d"] = esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)
         log(f"esrrf3d: {result['esrrf3d']}")
+    if not args.no_fht:
+        log("eSRRF use_fht ...")
+        result["use_fht"] = fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)
+        log(f"use_fht: {result['use_fht']}")
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
