@@ -127,13 +127,29 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
 // batch of kE3Batch frames and the batch's ring sums are kept in registers.
 constexpr int kE3Batch = 8;
 
-__global__ __launch_bounds__(256) void e3_rgc3d(E3Args a) {
-    const size_t nvox = static_cast<size_t>(a.SM) * a.RM * a.CM;
+// Voxels of a block: 64 consecutive columns (the lanes) x 4 waves; the waves take the Zb output slices of one
+// source slice (Zb = Mz when Mz is 2 or 4, else 1) times 4 / Zb output rows of one phase (stride M), so their window
+// samples come from the same source slices and largely the same magnified rows and meet in L1.
+constexpr int kE3Waves = 4;
+__host__ __device__ inline int e3_zb(int Mz) { return (Mz == 2 || Mz == 4) ? Mz : 1; }
+
+__global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
     const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, planeM = static_cast<size_t>(a.RM) * a.CM;
-    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < nvox;
-         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const int cM = static_cast<int>(t % a.CM), rM = static_cast<int>((t / a.CM) % a.RM),
-                  sM = static_cast<int>(t / planeM);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int Zb = e3_zb(a.Mz), Rb = kE3Waves / Zb;
+    const int ncx = (a.CM + 63) / 64, nG = (a.R + Rb - 1) / Rb, nz = (a.SM + Zb - 1) / Zb;
+    const size_t ntiles = static_cast<size_t>(ncx) * a.Mxy * nG * nz;
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        size_t rest = tile;
+        const int cx = static_cast<int>(rest % ncx);
+        rest /= ncx;
+        const int p = static_cast<int>(rest % a.Mxy);
+        rest /= a.Mxy;
+        const int g = static_cast<int>(rest % nG);
+        const int sz = static_cast<int>(rest / nG);
+        const int cM = cx * 64 + lane, rM = p + a.Mxy * (Rb * g + wv / Zb), sM = sz * Zb + wv % Zb;
+        if (cM >= a.CM || rM >= a.RM || sM >= a.SM) continue;
+        const size_t t = static_cast<size_t>(sM) * planeM + static_cast<size_t>(rM) * a.CM + cM;
         const bool in = sM >= a.margin_z && sM < a.SM - a.margin_z && rM >= a.margin && rM < a.RM - a.margin &&
                         cM >= a.margin && cM < a.CM - a.margin;
         float acc = a.f0 == 0 ? 0.f : a.out[t];
@@ -296,6 +312,11 @@ int e3_chunk(int F, int S, int R, int C, int Mxy) {
     return static_cast<int>(std::max<size_t>(1, std::min<size_t>(static_cast<size_t>(F), g_e3_chunk_bytes / per)));
 }
 
+size_t e3_tiles(const E3Args& a) {
+    const int Zb = e3_zb(a.Mz), Rb = kE3Waves / Zb;
+    return static_cast<size_t>((a.CM + 63) / 64) * a.Mxy * ((a.R + Rb - 1) / Rb) * ((a.SM + Zb - 1) / Zb);
+}
+
 int e3_grid(size_t n) { return static_cast<int>(std::min<size_t>((n + 255) / 256, size_t(65536))); }
 
 }  // namespace
@@ -376,7 +397,6 @@ int npx_esrrf3d(const void* d_img, int dtype, int F, int S, int R, int C, int Mx
     a.grad = reinterpret_cast<float*>(w + ((mag + 255) & ~size_t(255)));
     a.mean = mode == 1 ? reinterpret_cast<float*>(w + ((mag + 255) & ~size_t(255)) + ((grad + 255) & ~size_t(255)))
                        : nullptr;
-    const size_t nvox = static_cast<size_t>(a.SM) * a.RM * a.CM;
     for (int f0 = 0; f0 < F; f0 += ch) {
         a.f0 = f0;
         a.nf = std::min(ch, F - f0);
@@ -384,7 +404,8 @@ int npx_esrrf3d(const void* d_img, int dtype, int F, int S, int R, int C, int Mx
         const size_t ng = static_cast<size_t>(a.nf) * S * R * C, nm = static_cast<size_t>(a.nf) * S * a.RM * a.CM;
         hipLaunchKernelGGL(e3_grad3d, dim3(e3_grid(ng)), dim3(256), 0, st, a);
         hipLaunchKernelGGL(e3_magnify_xy, dim3(e3_grid(nm)), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(e3_rgc3d, dim3(e3_grid(nvox)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(e3_rgc3d, dim3(static_cast<int>(std::min<size_t>(e3_tiles(a), size_t(1) << 20))),
+                           dim3(64 * kE3Waves), 0, st, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return host::hip_fail(e, "eSRRF 3D launch");
     }
