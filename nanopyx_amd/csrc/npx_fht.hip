@@ -5,13 +5,14 @@
 //
 // One interpolation = for each tile of the frame (the whole frame when neither side exceeds the tile size) and each
 // batch of frames: the tile mirror-padded to a power-of-two square in float64 (fht_pad), a batched double-precision
-// real-to-complex 2D FFT, the half spectrum's row quadrants placed at the top and bottom of an M-times larger grid
-// (fht_embed), a batched complex-to-complex inverse 2D FFT, per-frame min/max of the float32 result (fht_minmax), and
-// one pass that rescales to the tile's input range, crops the padding and writes (or, for tiles, adds the
-// Hann-weighted tile into the output and its window into the weight; fht_finish); tiled outputs are then divided by
-// the summed window (fht_normalise). The reference's inverse is a half-spectrum real transform (irfft2): its result
-// equals the real part of the full complex inverse of the half spectrum with the interior columns doubled and the
-// columns beyond the half zero, which is what fht_embed forms, so no Hermitian completion is needed.
+// real-to-complex 2D FFT, then the reference's irfft2 of the M-times larger half spectrum whose row quadrants sit at
+// its top and bottom. Only the first Ww/2+1 columns of that spectrum are non-zero, so the inverse runs as: those
+// columns, stored column-major with the quadrants placed (fht_embed_cols), through a batched complex inverse FFT along
+// the rows; a transpose into the enlarged half spectrum, zero beyond those columns, with the imaginary parts of the
+// DC and Nyquist columns set to 0 as irfft ignores them (fht_to_half); a batched complex-to-real inverse FFT along
+// the columns. Then per-frame min/max of the float32 result (fht_minmax), and one pass that rescales to the tile's
+// input range, crops the padding and writes (or, for tiles, adds the Hann-weighted tile into the output and its
+// window into the weight; fht_finish); tiled outputs are then divided by the summed window (fht_normalise).
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
@@ -58,7 +59,9 @@ struct FhtTile {
     float* weight;       // (N, h*M, w*M) summed window (tiled), else unused
     double* pad;         // (B, Hw, Ww)
     double2* spec;       // (B, Hw, Ww/2+1)
-    double2* big;        // (B, HwM, WwM)
+    double2* cols;       // (B, Ww/2+1, HwM): the non-zero columns of the enlarged spectrum, column-major
+    double2* half;       // (B, HwM, WwM/2+1): enlarged half spectrum after the row pass
+    double* real;        // (B, HwM, WwM): the inverse
     float* mm;           // (B, 4): input min, input max, output min, output max (as ordered ints for atomics)
     const float* win_r;  // [th*M] window rows (tiled)
     const float* win_c;  // [tw*M]
@@ -131,30 +134,36 @@ __global__ __launch_bounds__(256) void fht_pad(FhtTile a) {
     }
 }
 
-// half spectrum (Hw x Ww/2+1) -> full complex grid (HwM x WwM): row quadrants at the top and bottom, interior
-// columns doubled, columns beyond the half zero
-__global__ __launch_bounds__(256) void fht_embed(FhtTile a) {
-    const int HM = a.Hw * a.M, WM = a.Ww * a.M, hc = a.Ww / 2 + 1, h2 = (a.Hw + 1) / 2;
-    const size_t plane = static_cast<size_t>(HM) * WM, n = plane * a.B;
+// the Ww/2+1 non-zero columns of the enlarged spectrum, column-major: rows [0, h2) from the top quadrant, rows
+// [HwM - (Hw - h2), HwM) from the bottom one, zero between (interpolation.py:86-89)
+__global__ __launch_bounds__(256) void fht_embed_cols(FhtTile a) {
+    const int HM = a.Hw * a.M, hc = a.Ww / 2 + 1, h2 = (a.Hw + 1) / 2;
+    const size_t plane = static_cast<size_t>(hc) * HM, n = plane * a.B;
     for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
          t += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const size_t fb = t / plane, p = t % plane;
-        const int R = static_cast<int>(p / WM), c = static_cast<int>(p % WM);
+        const int c = static_cast<int>(p / HM), R = static_cast<int>(p % HM);
+        int r = -1;
+        if (R < h2) r = R;
+        else if (R >= HM - (a.Hw - h2)) r = a.Hw - (HM - R);
+        a.cols[t] = r >= 0 ? a.spec[fb * static_cast<size_t>(a.Hw) * hc + static_cast<size_t>(r) * hc + c]
+                           : make_double2(0.0, 0.0);
+    }
+}
+
+// transposed into the enlarged half spectrum (HwM x WwM/2+1): columns beyond Ww/2 zero, imaginary parts of the DC
+// and (even WwM) Nyquist columns zero
+__global__ __launch_bounds__(256) void fht_to_half(FhtTile a) {
+    const int HM = a.Hw * a.M, WM = a.Ww * a.M, hc = a.Ww / 2 + 1, wh = WM / 2 + 1;
+    const size_t plane = static_cast<size_t>(HM) * wh, n = plane * a.B;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t fb = t / plane, p = t % plane;
+        const int y = static_cast<int>(p / wh), c = static_cast<int>(p % wh);
         double2 v = make_double2(0.0, 0.0);
-        if (c < hc && c <= WM / 2) {
-            int r = -1;
-            if (R < h2) r = R;
-            else if (R >= HM - (a.Hw - h2)) r = a.Hw - (HM - R);
-            if (r >= 0) {
-                v = a.spec[fb * static_cast<size_t>(a.Hw) * hc + static_cast<size_t>(r) * hc + c];
-                const bool single = c == 0 || (WM % 2 == 0 && c == WM / 2);
-                if (!single) {
-                    v.x *= 2.0;
-                    v.y *= 2.0;
-                }
-            }
-        }
-        a.big[t] = v;
+        if (c < hc) v = a.cols[fb * static_cast<size_t>(hc) * HM + static_cast<size_t>(c) * HM + y];
+        if (c == 0 || (WM % 2 == 0 && c == WM / 2)) v.y = 0.0;
+        a.half[t] = v;
     }
 }
 
@@ -167,7 +176,7 @@ __global__ __launch_bounds__(256) void fht_minmax(FhtTile a) {
     float lo = INFINITY, hi = -INFINITY;
     for (size_t p = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; p < plane;
          p += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const float v = static_cast<float>(a.big[fb * plane + p].x * norm);
+        const float v = static_cast<float>(a.real[fb * plane + p] * norm);
         lo = fminf(lo, v);
         hi = fmaxf(hi, v);
     }
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
         const int y = static_cast<int>(p / ow), x = static_cast<int>(p % ow);
         const int* m = reinterpret_cast<const int*>(a.mm) + 4 * fb;
         const float lo = o2f(m[0]), hi = o2f(m[1]), omin = o2f(m[2]), omax = o2f(m[3]);
-        float v = static_cast<float>(a.big[fb * plane + static_cast<size_t>(y + a.yroi) * WM + (x + a.xroi)].x * norm);
+        float v = static_cast<float>(a.real[fb * plane + static_cast<size_t>(y + a.yroi) * WM + (x + a.xroi)] * norm);
         if (omax != omin) {
             const float scale = static_cast<float>((static_cast<double>(hi) - lo) / (static_cast<double>(omax) - omin));
             v = v - omin;
@@ -276,7 +285,7 @@ std::vector<int> starts_of(int dim, int tile, int overlap) {
 }
 
 struct Plans {
-    hipfftHandle fwd = 0, inv = 0;
+    hipfftHandle fwd = 0, cols = 0, rows = 0;
 };
 std::mutex f_mu;
 std::map<std::tuple<int, int, int, int, int>, Plans> f_plans;  // (device, Hw, Ww, M, batch)
@@ -290,12 +299,20 @@ int plans_for(int Hw, int Ww, int M, int B, Plans* out) {
     auto it = f_plans.find(key);
     if (it == f_plans.end()) {
         Plans p;
-        int nf[2] = {Hw, Ww}, ni[2] = {Hw * M, Ww * M};
+        int nf[2] = {Hw, Ww}, nr[1] = {Hw * M}, nc[1] = {Ww * M};
+        const int hc = Ww / 2 + 1;
         if (hipfftPlanMany(&p.fwd, 2, nf, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_D2Z, B) != HIPFFT_SUCCESS)
             return fail(NPX_ERR_HIP, "hipfftPlanMany (forward)");
-        if (hipfftPlanMany(&p.inv, 2, ni, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2Z, B) != HIPFFT_SUCCESS) {
+        // inverse along the rows of every non-zero column (contiguous columns of length HwM)
+        if (hipfftPlanMany(&p.cols, 1, nr, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2Z, B * hc) != HIPFFT_SUCCESS) {
             hipfftDestroy(p.fwd);
-            return fail(NPX_ERR_HIP, "hipfftPlanMany (inverse)");
+            return fail(NPX_ERR_HIP, "hipfftPlanMany (columns)");
+        }
+        // complex-to-real inverse along the columns of every enlarged row
+        if (hipfftPlanMany(&p.rows, 1, nc, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2D, B * Hw * M) != HIPFFT_SUCCESS) {
+            hipfftDestroy(p.fwd);
+            hipfftDestroy(p.cols);
+            return fail(NPX_ERR_HIP, "hipfftPlanMany (rows)");
         }
         it = f_plans.emplace(key, p).first;
     }
@@ -308,22 +325,25 @@ constexpr size_t kFhtBatchBytes = size_t(1) << 31;  // the enlarged complex grid
 // Batch size and per-part bytes of the workspace for tiles of at most th x tw input pixels.
 struct Layout {
     int B;
-    size_t pad, spec, big, mm, win, weight, total;
+    size_t pad, spec, cols, half, real, mm, win, weight, total;
 };
 Layout layout_of(int N, int h, int w, int M, int th, int tw, bool mirror, bool tiled) {
     const Work k = work_of(th, tw, mirror);
-    const size_t big1 = static_cast<size_t>(k.Hw) * M * k.Ww * M * sizeof(double2);
+    const size_t HM = static_cast<size_t>(k.Hw) * M, WM = static_cast<size_t>(k.Ww) * M;
+    const size_t per = HM * (WM / 2 + 1) * sizeof(double2) + HM * WM * sizeof(double);
     Layout L{};
     L.B = static_cast<int>(std::max<size_t>(
-        1, std::min<size_t>({static_cast<size_t>(N), kFhtBatchBytes / big1, size_t(4096)})));
+        1, std::min<size_t>({static_cast<size_t>(N), kFhtBatchBytes / per, size_t(4096)})));
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
     L.pad = al(static_cast<size_t>(L.B) * k.Hw * k.Ww * sizeof(double));
     L.spec = al(static_cast<size_t>(L.B) * k.Hw * (k.Ww / 2 + 1) * sizeof(double2));
-    L.big = al(static_cast<size_t>(L.B) * big1);
+    L.cols = al(static_cast<size_t>(L.B) * (k.Ww / 2 + 1) * HM * sizeof(double2));
+    L.half = al(static_cast<size_t>(L.B) * HM * (WM / 2 + 1) * sizeof(double2));
+    L.real = al(static_cast<size_t>(L.B) * HM * WM * sizeof(double));
     L.mm = al(static_cast<size_t>(L.B) * 4 * sizeof(float));
     L.win = tiled ? al(static_cast<size_t>(th + tw) * M * sizeof(float)) : 0;
     L.weight = tiled ? al(static_cast<size_t>(N) * h * M * w * M * sizeof(float)) : 0;
-    L.total = L.pad + L.spec + L.big + L.mm + L.win + L.weight;
+    L.total = L.pad + L.spec + L.cols + L.half + L.real + L.mm + L.win + L.weight;
     return L;
 }
 
@@ -380,10 +400,18 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
     a.M = M;
     a.pad = reinterpret_cast<double*>(wb);
     a.spec = reinterpret_cast<double2*>(wb + L.pad);
-    a.big = reinterpret_cast<double2*>(wb + L.pad + L.spec);
-    a.mm = reinterpret_cast<float*>(wb + L.pad + L.spec + L.big);
-    float* win = tiled ? reinterpret_cast<float*>(wb + L.pad + L.spec + L.big + L.mm) : nullptr;
-    a.weight = tiled ? reinterpret_cast<float*>(wb + L.pad + L.spec + L.big + L.mm + L.win) : nullptr;
+    size_t off = L.pad + L.spec;
+    a.cols = reinterpret_cast<double2*>(wb + off);
+    off += L.cols;
+    a.half = reinterpret_cast<double2*>(wb + off);
+    off += L.half;
+    a.real = reinterpret_cast<double*>(wb + off);
+    off += L.real;
+    a.mm = reinterpret_cast<float*>(wb + off);
+    off += L.mm;
+    float* win = tiled ? reinterpret_cast<float*>(wb + off) : nullptr;
+    off += L.win;
+    a.weight = tiled ? reinterpret_cast<float*>(wb + off) : nullptr;
     a.tiled = tiled ? 1 : 0;
     a.mirror = tiled ? 1 : (mirror ? 1 : 0);
     const size_t nout = static_cast<size_t>(N) * h * M * w * M;
@@ -446,11 +474,18 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
                     hipfftExecD2Z(pl.fwd, reinterpret_cast<hipfftDoubleReal*>(a.pad),
                                   reinterpret_cast<hipfftDoubleComplex*>(a.spec)) != HIPFFT_SUCCESS)
                     return host::fail(NPX_ERR_HIP, "hipfftExecD2Z");
-                hipLaunchKernelGGL(fht_embed, dim3(grid_of(bigp * a.B)), dim3(256), 0, st, a);
-                if (hipfftSetStream(pl.inv, st) != HIPFFT_SUCCESS ||
-                    hipfftExecZ2Z(pl.inv, reinterpret_cast<hipfftDoubleComplex*>(a.big),
-                                  reinterpret_cast<hipfftDoubleComplex*>(a.big), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-                    return host::fail(NPX_ERR_HIP, "hipfftExecZ2Z");
+                const size_t ncol = static_cast<size_t>(a.Ww / 2 + 1) * a.Hw * M * a.B;
+                const size_t nhalf = static_cast<size_t>(a.Hw) * M * (a.Ww * M / 2 + 1) * a.B;
+                hipLaunchKernelGGL(fht_embed_cols, dim3(grid_of(ncol)), dim3(256), 0, st, a);
+                if (hipfftSetStream(pl.cols, st) != HIPFFT_SUCCESS ||
+                    hipfftExecZ2Z(pl.cols, reinterpret_cast<hipfftDoubleComplex*>(a.cols),
+                                  reinterpret_cast<hipfftDoubleComplex*>(a.cols), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+                    return host::fail(NPX_ERR_HIP, "hipfftExecZ2Z (columns)");
+                hipLaunchKernelGGL(fht_to_half, dim3(grid_of(nhalf)), dim3(256), 0, st, a);
+                if (hipfftSetStream(pl.rows, st) != HIPFFT_SUCCESS ||
+                    hipfftExecZ2D(pl.rows, reinterpret_cast<hipfftDoubleComplex*>(a.half),
+                                  reinterpret_cast<hipfftDoubleReal*>(a.real)) != HIPFFT_SUCCESS)
+                    return host::fail(NPX_ERR_HIP, "hipfftExecZ2D (rows)");
                 hipLaunchKernelGGL(fht_minmax, dim3(std::min<size_t>((bigp + 255) / 256, 1024), a.B), dim3(256), 0,
                                    st, a);
                 const size_t nt = static_cast<size_t>(a.th) * M * a.tw * M * a.B;
