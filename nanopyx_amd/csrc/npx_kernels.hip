@@ -156,6 +156,7 @@ __global__ __launch_bounds__(256) void rgc_arrays(const float* __restrict__ gxi,
     const int gw = static_cast<int>(WM * G2);
     const float* gx = gxi + static_cast<size_t>(f) * 4 * P;
     const float* gy = gyi + static_cast<size_t>(f) * 4 * P;
+    const float inv_tSS = 1.0f / tSS;
     float rgc = 0.f, dws = 0.f;
     for (int j = start; j < end; ++j) {
         const float vy = static_cast<float>(static_cast<int>(G2 * yc) + j) / G2;
@@ -173,14 +174,14 @@ __global__ __launch_bounds__(256) void rgc_arrays(const float* __restrict__ gxi,
             if (pxf < 0) pxf = 0;
             const size_t o = static_cast<size_t>(static_cast<int>(pyf)) * gw + static_cast<int>(pxf);
             const float Gx = gx[o], Gy = gy[o];
-            const float e = d * expf((-d * d) / tSS);
+            const float e = d * __expf(-(d * d) * inv_tSS);
             const float e2 = e * e;
             const float dw = e2 * e2;
             dws += dw;
+            // G.d < 0 implies G != 0, so |G| > 0 below (the reference's NaN branch is never taken on this path)
             if (Gx * dx + Gy * dy < 0) {
-                float Dk = fabsf(Gy * dx - Gx * dy) / sqrtf(Gx * Gx + Gy * Gy);
-                if (isnan(Dk)) Dk = d;
-                Dk = 1 - Dk / d;
+                const float Dk = 1.0f - fabsf(Gy * dx - Gx * dy) * __builtin_amdgcn_rsqf(Gx * Gx + Gy * Gy) *
+                                            __builtin_amdgcn_rcpf(d);
                 rgc += Dk * dw;
             }
         }
