@@ -214,6 +214,17 @@ int npx_fht_interpolate(const float *d_in, int N, int h, int w, int M, int mirro
                         float *d_out, void *d_work, size_t work_bytes, void *stream);
 
 /*
+ * Catmull-Rom shift and magnify of a device (N, H, W) stack (f32 or u16) into d_out (N, int(H*mag_row),
+ * int(W*mag_col)) float32: output (i, j) samples the input at (i / mag_row - shift_row, j / mag_col - shift_col),
+ * 0 outside it. Replaces ShiftAndMagnify._run_threaded (CRShiftAndMagnify),
+ * src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:143-170 (per pixel
+ * src/include/_c_interpolation_catmull_rom.c:21-52). npx_shift_magnify_shape gives the output size.
+ */
+int npx_shift_magnify_shape(int H, int W, float mag_row, float mag_col, int *HM, int *WM);
+int npx_shift_magnify(const void *d_img, int dtype, int N, int H, int W, float shift_row, float shift_col,
+                      float mag_row, float mag_col, float *d_out, void *stream);
+
+/*
  * In-place macro-pixel correction of a device (S, HM, WM) stack; d_cmap is S*M*M floats of scratch.
  * Replaces macro_pixel_corrector, src/nanopyx/core/transform/mpcorrector.pyx:7-73.
  */

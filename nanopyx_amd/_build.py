@@ -48,6 +48,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     srrf_o = os.path.join(BUILD_DIR, "npx_srrf.o")
     e3d_o = os.path.join(BUILD_DIR, "npx_esrrf3d.o")
     fht_o = os.path.join(BUILD_DIR, "npx_fht.o")
+    interp_o = os.path.join(BUILD_DIR, "npx_interp.o")
     hip = [HIPCC, "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *inc]
     steps = [
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", *inc, "-c",
@@ -59,6 +60,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
         [*hip, *extra, "-c", os.path.join(CSRC, "npx_srrf.hip"), "-o", srrf_o],
         [*hip, *extra, "-c", os.path.join(CSRC, "npx_esrrf3d.hip"), "-o", e3d_o],
         [*hip, *extra, "-c", os.path.join(CSRC, "npx_fht.hip"), "-o", fht_o],
+        [*hip, *extra, "-c", os.path.join(CSRC, "npx_interp.hip"), "-o", interp_o],
     ]
     # the translation units are independent: compile them concurrently
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(steps)) as ex:
@@ -66,7 +68,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
             f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", kern_o, batch_o, capi_o, sess_o, srrf_o, e3d_o, fht_o,
-          plan_o, "-L", os.path.join(ROCM, "lib"), "-lhipfft", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o",
+          interp_o, plan_o, "-L", os.path.join(ROCM, "lib"), "-lhipfft", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o",
           tmp], verbose)
     os.replace(tmp, LIB)
     return LIB

@@ -102,6 +102,9 @@ def lib():
         L.npx_esrrf3d_workspace.restype = _c_size_t
         L.npx_esrrf3d.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double,
                                   _c_double, _c_double, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]
+        L.npx_shift_magnify_shape.argtypes = [_c_int, _c_int, ctypes.c_float, ctypes.c_float, _ip, _ip]
+        L.npx_shift_magnify.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, _c_void_p, _c_void_p]
         L.npx_roberts.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]
         L.npx_fht_workspace.argtypes = [_c_int] * 7
         L.npx_fht_workspace.restype = _c_size_t
@@ -422,6 +425,19 @@ def radiality(stack, interp, magnification, ringRadius, border, positivity, iw, 
         out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
     _check(lib().npx_radiality(x.data_ptr(), _dtype_code(x), ip, N, H, W, M, float(ringRadius), int(border),
                                int(bool(positivity)), int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+    return out
+
+
+def shift_magnify(stack, shift_row, shift_col, mag_row, mag_col, stream=None):
+    """Catmull-Rom shift and magnify of an (N, H, W) stack: (N, int(H*mag_row), int(W*mag_col)) device tensor."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    hm, wm = ctypes.c_int(), ctypes.c_int()
+    _check(lib().npx_shift_magnify_shape(H, W, float(mag_row), float(mag_col), ctypes.byref(hm), ctypes.byref(wm)))
+    out = torch.empty((N, hm.value, wm.value), dtype=torch.float32, device=x.device)
+    _check(lib().npx_shift_magnify(x.data_ptr(), _dtype_code(x), N, H, W, float(shift_row), float(shift_col),
+                                   float(mag_row), float(mag_col), out.data_ptr(), _stream(torch, stream)))
     return out
 
 

@@ -35,6 +35,22 @@ CrTap cr_taps(int i, float mag, int n) {
     return t;
 }
 
+CrTap cr_taps_shift(int i, float mag, float shift, int n) {
+    CrTap t{};
+    const float r = static_cast<float>(i) / mag - shift;
+    t.base = 0;
+    for (int u = 0; u < 4; ++u) t.w[u] = 0.0f;
+    if (r < 0 || r >= static_cast<float>(n)) return t;
+    const int r_int = static_cast<int>(std::floor(static_cast<float>(static_cast<double>(r) - 0.5)));
+    t.base = r_int - 1;
+    for (int u = 0; u < 4; ++u) {
+        const int rn = r_int - 1 + u;
+        if (rn < 0 || rn >= n) continue;
+        t.w[u] = static_cast<float>(cubic(static_cast<double>(r) - (rn + 0.5)));
+    }
+    return t;
+}
+
 bool build_plan(int H, int W, int M, double R, PlanHost& p, std::string* err) {
     if (H < 1 || W < 1 || M < 1) {
         if (err) *err = "image size and magnification must be >= 1";

@@ -81,6 +81,10 @@ struct SrrfPlan {
 bool build_srrf_plan(int H, int W, int M, double ringRadius, int border, int TJ, int TI, SrrfPlan& p,
                      std::string* err);
 
+// Taps of one output coordinate of ShiftAndMagnify: float(i) / mag - shift, in float32 as
+// src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:165-168 forms it (0 weights outside [0, n)).
+CrTap cr_taps_shift(int i, float mag, float shift, int n);
+
 // Reference-exact Catmull-Rom taps of one output coordinate i sampled at float(i) / float(mag)
 // (src/nanopyx/core/transform/_le_interpolation_catmull_rom.pyx:165-168, _c_interpolation_catmull_rom.c:21-52).
 CrTap cr_taps(int i, float mag, int n);

@@ -104,3 +104,27 @@ def test_radiality_errors(torch):
         _hip.radiality(x, None, 2, -0.5, 0, True, True)
     with pytest.raises(ValueError):
         _hip.radiality(x, None, 2, 0.5, -1, True, True)
+
+
+@pytest.mark.parametrize("shift,mag", [((0.0, 0.0), (5.0, 5.0)), ((0.3, -1.2), (2.0, 3.0)), ((-0.7, 0.25), (2.5, 1.5)),
+                                       ((0.0, 0.0), (1.0, 1.0))])
+def test_cr_shift_magnify_matches_oracle(torch, oracle_built, shift, mag):
+    from nanopyx_amd.core.transform import CRShiftAndMagnify
+    x = _stack((3, 17, 23), seed=12)
+    ref = oracle_built.shift_magnify(x, mag[0], mag[1], shift[0], shift[1])
+    got = CRShiftAndMagnify(verbose=False).run(x, shift[0], shift[1], mag[0], mag[1], run_type="HIP_gfx950")
+    assert got.shape == ref.shape and got.dtype == np.float32
+    assert _report(f"CRShiftAndMagnify shift {shift} mag {mag}", got, ref) <= 1e-5
+
+
+def test_reference_srrf_call_sequence(torch, oracle_built):
+    """The reference's own SRRF test sequence: CRShiftAndMagnify x5, then Radiality on the stack and its
+    magnification (uint16 camera counts)."""
+    from nanopyx_amd.core.transform import CRShiftAndMagnify, Radiality
+    rng = np.random.default_rng(13)
+    x = (rng.poisson(120, size=(10, 20, 20)) + 100).astype(np.uint16)
+    interp = CRShiftAndMagnify(verbose=False).run(x, 0, 0, 5, 5)
+    rad = Radiality(verbose=False).run(x, interp)
+    xf = x.astype(np.float32)
+    ref = oracle_built.radiality(xf, oracle_built.shift_magnify(xf, 5, 5), 5, 0.5)
+    assert _report("CRShiftAndMagnify + Radiality", rad, ref) <= TOL
