@@ -215,8 +215,8 @@ def test_errors(torch):
     import nanopyx_amd
     from nanopyx_amd.core.transform import eSRRF_ST
     x = np.ones((2, 8, 8), np.float32)
-    with pytest.raises(ValueError):
-        eSRRF_ST(verbose=False).run(x, use_fht=True)
+    # use_fht runs the Fourier branch (its parity is in test_gpu_fht.py); a constant image converges to nothing
+    assert not eSRRF_ST(verbose=False).run(x, magnification=2, use_fht=True).any()
     with pytest.raises(ValueError):
         nanopyx_amd.eSRRF(x, temporal_correlation="XYZ")
     with pytest.raises(ValueError):
