@@ -276,7 +276,8 @@ int npx_timing(int on);
  * eSRRF kernel and of the moment merge that follows it. No reference counterpart (see npx_timing). */
 int npx_timing_read(int *n_calls, double *kernel_ms, double *merge_ms);
 
-/* Frees the cached device tables of every device (optional; the process exit frees them too). */
+/* Frees the cached device tables and FFT plans of every device (optional; the process exit frees them too). No call
+ * may be in flight. */
 void npx_release(void);
 
 #ifdef __cplusplus

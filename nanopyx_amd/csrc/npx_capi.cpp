@@ -878,6 +878,10 @@ void npx_release(void) {
         kv.second->dev.clear();
     }
     if (cur >= 0) (void)hipSetDevice(cur);
+    npx::host::release_srrf();
+    npx::host::release_esrrf3d();
+    npx::host::release_fht();
+    npx::host::release_interp();
 }
 
 }  // extern "C"

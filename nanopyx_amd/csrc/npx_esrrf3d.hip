@@ -320,6 +320,18 @@ size_t e3_tiles(const E3Args& a) {
 int e3_grid(size_t n) { return static_cast<int>(std::min<size_t>((n + 255) / 256, size_t(65536))); }
 
 }  // namespace
+
+namespace host {
+void release_esrrf3d() {
+    std::lock_guard<std::mutex> lk(e3_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    for (auto& kv : e3_tables)
+        if (hipSetDevice(std::get<0>(kv.first)) == hipSuccess) (void)hipFree(kv.second.mem);
+    e3_tables.clear();
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+}  // namespace host
 }  // namespace npx
 
 extern "C" {

@@ -350,6 +350,22 @@ Layout layout_of(int N, int h, int w, int M, int th, int tw, bool mirror, bool t
 bool tiled_of(int h, int w, int tile) { return h > tile || w > tile; }
 
 }  // namespace
+
+namespace host {
+void release_fht() {
+    std::lock_guard<std::mutex> lk(f_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    for (auto& kv : f_plans) {
+        if (hipSetDevice(std::get<0>(kv.first)) != hipSuccess) continue;
+        hipfftDestroy(kv.second.fwd);
+        hipfftDestroy(kv.second.cols);
+        hipfftDestroy(kv.second.rows);
+    }
+    f_plans.clear();
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+}  // namespace host
 }  // namespace npx
 
 extern "C" {

@@ -45,5 +45,12 @@ void add_sensitivity(FusedArgs& a, double S);
 
 bool capturing(hipStream_t st);
 
+// Frees the device caches of the other translation units (called by npx_release): radiality plans, eSRRF 3D
+// tables, Fourier-interpolation FFT plans, shift-magnify tables.
+void release_srrf();
+void release_esrrf3d();
+void release_fht();
+void release_interp();
+
 }  // namespace host
 }  // namespace npx

@@ -123,6 +123,18 @@ int sm_tables_for(int H, int W, float sr, float sc, float mr, float mc, int HM, 
 }
 
 }  // namespace
+
+namespace host {
+void release_interp() {
+    std::lock_guard<std::mutex> lk(sm_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    for (auto& kv : sm_tables)
+        if (hipSetDevice(std::get<0>(kv.first)) == hipSuccess) (void)hipFree(kv.second.mem);
+    sm_tables.clear();
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+}  // namespace host
 }  // namespace npx
 
 extern "C" {

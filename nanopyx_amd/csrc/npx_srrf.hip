@@ -324,6 +324,20 @@ int srrf_plan(int H, int W, int M, double ringRadius, int border, hipStream_t st
 int g_srrf_target_blocks = 2048;
 
 }  // namespace
+
+namespace host {
+void release_srrf() {
+    std::lock_guard<std::mutex> lk(s_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    for (auto& kv : s_plans) {
+        for (auto& d : kv.second->dev)
+            if (hipSetDevice(d.first) == hipSuccess) (void)hipFree(d.second.mem);
+        kv.second->dev.clear();
+    }
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+}  // namespace host
 }  // namespace npx
 
 extern "C" {

@@ -128,3 +128,25 @@ def test_reference_srrf_call_sequence(torch, oracle_built):
     xf = x.astype(np.float32)
     ref = oracle_built.radiality(xf, oracle_built.shift_magnify(xf, 5, 5), 5, 0.5)
     assert _report("CRShiftAndMagnify + Radiality", rad, ref) <= TOL
+
+
+def test_release_and_rebuild_caches(torch):
+    """npx_release frees every cached plan (eSRRF, radiality, eSRRF 3D, Fourier interpolation, shift-magnify); the
+    next calls rebuild them and give the same results."""
+    from nanopyx_amd import _hip
+    x = _stack((2, 20, 24), seed=14)
+    x3 = _stack((2, 6, 16, 16), seed=15)
+
+    def run_all():
+        torch.cuda.synchronize()
+        return [_hip.radiality(x, None, 2, 0.5, 0, True, True).cpu(), _hip.esrrf3d(x3, 2, 2, 1.5, 1.05, 1, "average",
+                                                                                 True).cpu(),
+                _hip.fht_interpolate(x, 2).cpu(), _hip.shift_magnify(x, 0.2, 0.1, 2, 3).cpu(),
+                _hip.esrrf_frames(x, 2, 1.5, 1, True).cpu()]
+
+    a = run_all()
+    torch.cuda.synchronize()
+    _hip.lib().npx_release()
+    b = run_all()
+    for u, v in zip(a, b):
+        assert torch.equal(u, v)
