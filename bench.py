@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
     ap.add_argument("--no-esrrf3d", action="store_true")
     ap.add_argument("--no-fht", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one plain 1 GiB device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fht-only", action="store_true", help="run only the use_fht leg and print its JSON")
     ap.add_argument("--esrrf3d-only", action="store_true", help="run only the eSRRF 3D leg and print its JSON")
     ap.add_argument("--serving-requests", type=int, default=64)
@@ -421,6 +423,17 @@ def main():
               f"(--gpus {args.gpus} is not supported; multi-GPU sharding is a design note only)", flush=True)
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+        n = 1 << 30
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)                        # warm-up copy
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 2}), flush=True)
+        return
     if args.srrf_only:
         print(json.dumps(srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)), flush=True)
         return
