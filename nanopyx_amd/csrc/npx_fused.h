@@ -2,8 +2,8 @@
 //
 // A block owns a 64-column x Q-row region of the input image and up to 16 wave slots; a wave slot is one
 // (radius, output phase) pair, one lane per input column, Q rows per lane. Every lane of a wavefront shares
-// the phase and the radius, so the ring geometry (A, B, w per window offset) is wave-uniform and is read
-// through scalar loads. Per frame the block stages the input tile + halo, the Roberts gradients and the
+// the phase and the radius, so the ring geometry (offsets, distances, weights per window offset) is wave-uniform;
+// it is staged per wave in LDS and read into VGPRs. Per frame the block stages the input tile + halo, the Roberts gradients and the
 // normalised gradient lattice in LDS (the lattice is stored in two column-parity planes so that the 64 lanes
 // of a read touch 64 consecutive float2). Ring sums accumulate in registers.
 //
@@ -94,11 +94,15 @@ __device__ __forceinline__ void moments_push(float v, int n, float inv_np1, floa
         prev = v;
         return;
     }
+    // d = v - mu, e = d / (n + 1), mu' = mu + e, a = prev - mu, b = first - mu. Re-centring C on mu' adds
+    // e (a + b) + (n - 1) e^2 (the deviations of the first and of the last n - 1 values sum to -a and -b), the new
+    // pair adds (a - e)(d - e); with n e = d - e the sum is a d + e (b - e) = a d + e (first - mu').
     const float delta = v - mu;
-    const float mun = mu + delta * inv_np1;
-    const float dmu = mun - mu;
-    c = c + dmu * ((prev - mu) + (first - mu)) + static_cast<float>(n - 1) * dmu * dmu + (prev - mun) * (v - mun);
-    if (WITH_M2) m2 = m2 + delta * (v - mun);
+    const float e = delta * inv_np1;
+    const float mun = mu + e;
+    c = fmaf(prev - mu, delta, c);
+    c = fmaf(e, first - mun, c);
+    if (WITH_M2) m2 = fmaf(delta, v - mun, m2);
     mu = mun;
     prev = v;
 }
@@ -155,21 +159,21 @@ inline size_t lds_bytes_kq(int nwaves, int mode) {
 
 // Window rows [J0, J1) of the ring sums for column i: constants of those rows in VGPRs, lattice rows streamed.
 template <int K, int KL, int Q, int J0, int J1>
-__device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, float dxi, const float* rl, int i) {
+__device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, float dxn, const float* rl, int i) {
     using GL = Geometry<KL, Q>;
     constexpr int NW = 2 * K + 1;
     constexpr int LP = GL::LP;
     constexpr int NJ = J1 - J0;
     constexpr int R0 = J0, R1 = 2 * (Q - 1) + J1;  // lattice rows these window rows read: [R0, R1)
     constexpr int RDT = 2 * NW;
-    float dyv[NJ], dv[NJ], tv[NJ];
+    float dyn[NJ], tdv[NJ], tsv[NJ];
     const float2* dt = reinterpret_cast<const float2*>(rl + RDT) + i * NW + J0;
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj) {
-        dyv[jj] = rl[J0 + jj];
+        dyn[jj] = rl[J0 + jj];
         const float2 e = dt[jj];
-        dv[jj] = e.x;
-        tv[jj] = e.y;
+        tdv[jj] = e.x;
+        tsv[jj] = e.y;
     }
     // lattice rows are read two ahead of their use so the LDS latency overlaps the ring arithmetic
     float2 g1 = col[R0 * LP];
@@ -179,17 +183,17 @@ __device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, fl
         const float2 g = g1;
         g1 = g2;
         if (r + 2 < R1) g2 = col[(r + 2) * LP];
-        const float px_ = g.x * dxi;
-        const float py_ = g.y * dxi;
+        const float pxn = g.x * dxn;     // -s g.x dx
+        const float pys = -(g.y * dxn);  //  s g.y dx
 #pragma unroll
         for (int jj = 0; jj < NJ; ++jj) {
             const int rq = r - (J0 + jj);
             if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
             const int q = rq >> 1;
-            const float u = fmaf(g.y, dyv[jj], px_);
-            const float v = fmaf(-g.x, dyv[jj], py_);
-            const float m = __builtin_amdgcn_fmed3f(0.f, -u, dv[jj] - fabsf(v));
-            acc[q] = fmaf(tv[jj], m, acc[q]);
+            const float c = __builtin_amdgcn_fmed3f(fmaf(g.y, dyn[jj], pxn), 0.f, 1.f);  // clamp(-s u, 0, 1)
+            const float vs = fmaf(g.x, dyn[jj], pys);                                    // s v
+            const float x = fmaf(fabsf(vs), tsv[jj], tdv[jj]);                            // t (d - |v|)
+            acc[q] = fmaf(x, c, acc[q]);
         }
     }
 }
@@ -197,8 +201,11 @@ __device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, fl
 // Ring sums of one wave (radius half-width K) over a lattice built for half-width KL >= K. Sample (j, i) of
 // pixel q reads lattice row 2q + ey + j + (KL - K), column 2*lane + ex + i + (KL - K). With u = g.(dx, dy)
 // and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's contribution
-// [G.d < 0] * w * (1 - |G x d| / (|G| d)) is t * med3(0, -u, d - |v|). g.x*dx and g.y*dx are formed once per
-// lattice point of the column and shared by the samples that read it. The ring constants come from the wave's
+// [G.d < 0] * w * (1 - |G x d| / (|G| d)) is [u < 0] * t * (d - |v|). The indicator is taken as
+// clamp(-s u, 0, 1) with s = kRingScale, which the compiler folds into the clamp bit of the FMA forming -s u, so
+// a sample costs four FMAs: it equals the indicator for |u| >= 1/s, and below that the term it scales is itself
+// below t u^2 / d (u^2 + v^2 = d^2, so d - |v| <= u^2 / d), about 1e-10 t at s = 2^16. g.x*dx and g.y*dx are
+// formed once per lattice point of the column and shared by the samples that read it. The ring constants come from the wave's
 // LDS copy, so they sit in VGPRs (on gfx950 a VALU operation with an SGPR operand issues at about half the rate
 // of one whose operands are all VGPRs); the window rows are taken in NPX_RING_GROUPS groups so that only one
 // group's constants are live at a time.
@@ -219,21 +226,21 @@ __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, i
     for (int i = 0; i < NW; ++i) {
         const int c = ex + i + DK;
         const float2* col = s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
-        const float dxi = rl[RDX + i];
+        const float dxn = rl[RDX + i];  // -s dx
         // scheduling barriers keep each group's loads inside the group (no hoisting that would keep every
         // group's constants live at once)
-        ring_rows<K, KL, Q, 0, (JA < NW ? JA : NW)>(acc, col, dxi, rl, i);
+        ring_rows<K, KL, Q, 0, (JA < NW ? JA : NW)>(acc, col, dxn, rl, i);
         if constexpr (JA < NW) {
             __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, JA, (2 * JA < NW ? 2 * JA : NW)>(acc, col, dxi, rl, i);
+            ring_rows<K, KL, Q, JA, (2 * JA < NW ? 2 * JA : NW)>(acc, col, dxn, rl, i);
         }
         if constexpr (2 * JA < NW) {
             __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, 2 * JA, (3 * JA < NW ? 3 * JA : NW)>(acc, col, dxi, rl, i);
+            ring_rows<K, KL, Q, 2 * JA, (3 * JA < NW ? 3 * JA : NW)>(acc, col, dxn, rl, i);
         }
         if constexpr (3 * JA < NW) {
             __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, 3 * JA, NW>(acc, col, dxi, rl, i);
+            ring_rows<K, KL, Q, 3 * JA, NW>(acc, col, dxn, rl, i);
         }
     }
 }
@@ -325,8 +332,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     const int margin = active ? a.margin[rad] : 0;
 
     // ---- stage the frame-invariant tables. Per wave (block of RWL floats): the ring constants of its radius and
-    // phase (dy[j], dx[i], (d, t)[i][j]), then the normaliser part of its ring block (wsum, then the (NW+1)^2
-    // prefix sums of w).
+    // phase in the form the ring sums use (-s dy[j], -s dx[i], (t d, -t / s)[i][j] with s = kRingScale), then the
+    // normaliser part of its ring block (wsum, then the (NW+1)^2 prefix sums of w).
     for (int k = tid; k < a.nwaves * RWL; k += nthr) {
         const int w = k / RWL, e = k - w * RWL;
         const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
@@ -335,7 +342,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             const int nw = 2 * a.K[rw] + 1;
             const float* blk = a.ring[rw] + static_cast<size_t>(pw) * a.rs[rw];
             if (e < RCL) {
-                if (e < 2 * nw + 2 * nw * nw) v = blk[e];
+                if (e < 2 * nw) v = -kRingScale * blk[e];
+                else if (e < 2 * nw + 2 * nw * nw)  // host pairs (d, t)
+                    v = ((e - 2 * nw) & 1) ? blk[e] * (-1.f / kRingScale) : blk[e + 1] * blk[e];
             } else if (e - RCL < 1 + (nw + 1) * (nw + 1)) {
                 v = blk[2 * nw + 3 * nw * nw + (e - RCL)];
             }

@@ -14,6 +14,12 @@ import numpy as np
 from nanopyx_amd import _hip
 
 f32 = np.float32
+RING_SCALE = 65536.0  # kRingScale of the kernel: s in the ring sums' indicator clamp(-s u, 0, 1)
+
+
+def _fma(a, b, c):
+    """float32 a * b + c with one rounding of the exact product sum (float64 holds the product exactly)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
 
 
 def model_frames(stack, M, R, S, iw=True, Q=None):
@@ -150,21 +156,23 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                             # each read stays inside its own column-parity plane
                             assert ((rows - (c & 1) * LPL) < LPL).all()
                             g = s_lat[rows]                       # (NR, 64, 2)
-                            dxi = dxt[ph, i]
+                            # the kernel stages -s dx, -s dy, t d and -t / s (s = RING_SCALE)
+                            dxn = np.float32(-RING_SCALE * dxt[ph, i])
                             for r in range(NR):
-                                px_ = (g[r, :, 0] * dxi).astype(np.float32)
-                                py_ = (g[r, :, 1] * dxi).astype(np.float32)
+                                pxn = (g[r, :, 0] * dxn).astype(np.float32)
+                                pys = (-(g[r, :, 1] * dxn)).astype(np.float32)
                                 for jj in range(NW):
                                     rq = r - jj
                                     if rq < 0 or rq % 2 or rq // 2 >= Q:
                                         continue
                                     q = rq // 2
-                                    dyj = dyt[ph, jj]
-                                    u = (g[r, :, 1] * dyj + px_).astype(np.float32)
-                                    v = (-g[r, :, 0] * dyj + py_).astype(np.float32)
-                                    y = (dtab[ph, i, jj] - np.abs(v)).astype(np.float32)
-                                    m = np.median(np.stack([np.zeros(64, np.float32), -u, y]), axis=0)
-                                    acc[q] = (acc[q] + ttab[ph, i, jj] * m).astype(np.float32)
+                                    dyn = np.float32(-RING_SCALE * dyt[ph, jj])
+                                    td = np.float32(ttab[ph, i, jj] * dtab[ph, i, jj])
+                                    ts = np.float32(ttab[ph, i, jj] * np.float32(-1.0 / RING_SCALE))
+                                    c = np.clip(_fma(g[r, :, 1], dyn, pxn), 0, 1).astype(np.float32)
+                                    vs = _fma(g[r, :, 0], dyn, pys)
+                                    x = _fma(np.abs(vs), ts, td)
+                                    acc[q] = _fma(x, c, acc[q])
                         cMs = np.minimum(cM, WM - 1)
                         wcol = t["ccw"][cMs]
                         ccol = t["ccb"][cMs] - (x0 - HI)
