@@ -178,15 +178,17 @@ __global__ __launch_bounds__(kSrrfThreads) void srrf_radiality(SrrfArgs a) {
                         qx = fmaf(t3.x, wy.w, qx);
                         qy = fmaf(t3.y, wy.w, qy);
                         // Dk = (1 - |G x (xc - x0, yc - y0)| / (|G| R))^2, 0 where |G| = 0 (_c_sr_radiality.c:6-13);
-                        // dx, dy are (xc - x0) / R and (yc - y0) / R. The sample counts -Dk where G points along the
+                        // dx, dy are (xc - x0) / R and (yc - y0) / R, a unit vector, so 1 - |cr| / |G| lies in [0, 1]:
+                        // the clamp to [0, 1] (the FMA's clamp bit) changes nothing there, and where |G| = 0
+                        // (0 * inf = NaN) it gives the reference's 0. The sample counts -Dk where G points along the
                         // ring vector (x0 - xc, y0 - yc), i.e. where G . (dx, dy) < 0 (the reference tests G . ring
-                        // with the unrounded ring coordinates; the two differ only where G . ring ~ 0, where Dk ~ 0)
+                        // with the unrounded ring coordinates; the two differ only where G . ring ~ 0, where Dk ~ 0):
+                        // dk times dk with the dot product's sign copied onto it.
                         const float g2 = fmaf(qx, qx, qy * qy);
                         const float cr = fabsf(fmaf(qy, dx[k], -qx * dy));
-                        float dk = fmaf(-cr, __builtin_amdgcn_rsqf(g2), 1.f);
-                        dk = g2 > 0.f ? dk * dk : 0.f;
+                        const float dk = __builtin_amdgcn_fmed3f(fmaf(-cr, __builtin_amdgcn_rsqf(g2), 1.f), 0.f, 1.f);
                         const float dot = fmaf(qx, dx[k], qy * dy);
-                        div += dot < 0.f ? -dk : dk;
+                        div = fmaf(dk, copysignf(dk, dot), div);
                     }
                 }
                 if (grp + 1 < 12 / kSrrfGroup) {
