@@ -118,6 +118,12 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
             }
             r = make_float4(q0, q1, q2, q3);
         }
+        // the convergence uses only G's direction (the sign of G . d and |G x d| / |G|): store G / |G| (0 for G = 0)
+        const float n2 = fmaf(r.x, r.x, fmaf(r.y, r.y, r.z * r.z));
+        const float inv = n2 > 0.f ? rsqrtf(n2) : 0.f;
+        r.x *= inv;
+        r.y *= inv;
+        r.z *= inv;
         a.mag[t] = r;
     }
 }
@@ -126,12 +132,70 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
 // (validity, offsets, distance, weight and the weight sum) does not depend on the frame, so it is formed once per
 // batch of kE3Batch frames and the batch's ring sums are kept in registers.
 constexpr int kE3Batch = 8;
+constexpr float kE3Scale = 65536.f;  // s of the indicator clamp(-s g . d, 0, 1) (a power of two)
 
 // Voxels of a block: 64 consecutive columns (the lanes) x 4 waves; the waves take the Zb output slices of one
 // source slice (Zb = Mz when Mz is 2 or 4, else 1) times 4 / Zb output rows of one phase (stride M), so their window
 // samples come from the same source slices and largely the same magnified rows and meet in L1.
 constexpr int kE3Waves = 4;
 __host__ __device__ inline int e3_zb(int Mz) { return (Mz == 2 || Mz == 4) ? Mz : 1; }
+
+// Window sums of one output voxel for NB frames of a batch (rg[0..NB)) and the weight sum. The NB gathers of a
+// window sample are issued together, before any of them is used.
+template <int NB>
+__device__ __forceinline__ void e3_window(const E3Args& a, const float4* m, size_t volM, size_t planeM, float xc,
+                                          float yc, float zc, float ylim, float xlim, float zlim,
+                                          float (&rg)[kE3Batch], float& wsum) {
+    for (int j = a.j0; j <= a.j1; ++j) {
+        const float vy = yc + j;
+        if (!(0.f < vy && vy <= ylim)) continue;
+        const float dy = vy - yc;
+        const size_t oy = static_cast<size_t>(static_cast<int>(a.Mxy * vy)) * a.CM;
+        for (int i = a.j0; i <= a.j1; ++i) {
+            const float vx = xc + i;
+            if (!(0.f < vx && vx <= xlim)) continue;
+            const float dx = vx - xc;
+            const float dxy2 = dx * dx + dy * dy;
+            if (!(sqrtf(dxy2) <= a.tSO)) continue;
+            const size_t oyx = oy + static_cast<int>(a.Mxy * vx);
+            const float wxy = __expf(-dxy2 / a.tSS);
+            for (int k = a.k0; k <= a.k1; ++k) {
+                const float vz = zc + k;
+                if (!(0.f < vz && vz <= zlim)) continue;
+                const float dz = vz - zc;
+                const float d2 = dxy2 + dz * dz;
+                if (d2 == 0.f || !(dz <= a.tSO_z)) continue;
+                const float4* g = m + static_cast<size_t>(static_cast<int>(vz * a.Mz) / a.Mz) * planeM +
+                                  oyx;  // source slice of the repeated volume
+                float3 G[NB];  // (g, I) records with g = G / |G|; only g is read here
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    const float4 r = g[q * volM];
+                    G[q] = make_float3(r.x, r.y, r.z);
+                }
+                const float dist = sqrtf(d2);
+                float w = dist * (wxy * __expf(-dz * dz / a.tSS_z));
+                w = (w * w) * (w * w);
+                wsum += w;
+                const float wd = w * __builtin_amdgcn_rcpf(dist);  // Dk w = w - |g x d| * w / |d|
+                // [g . d < 0] as clamp(-s g . d, 0, 1) on the clamp bit of the last FMA: exact for |g . d| >= 1/s;
+                // below that the term it scales is under w (g . d)^2 / |d|^2
+                const float sdx = -kE3Scale * dx, sdy = -kE3Scale * dy, sdz = -kE3Scale * dz;
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    const float3 gq = G[q];
+                    const float neg =
+                        __builtin_amdgcn_fmed3f(fmaf(gq.x, sdx, fmaf(gq.y, sdy, gq.z * sdz)), 0.f, 1.f);
+                    const float c1 = fmaf(gq.y, dz, -gq.z * dy), c2 = fmaf(gq.z, dx, -gq.x * dz),
+                                c3 = fmaf(gq.x, dy, -gq.y * dx);
+                    const float cr2 = fmaf(c1, c1, fmaf(c2, c2, c3 * c3));
+                    const float term = fmaf(-__builtin_amdgcn_sqrtf(cr2), wd, w);
+                    rg[q] = fmaf(neg, term, rg[q]);
+                }
+            }
+        }
+    }
+}
 
 __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
     const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, planeM = static_cast<size_t>(a.RM) * a.CM;
@@ -170,47 +234,13 @@ __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
 #pragma unroll
                 for (int q = 0; q < kE3Batch; ++q) rg[q] = 0.f;
                 float wsum = 0.f;
-                for (int j = a.j0; j <= a.j1; ++j) {
-                    const float vy = yc + j;
-                    if (!(0.f < vy && vy <= ylim)) continue;
-                    const float dy = vy - yc;
-                    const size_t oy = static_cast<size_t>(static_cast<int>(a.Mxy * vy)) * a.CM;
-                    for (int i = a.j0; i <= a.j1; ++i) {
-                        const float vx = xc + i;
-                        if (!(0.f < vx && vx <= xlim)) continue;
-                        const float dx = vx - xc;
-                        const float dxy2 = dx * dx + dy * dy;
-                        if (!(sqrtf(dxy2) <= a.tSO)) continue;
-                        const size_t oyx = oy + static_cast<int>(a.Mxy * vx);
-                        const float wxy = __expf(-dxy2 / a.tSS);
-                        for (int k = a.k0; k <= a.k1; ++k) {
-                            const float vz = zc + k;
-                            if (!(0.f < vz && vz <= zlim)) continue;
-                            const float dz = vz - zc;
-                            const float d2 = dxy2 + dz * dz;
-                            if (d2 == 0.f || !(dz <= a.tSO_z)) continue;
-                            const float dist = sqrtf(d2);
-                            float w = dist * (wxy * __expf(-dz * dz / a.tSS_z));
-                            w = (w * w) * (w * w);
-                            wsum += w;
-                            const float wd = w * __builtin_amdgcn_rcpf(dist);  // Dk w = w - |G x d| / |G| * w / |d|
-                            const float4* g = m + static_cast<size_t>(static_cast<int>(vz * a.Mz) / a.Mz) * planeM +
-                                              oyx;  // source slice of the repeated volume
-#pragma unroll
-                            for (int q = 0; q < kE3Batch; ++q) {
-                                if (q >= nb) break;
-                                const float4 G = g[q * volM];
-                                const float gdr = fmaf(G.x, dx, fmaf(G.y, dy, G.z * dz));
-                                const float c1 = fmaf(G.y, dz, -G.z * dy), c2 = fmaf(G.z, dx, -G.x * dz),
-                                            c3 = fmaf(G.x, dy, -G.y * dx);
-                                const float cr2 = fmaf(c1, c1, fmaf(c2, c2, c3 * c3));
-                                const float g2 = fmaf(G.x, G.x, fmaf(G.y, G.y, G.z * G.z));
-                                const float term =
-                                    fmaf(-__builtin_amdgcn_sqrtf(cr2 * __builtin_amdgcn_rcpf(g2)), wd, w);
-                                rg[q] += gdr < 0.f ? term : 0.f;
-                            }
-                        }
-                    }
+                switch (nb) {
+#define NPX_E3_CASE(n) \
+    case n: e3_window<n>(a, m, volM, planeM, xc, yc, zc, ylim, xlim, zlim, rg, wsum); break;
+                    NPX_E3_CASE(1) NPX_E3_CASE(2) NPX_E3_CASE(3) NPX_E3_CASE(4)
+                    NPX_E3_CASE(5) NPX_E3_CASE(6) NPX_E3_CASE(7) NPX_E3_CASE(8)
+#undef NPX_E3_CASE
+                    default: break;
                 }
 #pragma unroll
                 for (int q = 0; q < kE3Batch; ++q) {
