@@ -79,3 +79,20 @@ def test_esrrf3d_workflow(torch, oracle_built):
         assert _report(f"eSRRF3D timepoint {t}", got[t], ref) <= TOL
     raw = eSRRF3D(x, macro_pixel_correction=False, mode="std")
     assert _report("eSRRF3D std", raw[0], oracle_built.esrrf3d(x, 2, 2, mode="std")) <= TOL
+
+
+def test_esrrf3d_degenerate_gradients(torch, oracle_built):
+    """A flat volume (G = 0: the unit gradient is stored as 0) and a ramp along z against the oracle."""
+    from nanopyx_amd import _hip
+    z = np.arange(8, dtype=np.float32)[:, None, None]
+    vols = {"flat": np.full((8, 12, 14), 40.0, np.float32),
+            "ramp_z": np.broadcast_to(3.0 * z + 10.0, (8, 12, 14)).astype(np.float32)}
+    for name, v in vols.items():
+        print(f"esrrf3d degenerate {name}", flush=True)
+        x = np.stack([v, v * 1.2 + 1.0]).astype(np.float32)
+        ref = oracle_built.esrrf3d(x, 2, 2, 1.5, 2.8, 4.0, 1, "average", True)
+        got = _hip.esrrf3d(x, 2, 2, 1.5, 1.5 * 2.8 / 4.0, 1, "average", True).cpu().numpy()
+        assert np.isfinite(got).all()
+        if name == "flat":
+            assert not np.abs(ref).any() and not np.abs(got).any()
+        assert _report(f"esrrf3d degenerate {name}", got, ref) <= TOL

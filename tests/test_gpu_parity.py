@@ -246,3 +246,31 @@ def test_autotune_shape_and_store(torch, tmp_path, monkeypatch):
     assert autotune.load() == 1
     assert _hip.tune_get(40, 72, M, R)[:2] == (entry["max_waves"], entry["target_blocks"])
     _hip.tune_set(40, 72, M, R, 0, 0)
+
+
+def _degenerate_images(H=24, W=70):
+    """Images whose gradients vanish or are constant: flat (G = 0 everywhere), ramps along x, y and the diagonal
+    (constant G, so ring samples perpendicular to G have G . d = 0 exactly), and a step edge."""
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    step = np.where(x < W // 2, 10.0, 200.0).astype(np.float32)
+    return {"flat": np.full((H, W), 37.0, np.float32), "ramp_x": 3.0 * x + 5.0, "ramp_y": 2.0 * y + 1.0,
+            "ramp_diag": x + y + 2.0, "step": step}
+
+
+@pytest.mark.parametrize("M,R,S", [(4, 2.0, 2.0), (5, 1.5, 1.0)])
+def test_degenerate_gradients(torch, oracle_built, M, R, S):
+    """Flat, ramp and step images: the zero-gradient guard and the [G . d < 0] indicator at G . d = 0 give the
+    reference's values (exact zeros on the flat image), per frame and through the temporal kernel."""
+    from nanopyx_amd import _hip
+    from nanopyx_amd.core.transform import eSRRF_ST
+    for name, img in _degenerate_images().items():
+        print(f"degenerate {name} M{M} R{R} S{S}", flush=True)
+        x = np.stack([img, img * 1.5 + 3.0]).astype(np.float32)
+        ref = oracle_built.esrrf(x, M, R, S)
+        out = eSRRF_ST(verbose=False).run(x, magnification=M, radius=R, sensitivity=S)
+        assert np.isfinite(out).all()
+        if name == "flat":
+            assert not np.abs(ref).any() and not np.abs(out).any()
+        assert _report(f"degenerate {name} frames", out, ref) <= TOL
+        avg = _hip.esrrf_temporal(torch.from_numpy(x).cuda(), M, R, S, True, ("AVG",))["AVG"].cpu().numpy()
+        assert _report(f"degenerate {name} AVG", avg, ref.mean(axis=0)) <= TOL

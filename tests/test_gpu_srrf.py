@@ -150,3 +150,27 @@ def test_release_and_rebuild_caches(torch):
     b = run_all()
     for u, v in zip(a, b):
         assert torch.equal(u, v)
+
+
+def test_radiality_degenerate_gradients(torch, oracle_built):
+    """Flat image (G = 0: Dk is 0 there, not 0 * inf), ramps and a step edge against the oracle. On a linear ramp the
+    opposite ring samples cancel, so both sides are rounding noise around 0 (about 1e-7 of the intensity): those are
+    held to an absolute bound of 1e-5 of the image's largest value instead of the relative metric."""
+    from nanopyx_amd import _hip
+    y, x = np.mgrid[0:20, 0:40].astype(np.float32)
+    images = {"flat": np.full((20, 40), 50.0, np.float32), "ramp_x": 2.0 * x + 1.0, "ramp_diag": x - y + 60.0,
+              "step": np.where(y < 10, 5.0, 90.0).astype(np.float32)}
+    for name, img in images.items():
+        print(f"radiality degenerate {name}", flush=True)
+        st = np.stack([img, img + 7.0]).astype(np.float32)
+        ref = oracle_built.radiality(st, oracle_built.shift_magnify(st, 5, 5), 5, 0.5)
+        got = _hip.radiality(st, None, 5, 0.5, 0, True, True).cpu().numpy()
+        assert np.isfinite(got).all()
+        if name == "flat":
+            assert not np.abs(ref).any() and not np.abs(got).any()
+        if name.startswith("ramp"):
+            diff = float(np.abs(got - ref).max())
+            print(f"radiality degenerate {name}: max |diff| {diff:.3e}, max |ref| {np.abs(ref).max():.3e}", flush=True)
+            assert diff <= 1e-5 * float(img.max())
+        else:
+            assert _report(f"radiality degenerate {name}", got, ref) <= TOL
