@@ -649,9 +649,39 @@ int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int
     const float tSS = static_cast<float>(2.0f * sigma) * sigma;
     const float tSO = static_cast<float>(2.0f * sigma) + 1.0f;
     const int margin = pad_edges ? 0 : static_cast<int>(static_cast<double>(fwhm) * M);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // An image of H x W input pixels magnified M times: the fused kernel runs on the gradient lattice (the caller's
+    // arrays at every M-th position) with the plan's ring tables, as the eSRRF path does on its own lattice.
+    // Other shapes, and radii or sizes the plan does not cover, take the per-pixel kernel below.
+    if (HM % M == 0 && WM % M == 0 && M <= 64) {
+        const int H = HM / M, W = WM / M;
+        const npx::DevTables* t = nullptr;
+        Plan* p = nullptr;
+        const std::string prev = g_err;
+        if (plan_and_tables(H, W, M, R, &p, &t, st) == NPX_OK) {
+            const LaunchCfg c = launch_cfg(H, W, M, R);
+            Shape s = fused_shape(H, W, M, c);
+            s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
+            npx::FusedArgs a = base_args(d_img, NPX_DTYPE_F32, N, H, W, M, p->host, t, s, S, iw);
+            a.agx = d_gx;
+            a.agy = d_gy;
+            a.aimg = d_img;
+            a.margin[0] = margin;
+            const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
+            long long want = std::max<long long>(1, (c.target_blocks + spatial - 1) / spatial);
+            want = std::min<long long>({want, N, 65535 / s.ngroups});
+            a.fpc = static_cast<int>((N + want - 1) / want);
+            a.nchunk = (N + a.fpc - 1) / a.fpc;
+            a.out = d_out;
+            hipError_t e = npx::launch_fused(a, dim3(s.nregx, s.nregy, s.ngroups * a.nchunk), dim3(64 * s.nwaves),
+                                             npx::kModeArrays, st);
+            if (e != hipSuccess) return hip_fail(e, "esrrf_fused(arrays) launch");
+            return NPX_OK;
+        }
+        g_err = prev;
+    }
     hipError_t e = npx::launch_rgc_arrays(d_gx, d_gy, d_img, N, HM, WM, M, fwhm, tSO, tSS, pow_mode_of(S),
-                                          static_cast<float>(S), iw ? 1 : 0, margin, d_out,
-                                          static_cast<hipStream_t>(stream));
+                                          static_cast<float>(S), iw ? 1 : 0, margin, d_out, st);
     if (e != hipSuccess) return hip_fail(e, "rgc_arrays launch");
     return NPX_OK;
 }

@@ -285,6 +285,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     constexpr bool TEMPORAL = MODE == kModeTemporal || MODE == kModeTemporalVar;
     constexpr bool SWEEP = MODE == kModeSweep;
     constexpr bool WITH_M2 = MODE == kModeTemporalVar;
+    constexpr bool ARRAYS = MODE == kModeArrays;                // lattice and intensity from the caller's arrays
+    constexpr bool FRAMES = MODE == kModeFrames || ARRAYS;      // per-frame maps out
     constexpr int NSM = SWEEP ? kMaxSens : 1;
 
     float* const s_in = reinterpret_cast<float*>(smem + L::IN);
@@ -446,7 +448,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             s_in[idx] = load_px(a, f, y0 - HI + r, x0 - HI + c);
         }
     };
-    if (f_begin < f_end) {
+    if (!ARRAYS && f_begin < f_end) {
         fetch(f_begin);
         store_tile(f_begin);
         if (f_begin + 1 < f_end) fetch(f_begin + 1);
@@ -473,78 +475,106 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 
     for (int f = f_begin; f < f_end; ++f) {
         // intensity rows with their own storage are computed here, beside Roberts (both only read the tile)
-        if (T_OWN && a.iw) intensity_rows();
-        // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
-        for (int idx = tid; idx < TGR * TGC; idx += nthr) {
-            const int r = idx / TGC, c = idx - r * TGC;
-            const int gy = y0 - HG + r, gx = x0 - HG + c;
-            float gcv = 0.f, grv = 0.f;
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-                const int r1 = gy - (y0 - HI), c1 = gx - (x0 - HI);
-                const int r0 = (gy > 0 ? gy - 1 : 0) - (y0 - HI);
-                const int c0 = (gx > 0 ? gx - 1 : 0) - (x0 - HI);
-                const float a_r0c1 = s_in[r0 * TIC + c1];
-                const float a_r1c0 = s_in[r1 * TIC + c0];
-                const float a_r0c0 = s_in[r0 * TIC + c0];
-                const float a_r1c1 = s_in[r1 * TIC + c1];
-                gcv = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0;
-                grv = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0;
+        if (!ARRAYS && T_OWN && a.iw) intensity_rows();
+        if constexpr (ARRAYS) {
+            // lattice point (a, b) is the caller's gradient at row min(a M, 2 HM - 1), column min(b M, 2 WM - 1) of
+            // its x2M arrays, normalised (_c_sr_radial_gradient_convergence.c:45-52 reads those positions); points
+            // outside a in [1, 2 HM - 2], b in [1, 2 WM - 2] are outside every window and hold 0
+            const size_t P4 = static_cast<size_t>(4) * a.HM * a.WM;
+            for (int idx = tid; idx < 2 * LPL; idx += nthr) {
+                const int plane = idx / LPL;
+                const int rem = idx - plane * LPL;
+                const int la = rem / LP;
+                const int pos = rem - la * LP;
+                const int aa = a_lo + la, bb = b_lo + 2 * pos + plane;
+                float2 g = make_float2(0.f, 0.f);
+                if (aa >= 1 && aa <= 2 * a.HM - 2 && bb >= 1 && bb <= 2 * a.WM - 2) {
+                    const size_t o = static_cast<size_t>(f) * P4 +
+                                     static_cast<size_t>(min(aa * M, 2 * a.HM - 1)) * (2 * a.WM) +
+                                     min(bb * M, 2 * a.WM - 1);
+                    const float gxv = a.agx[o], gyv = a.agy[o];
+                    const float n2 = gxv * gxv + gyv * gyv;
+                    if (n2 > 0.f) {
+                        const float inv = rsqrtf(n2);
+                        g = make_float2(gxv * inv, gyv * inv);
+                    }
+                }
+                s_lat[idx] = g;
             }
-            s_g[idx] = make_float2(gcv, grv);
-        }
-        __syncthreads();
-        // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
-        for (int idx = tid; idx < LR * TGC; idx += nthr) {
-            const int la = idx / TGC, c = idx - la * TGC;
-            const float4 wr = s_lrw[la];
-            const int rb = s_lrb[la];
-            const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
-            float sx = 0.f, sy = 0.f;
+            __syncthreads();
+        } else {
+            // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
+            for (int idx = tid; idx < TGR * TGC; idx += nthr) {
+                const int r = idx / TGC, c = idx - r * TGC;
+                const int gy = y0 - HG + r, gx = x0 - HG + c;
+                float gcv = 0.f, grv = 0.f;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                    const int r1 = gy - (y0 - HI), c1 = gx - (x0 - HI);
+                    const int r0 = (gy > 0 ? gy - 1 : 0) - (y0 - HI);
+                    const int c0 = (gx > 0 ? gx - 1 : 0) - (x0 - HI);
+                    const float a_r0c1 = s_in[r0 * TIC + c1];
+                    const float a_r1c0 = s_in[r1 * TIC + c0];
+                    const float a_r0c0 = s_in[r0 * TIC + c0];
+                    const float a_r1c1 = s_in[r1 * TIC + c1];
+                    gcv = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0;
+                    grv = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0;
+                }
+                s_g[idx] = make_float2(gcv, grv);
+            }
+            __syncthreads();
+            // (3a) Catmull-Rom along rows for every lattice row: the x2M gradient image sampled at row a/2
+            for (int idx = tid; idx < LR * TGC; idx += nthr) {
+                const int la = idx / TGC, c = idx - la * TGC;
+                const float4 wr = s_lrw[la];
+                const int rb = s_lrb[la];
+                const float wrv[4] = {wr.x, wr.y, wr.z, wr.w};
+                float sx = 0.f, sy = 0.f;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int tr = min(max(rb + u, 0), TGR - 1);
-                const float2 gv = s_g[tr * TGC + c];
-                sx += gv.x * wrv[u];
-                sy += gv.y * wrv[u];
+                for (int u = 0; u < 4; ++u) {
+                    const int tr = min(max(rb + u, 0), TGR - 1);
+                    const float2 gv = s_g[tr * TGC + c];
+                    sx += gv.x * wrv[u];
+                    sy += gv.y * wrv[u];
+                }
+                s_rxy[idx] = make_float2(sx, sy);
             }
-            s_rxy[idx] = make_float2(sx, sy);
-        }
-        __syncthreads();
-        // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
-        // column-parity planes so the 64 lanes of one read touch 64 consecutive float2
-        for (int idx = tid; idx < 2 * LPL; idx += nthr) {
-            const int plane = idx / LPL;
-            const int rem = idx - plane * LPL;
-            const int la = rem / LP;
-            const int pos = rem - la * LP;
-            const int lb = 2 * pos + plane;
-            const float4 wc = s_lcw[lb];
-            const int cb = s_lcb[lb];
-            const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
-            float gx = 0.f, gy = 0.f;
+            __syncthreads();
+            // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
+            // column-parity planes so the 64 lanes of one read touch 64 consecutive float2
+            for (int idx = tid; idx < 2 * LPL; idx += nthr) {
+                const int plane = idx / LPL;
+                const int rem = idx - plane * LPL;
+                const int la = rem / LP;
+                const int pos = rem - la * LP;
+                const int lb = 2 * pos + plane;
+                const float4 wc = s_lcw[lb];
+                const int cb = s_lcb[lb];
+                const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
+                float gx = 0.f, gy = 0.f;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int tc = min(max(cb + v, 0), TGC - 1);
-                const float2 rv = s_rxy[la * TGC + tc];
-                gx += rv.x * wcv[v];
-                gy += rv.y * wcv[v];
+                for (int v = 0; v < 4; ++v) {
+                    const int tc = min(max(cb + v, 0), TGC - 1);
+                    const float2 rv = s_rxy[la * TGC + tc];
+                    gx += rv.x * wcv[v];
+                    gy += rv.y * wcv[v];
+                }
+                const float n2 = gx * gx + gy * gy;
+                float2 g = make_float2(0.f, 0.f);
+                if (n2 > 0.f) {
+                    const float inv = rsqrtf(n2);
+                    g = make_float2(gx * inv, gy * inv);
+                }
+                s_lat[idx] = g;
             }
-            const float n2 = gx * gx + gy * gy;
-            float2 g = make_float2(0.f, 0.f);
-            if (n2 > 0.f) {
-                const float inv = rsqrtf(n2);
-                g = make_float2(gx * inv, gy * inv);
-            }
-            s_lat[idx] = g;
+            __syncthreads();
         }
-        __syncthreads();
         // otherwise they go to the lattice-row scratch, free from here on
-        if (!T_OWN && a.iw) {
+        if (!ARRAYS && !T_OWN && a.iw) {
             intensity_rows();
             __syncthreads();
         }
         // the next frame's tile: the tile of frame f has no reader left
-        if (f + 1 < f_end) {
+        if (!ARRAYS && f + 1 < f_end) {
             store_tile(f + 1);
             if (f + 2 < f_end) fetch(f + 2);
         }
@@ -617,16 +647,22 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 if (a.iw) {
 #pragma unroll
                     for (int q = 0; q < Q; ++q) {
-                        const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
-                        float sacc = 0.f;
+                        float sacc;
+                        if constexpr (ARRAYS) {
+                            const int rM = min(M * (y0 + q) + py, a.HM - 1);
+                            sacc = a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs];
+                        } else {
+                            const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
+                            sacc = 0.f;
 #pragma unroll
-                        for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
+                            for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
+                        }
                         acc[q] = acc[q] * sacc;
                     }
                 }
 #pragma unroll
                 for (int q = 0; q < Q; ++q) acc[q] = pixel_ok(q) ? acc[q] : 0.f;
-                if constexpr (MODE == kModeFrames) {
+                if constexpr (FRAMES) {
 #pragma unroll
                     for (int q = 0; q < Q; ++q) {
                         const int yi = y0 + q;
@@ -654,7 +690,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         __syncthreads();
     }
 
-    if (MODE != kModeFrames && active && f_end > f_begin) {
+    if (!FRAMES && active && f_end > f_begin) {
         const size_t P = static_cast<size_t>(a.HM) * a.WM;
         const int pslot = a.chunk0 + chunk;
 #pragma unroll

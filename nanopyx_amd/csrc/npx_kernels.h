@@ -9,6 +9,7 @@ constexpr int kModeFrames = 0;       // per-frame RGC maps (eSRRF_ST.run contrac
 constexpr int kModeTemporal = 1;     // AVG / TAC2 moments streamed over the frames
 constexpr int kModeTemporalVar = 2;  // ... plus the sum of squared deviations for VAR
 constexpr int kModeSweep = 3;        // AVG for up to kMaxRad radii x kMaxSens sensitivities of one magnification
+constexpr int kModeArrays = 4;       // per-frame RGC maps from caller gradients sampled x2M and a magnified image
 constexpr int kMaxRad = 4;           // radii of one sweep job (they share the gradient lattice)
 constexpr int kMaxSens = 4;          // sensitivities of one sweep job (they share the ring sums)
 constexpr int kMaxJobs = 8;          // jobs of one batched launch (passed by value as kernel arguments)
@@ -97,6 +98,10 @@ struct FusedArgs {
     float* var;
     float* tac2;
     float* sweep_avg[kMaxRad * kMaxSens];  // sweep: AVG map of (radius r, sensitivity s) at [r * kMaxSens + s]
+    // arrays mode: column and row gradients (N, 2 HM, 2 WM) and the intensity image (N, HM, WM) of the caller
+    const float* agx;
+    const float* agy;
+    const float* aimg;
 };
 
 // Several jobs in one launch: job j owns blocks [start[j], start[j + 1]), laid out as

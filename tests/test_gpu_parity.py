@@ -274,3 +274,25 @@ def test_degenerate_gradients(torch, oracle_built, M, R, S):
         assert _report(f"degenerate {name} frames", out, ref) <= TOL
         avg = _hip.esrrf_temporal(torch.from_numpy(x).cuda(), M, R, S, True, ("AVG",))["AVG"].cpu().numpy()
         assert _report(f"degenerate {name} AVG", avg, ref.mean(axis=0)) <= TOL
+
+
+@pytest.mark.parametrize("N,HM,WM,M,R,S,iw,pad", [
+    (3, 40, 70, 2, 1.0, 3.0, False, False),     # fused kernel on the caller's lattice
+    (2, 50, 45, 5, 1.5, 1.0, True, False),
+    (2, 64, 136, 4, 2.0, 2.0, True, True),
+    (1, 70, 35, 7, 2.5, 1.7, True, True),       # 49 phases in groups; without pad_edges the margin covers it all
+    (2, 41, 71, 2, 1.0, 1.0, True, False),      # H x M does not tile the image: the per-pixel kernel
+])
+def test_rgc_arrays_paths(torch, oracle_built, N, HM, WM, M, R, S, iw, pad):
+    """RadialGradientConvergence on caller arrays (gradients x2M, magnified image) against the oracle's restatement
+    of the reference loop, through both kernels behind npx_rgc."""
+    from nanopyx_amd.core.transform import RadialGradientConvergence
+    print(f"rgc arrays N{N} {HM}x{WM} M{M} R{R} S{S} iw={iw} pad={pad}", flush=True)
+    rng = np.random.default_rng(HM * WM + M)
+    gx = rng.normal(0, 10, (N, 2 * HM, 2 * WM)).astype(np.float32)
+    gy = rng.normal(0, 10, (N, 2 * HM, 2 * WM)).astype(np.float32)
+    img = (rng.random((N, HM, WM)) * 100 + 1).astype(np.float32)
+    ref = oracle_built.rgc(gx, gy, img, M, R, S, iw, pad)
+    out = RadialGradientConvergence(verbose=False).run(gx, gy, img, magnification=M, radius=R, sensitivity=S,
+                                                       doIntensityWeighting=iw, pad_edges=pad)
+    assert _report(f"rgc arrays {HM}x{WM} M{M}", out, ref) <= TOL
