@@ -66,7 +66,9 @@ int npx_esrrf_temporal(const void *d_in, int dtype, int N, int H, int W, int M, 
 
 /*
  * RadialGradientConvergence on caller arrays: d_gx, d_gy (N, 2*HM, 2*WM), d_img (N, HM, WM), indexed exactly as
- * _c_calculate_rgc does (src/include/_c_sr_radial_gradient_convergence.c:17-90). Replaces
+ * _c_calculate_rgc does (src/include/_c_sr_radial_gradient_convergence.c:17-90). When HM and WM are multiples of M
+ * (and the radius is within the eSRRF kernels' range) the fused eSRRF kernel runs on the lattice the reference reads
+ * (every M-th position of the x2M arrays); other shapes run a per-pixel kernel. Replaces
  * RadialGradientConvergence._run_threaded / _run_opencl, src/nanopyx/core/transform/_le_radial_gradient_convergence.pyx:79-115
  * and :229-332 (pad_edges -> margin 0, :92-95).
  */
