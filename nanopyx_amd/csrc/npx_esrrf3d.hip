@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
 // (3) RGC of every output voxel over the chunk's frames, folded into the running statistics. The window geometry
 // (validity, offsets, distance, weight and the weight sum) does not depend on the frame, so it is formed once per
 // batch of kE3Batch frames and the batch's ring sums are kept in registers.
-constexpr int kE3Batch = 8;
+constexpr int kE3Batch = 16;
 constexpr float kE3Scale = 65536.f;  // s of the indicator clamp(-s g . d, 0, 1) (a power of two)
 
 // Voxels of a block: 64 consecutive columns (the lanes) x 4 waves; the waves take the Zb output slices of one
@@ -239,6 +239,8 @@ __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
     case n: e3_window<n>(a, m, volM, planeM, xc, yc, zc, ylim, xlim, zlim, rg, wsum); break;
                     NPX_E3_CASE(1) NPX_E3_CASE(2) NPX_E3_CASE(3) NPX_E3_CASE(4)
                     NPX_E3_CASE(5) NPX_E3_CASE(6) NPX_E3_CASE(7) NPX_E3_CASE(8)
+                    NPX_E3_CASE(9) NPX_E3_CASE(10) NPX_E3_CASE(11) NPX_E3_CASE(12)
+                    NPX_E3_CASE(13) NPX_E3_CASE(14) NPX_E3_CASE(15) NPX_E3_CASE(16)
 #undef NPX_E3_CASE
                     default: break;
                 }
