@@ -76,17 +76,18 @@ __global__ __launch_bounds__(256) void e3_grad3d(E3Args a) {
     }
 }
 
-// (2) Catmull-Rom xMxy of (Gx, Gy, Gz, intensity) of every slice: columns outer, rows inner, as _c_interpolate
-__global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
-    const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, vol = static_cast<size_t>(a.S) * a.R * a.C;
-    const size_t n = volM * a.nf, plane = static_cast<size_t>(a.R) * a.C;
-    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
-         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const size_t fl = t / volM, v = t % volM;
-        const int cM = static_cast<int>(v % a.CM), rM = static_cast<int>((v / a.CM) % a.RM),
-                  s = static_cast<int>(v / (static_cast<size_t>(a.RM) * a.CM));
-        const float* g = a.grad + fl * 3 * vol + s * plane;
-        const size_t ibase = (a.f0 + fl) * vol + s * plane;
+// (2) Catmull-Rom xMxy of (Gx, Gy, Gz, intensity) of every slice: columns outer, rows inner, as _c_interpolate.
+// Blocks of 64 x 4 output pixels of one (frame, slice) plane: x covers CM, y covers RM, z the planes (strided).
+constexpr int kE3MagRows = 4;
+__global__ __launch_bounds__(64 * kE3MagRows) void e3_magnify_xy(E3Args a) {
+    const size_t vol = static_cast<size_t>(a.S) * a.R * a.C, plane = static_cast<size_t>(a.R) * a.C;
+    const size_t planeM = static_cast<size_t>(a.RM) * a.CM;
+    const int cM = blockIdx.x * 64 + threadIdx.x, rM = blockIdx.y * kE3MagRows + threadIdx.y;
+    if (cM >= a.CM || rM >= a.RM) return;
+    for (int zs = blockIdx.z; zs < a.S * a.nf; zs += gridDim.z) {
+        const int fl = zs / a.S, s = zs - fl * a.S;
+        const float* g = a.grad + static_cast<size_t>(fl) * 3 * vol + s * plane;
+        const size_t ibase = (static_cast<size_t>(a.f0) + fl) * vol + s * plane;
         float4 r;
         if (a.Mxy == 1) {
             const size_t o = static_cast<size_t>(rM) * a.C + cM;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void e3_magnify_xy(E3Args a) {
         r.x *= inv;
         r.y *= inv;
         r.z *= inv;
-        a.mag[t] = r;
+        a.mag[static_cast<size_t>(zs) * planeM + static_cast<size_t>(rM) * a.CM + cM] = r;
     }
 }
 
@@ -445,9 +446,11 @@ int npx_esrrf3d(const void* d_img, int dtype, int F, int S, int R, int C, int Mx
         a.f0 = f0;
         a.nf = std::min(ch, F - f0);
         a.last = f0 + a.nf >= F;
-        const size_t ng = static_cast<size_t>(a.nf) * S * R * C, nm = static_cast<size_t>(a.nf) * S * a.RM * a.CM;
+        const size_t ng = static_cast<size_t>(a.nf) * S * R * C;
         hipLaunchKernelGGL(e3_grad3d, dim3(e3_grid(ng)), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(e3_magnify_xy, dim3(e3_grid(nm)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(e3_magnify_xy,
+                           dim3((a.CM + 63) / 64, (a.RM + kE3MagRows - 1) / kE3MagRows, std::min(S * a.nf, 65535)),
+                           dim3(64, kE3MagRows), 0, st, a);
         hipLaunchKernelGGL(e3_rgc3d, dim3(static_cast<int>(std::min<size_t>(e3_tiles(a), size_t(1) << 20))),
                            dim3(64 * kE3Waves), 0, st, a);
         hipError_t e = hipGetLastError();
