@@ -396,6 +396,39 @@ def end_to_end(x_host, e, reps=2):
                                             f"{reps} after one warm-up"}
 
 
+def end_to_end_streamed(x_host, e, reps=2):
+    """Host stack -> maps on the host through a serving session: the session uploads the pinned stack in pieces of
+    about 32 MB and runs each slice of frames as soon as its piece has arrived, so the copy overlaps the kernels;
+    wall clock per call (submit, step until done, maps copied out)."""
+    import torch
+    from nanopyx_amd import _hip
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    modes = tuple(e["temporal"])
+    pinned = torch.from_numpy(x_host).pin_memory()
+    N, H, W = x_host.shape
+    need = _hip.Session.job_bytes(N, H, W, M, R, x_host.dtype == np.uint16, True)
+    best = None
+    for _ in range(reps + 1):
+        outs = {m: torch.empty((H * M, W * M), device="cuda") for m in modes}
+        sess = _hip.Session(need + (64 << 20), 1, 1 << 29)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        jid = sess.submit(pinned, M, R, S, iw, modes, outs)
+        while sess.poll(jid) != _hip.Session.DONE:
+            sess.step()
+        host = {k: v.cpu() for k, v in outs.items()}
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = sess.stats()
+        sess.close()
+        best = dt if best is None or dt < best else best
+        del outs, host
+    return {"ms": round(best * 1e3, 3), "value": round(N * H * M * W * M / 1e6 / best, 1),
+            "unit": "Mpx·frames/s", "slices": st["slices"], "launches": st["launches"],
+            "note": f"pinned host stack uploaded in pieces overlapping the kernels (serving session), maps copied "
+                    f"out, best of {reps} after one warm-up"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -552,6 +585,7 @@ def main():
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
     log("end to end (host stack in, maps out) ...")
     result["end_to_end_with_pcie"] = end_to_end(x_host, e)
+    result["end_to_end_streamed"] = end_to_end_streamed(x_host, e)
     if not args.no_cpu:
         log("timing the reference CPU path ...")
         result["cpu_baseline"], ref_maps = cpu_baseline(x_host, e, args.cpu_seconds)

@@ -125,11 +125,12 @@ int npx_graph_destroy(void *graph);
  * max_active jobs are admitted at once; one step aims to process about px_frames_per_step output pixel-frames.
  *
  * npx_session_submit queues a job (never blocks). src is a device pointer (src_is_host 0; it must stay valid until
- * the job is done) or a host pointer, preferably pinned (src_is_host 1; copied into the pool on admission, it must
- * stay valid until then). Outputs are caller-owned device maps (H*M, W*M); those of unset mode bits may be NULL.
+ * the job is done) or a host pointer, preferably pinned (src_is_host 1; copied into the pool from admission on, in
+ * pieces of about 32 MB whose frames become runnable as each piece lands; it must stay valid until the job is done).
+ * Outputs are caller-owned device maps (H*M, W*M); those of unset mode bits may be NULL.
  *
  * npx_session_step retires finished jobs, admits queued jobs that fit the pool (arrival order; a job that does not
- * fit yet lets later ones pass), then launches one frame slice of every admitted job whose copy has landed, packed
+ * fit yet lets later ones pass), then launches one frame slice of every admitted job whose frames have landed, packed
  * up to 8 jobs per launch, on `stream`. It never blocks; it launches nothing while two earlier steps are still
  * running on the GPU. *launches receives the number of launches made.
  *

@@ -111,7 +111,13 @@ struct Job {
     host::TableBlob blob;       // host copy of the tables (kept until retirement: source of the async copy)
     DevTables t{};
     const void* d_in = nullptr;
-    hipEvent_t copied = nullptr, done = nullptr;
+    hipEvent_t copied = nullptr, done = nullptr;  // copied: the tables are on the device
+    // a host stack is uploaded in pieces, each followed by an event, so that its first slices run while the rest
+    // is still being copied: pieces[k] marks frames [0, piece_end[k]) on the device
+    std::vector<hipEvent_t> pieces;
+    std::vector<int> piece_end;
+    size_t pieces_done = 0;
+    int arrived = 0;  // frames known to be on the device
     int next_frame = 0;
     bool final_launched = false;
     int state = kQueued;
@@ -137,12 +143,15 @@ constexpr size_t kMaxInflight = 2;  // steps queued on the GPU ahead of the host
 constexpr int kMinSliceFrames = 8;  // frames per slice (and per chunk of a slice) at least
 constexpr int kMaxChunks = 8;       // frame chunks of one slice at most
 constexpr int kTargetBlocks = 1024; // blocks a lone job's slice aims for (chunks add blocks to small jobs)
+constexpr size_t kPieceBytes = size_t(32) << 20;  // host-stack upload piece, about (whole frames, whole chunks)
 
 // Releases a finished job's pool block and events (its maps are written and complete).
 void retire(Session& s, Job& j) {
     s.pool.release(j.off, j.bytes);
     if (j.copied) (void)hipEventDestroy(j.copied);
     if (j.done) (void)hipEventDestroy(j.done);
+    for (hipEvent_t ev : j.pieces) (void)hipEventDestroy(ev);
+    j.pieces.clear();
     j.copied = j.done = nullptr;
     j.blob.bytes.clear();
     j.blob.bytes.shrink_to_fit();
@@ -173,11 +182,26 @@ int admit(Session& s) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&j.done, hipEventDisableTiming);
         if (e == hipSuccess)
             e = hipMemcpyAsync(base, j.blob.bytes.data(), j.blob.bytes.size(), hipMemcpyHostToDevice, s.copy);
-        if (e == hipSuccess && j.src_host)
-            e = hipMemcpyAsync(base + j.stack_at, j.src,
-                               static_cast<size_t>(j.N) * j.H * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4),
-                               hipMemcpyHostToDevice, s.copy);
         if (e == hipSuccess) e = hipEventRecord(j.copied, s.copy);
+        if (j.src_host) {
+            // pieces of whole frames (and whole chunks when a slice runs as chunks)
+            const size_t fb = static_cast<size_t>(j.H) * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4);
+            int pf = static_cast<int>(std::max<size_t>(kMinSliceFrames, kPieceBytes / fb));
+            if (j.cmax > 1) pf = std::max(1, pf / j.cfpc) * j.cfpc;
+            for (int f0 = 0; e == hipSuccess && f0 < j.N; f0 += pf) {
+                const int nf = std::min(pf, j.N - f0);
+                hipEvent_t ev = nullptr;
+                e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+                if (e != hipSuccess) break;
+                j.pieces.push_back(ev);
+                j.piece_end.push_back(f0 + nf);
+                e = hipMemcpyAsync(base + j.stack_at + f0 * fb, static_cast<const uint8_t*>(j.src) + f0 * fb,
+                                   nf * fb, hipMemcpyHostToDevice, s.copy);
+                if (e == hipSuccess) e = hipEventRecord(ev, s.copy);
+            }
+        } else {
+            j.arrived = j.N;
+        }
         if (e != hipSuccess) {
             s.pool.release(off, j.bytes);
             return hip_fail(e, "session admission copy");
@@ -208,7 +232,10 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
     std::vector<Job*> run;
     for (auto& kv : s.jobs) {
         Job& j = kv.second;
-        if (j.state == kAdmitted && !j.final_launched && hipEventQuery(j.copied) == hipSuccess) run.push_back(&j);
+        if (j.state != kAdmitted || j.final_launched || hipEventQuery(j.copied) != hipSuccess) continue;
+        while (j.pieces_done < j.pieces.size() && hipEventQuery(j.pieces[j.pieces_done]) == hipSuccess)
+            j.arrived = j.piece_end[j.pieces_done++];
+        if (j.arrived > j.next_frame) run.push_back(&j);
     }
     if (run.empty()) return NPX_OK;
     // Frames of each job this step. Small jobs (remaining output pixel-frames at most a quarter of the step's
@@ -235,7 +262,7 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
         const long long P = static_cast<long long>(j.plan->HM) * j.plan->WM;
         long long f = std::max<long long>(kMinSliceFrames, left / P);
         if (j.cmax > 1) f = std::max<long long>(1, f / j.cfpc) * j.cfpc;  // whole chunks
-        f = std::min<long long>(j.N - j.next_frame, f);
+        f = std::min<long long>(j.arrived - j.next_frame, f);  // frames on the device (pieces end on chunk bounds)
         frames.push_back(static_cast<int>(f));
         left -= f * P;
         keep = k + 1;
@@ -351,6 +378,12 @@ void layout(Job& j, const PlanHost& plan) {
                                                                                          spatial)));
     j.cmax = std::min(j.cmax, std::max(1, j.N / kMinSliceFrames));
     j.cfpc = std::max(kMinSliceFrames, (j.N + j.cmax - 1) / j.cmax);
+    if (j.src_host) {
+        // chunks no longer than an upload piece, so that slices can follow the pieces as they arrive
+        const size_t fb = static_cast<size_t>(j.H) * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4);
+        const int pf = static_cast<int>(std::max<size_t>(kMinSliceFrames, kPieceBytes / fb));
+        j.cfpc = std::max(kMinSliceFrames, std::min(j.cfpc, pf));
+    }
     j.state_at = align_up(j.blob.bytes.size());
     j.part_at = j.state_at + align_up(5 * P * sizeof(float));
     j.stack_at = j.part_at + (j.cmax > 1 ? align_up(static_cast<size_t>(j.cmax) * 5 * P * sizeof(float)) : 0);
@@ -496,6 +529,7 @@ int npx_session_destroy(void* session) {
     if (e != hipSuccess) rc = npx::host::hip_fail(e, "hipDeviceSynchronize(session destroy)");
     for (auto& kv : s->jobs) {
         if (kv.second.copied) (void)hipEventDestroy(kv.second.copied);
+        for (hipEvent_t ev : kv.second.pieces) (void)hipEventDestroy(ev);
         if (kv.second.done) (void)hipEventDestroy(kv.second.done);
     }
     for (hipEvent_t ev : s->inflight) (void)hipEventDestroy(ev);
