@@ -203,6 +203,14 @@ int admit(Session& s) {
             j.arrived = j.N;
         }
         if (e != hipSuccess) {
+            // the job stays queued; nothing may still write into the block it gives back
+            (void)hipStreamSynchronize(s.copy);
+            if (j.copied) (void)hipEventDestroy(j.copied);
+            if (j.done) (void)hipEventDestroy(j.done);
+            for (hipEvent_t ev : j.pieces) (void)hipEventDestroy(ev);
+            j.copied = j.done = nullptr;
+            j.pieces.clear();
+            j.piece_end.clear();
             s.pool.release(off, j.bytes);
             return hip_fail(e, "session admission copy");
         }
