@@ -540,31 +540,40 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             }
             __syncthreads();
             // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
-            // column-parity planes so the 64 lanes of one read touch 64 consecutive float2
-            for (int idx = tid; idx < 2 * LPL; idx += nthr) {
-                const int plane = idx / LPL;
-                const int rem = idx - plane * LPL;
-                const int la = rem / LP;
-                const int pos = rem - la * LP;
-                const int lb = 2 * pos + plane;
-                const float4 wc = s_lcw[lb];
-                const int cb = s_lcb[lb];
-                const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
-                float gx = 0.f, gy = 0.f;
+            // column-parity planes so the 64 lanes of one read touch 64 consecutive float2. A thread keeps one
+            // lattice column (its taps and weights) and takes every rg_n-th lattice row of it.
+            {
+                constexpr int NCOL = 2 * LP;  // (plane, position) columns of the two planes
+                const int rg_n = max(1, nthr / NCOL);
+                for (int t = tid; t < NCOL * rg_n; t += nthr) {
+                    const int col = t % NCOL, rg = t / NCOL;
+                    const int plane = col >= LP ? 1 : 0;
+                    const int pos = col - plane * LP;
+                    const int lb = 2 * pos + plane;
+                    const float4 wc = s_lcw[lb];
+                    const int cb = s_lcb[lb];
+                    const float wcv[4] = {wc.x, wc.y, wc.z, wc.w};
+                    int tcv[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int tc = min(max(cb + v, 0), TGC - 1);
-                    const float2 rv = s_rxy[la * TGC + tc];
-                    gx += rv.x * wcv[v];
-                    gy += rv.y * wcv[v];
+                    for (int v = 0; v < 4; ++v) tcv[v] = min(max(cb + v, 0), TGC - 1);
+                    for (int la = rg; la < LR; la += rg_n) {
+                        const float2* row = s_rxy + la * TGC;
+                        float gx = 0.f, gy = 0.f;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const float2 rv = row[tcv[v]];
+                            gx += rv.x * wcv[v];
+                            gy += rv.y * wcv[v];
+                        }
+                        const float n2 = gx * gx + gy * gy;
+                        float2 g = make_float2(0.f, 0.f);
+                        if (n2 > 0.f) {
+                            const float inv = rsqrtf(n2);
+                            g = make_float2(gx * inv, gy * inv);
+                        }
+                        s_lat[plane * LPL + la * LP + pos] = g;
+                    }
                 }
-                const float n2 = gx * gx + gy * gy;
-                float2 g = make_float2(0.f, 0.f);
-                if (n2 > 0.f) {
-                    const float inv = rsqrtf(n2);
-                    g = make_float2(gx * inv, gy * inv);
-                }
-                s_lat[idx] = g;
             }
             __syncthreads();
         }

@@ -109,29 +109,40 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                         sx = (sx + s_gc[tr] * wr[u]).astype(np.float32)
                         sy = (sy + s_gr[tr] * wr[u]).astype(np.float32)
                     s_rx[la], s_ry[la] = sx, sy
-                # (3b) columns + normalisation into the two column-parity planes
+                # (3b) columns + normalisation into the two column-parity planes: one lattice column per thread
+                # (plane, position), every rg_n-th lattice row of it
                 s_lat = np.zeros((2 * LPL, 2), np.float32)
-                for idx in range(2 * LPL):
-                    plane, rem = divmod(idx, LPL)
-                    la, pos = divmod(rem, LP)
+                seen = np.zeros(2 * LPL, bool)
+                NCOL = 2 * LP
+                NTHR = 64 * min(16, nphase * len(radii))  # threads of a block (up to 16 wave slots)
+                rg_n = max(1, NTHR // NCOL)
+                for tt in range(NCOL * rg_n):
+                    col, rg = tt % NCOL, tt // NCOL
+                    plane = 1 if col >= LP else 0
+                    pos = col - plane * LP
                     bb = b_lo + 2 * pos + plane
-                    if not (1 <= bb <= 2 * WM - 2):
-                        continue
-                    bi = min(bb, 2 * W)
-                    wc = t["lcw"][bi]
-                    cb = t["lcb"][bi] - (x0 - HG)
-                    gxv = f32(0)
-                    gyv = f32(0)
-                    for v in range(4):
-                        tc = min(max(cb + v, 0), TGC - 1)
-                        if wc[v] != 0:
-                            assert 0 <= cb + v < TGC, ("lattice column tap outside tile", bb)
-                        gxv = f32(gxv + s_rx[la, tc] * wc[v])
-                        gyv = f32(gyv + s_ry[la, tc] * wc[v])
-                    n2 = f32(gxv * gxv + gyv * gyv)
-                    if n2 > 0:
-                        inv = f32(1.0 / np.sqrt(np.float64(n2)))
-                        s_lat[idx] = (gxv * inv, gyv * inv)
+                    for la in range(rg, LR, rg_n):
+                        idx = plane * LPL + la * LP + pos
+                        assert 0 <= idx < 2 * LPL and not seen[idx]
+                        seen[idx] = True
+                        if not (1 <= bb <= 2 * WM - 2):
+                            continue
+                        bi = min(bb, 2 * W)
+                        wc = t["lcw"][bi]
+                        cb = t["lcb"][bi] - (x0 - HG)
+                        gxv = f32(0)
+                        gyv = f32(0)
+                        for v in range(4):
+                            tc = min(max(cb + v, 0), TGC - 1)
+                            if wc[v] != 0:
+                                assert 0 <= cb + v < TGC, ("lattice column tap outside tile", bb)
+                            gxv = f32(gxv + s_rx[la, tc] * wc[v])
+                            gyv = f32(gyv + s_ry[la, tc] * wc[v])
+                        n2 = f32(gxv * gxv + gyv * gyv)
+                        if n2 > 0:
+                            inv = f32(1.0 / np.sqrt(np.float64(n2)))
+                            s_lat[idx] = (gxv * inv, gyv * inv)
+                assert seen.all(), "lattice points left unwritten"
                 # (4)+(5) waves: one per (radius, phase)
                 for ri, R in enumerate(radii):
                     tr_ = tabs[ri]
