@@ -1,7 +1,7 @@
 """Baseline TIFF stacks: what the eSRRF command line reads and writes.
 
 The reference's command line (src/nanopyx/cli/superresolution.py:1-3, :90-104) uses ``tifffile.imread`` /
-``tifffile.imwrite``. That package is not part of this build's environment, so this module implements the subset
+``tifffile.imwrite``. That package is not a dependency of this package, so this module implements the subset
 of TIFF 6.0 those calls exchange for microscopy stacks: one image per page (IFD), uncompressed, one or several
 strips, grayscale, 8/16/32-bit unsigned integers or 32/64-bit IEEE floats, either byte order. ``imread`` returns
 (pages, H, W) for multi-page files and (H, W) for single pages, as tifffile does for plain multi-page files;
