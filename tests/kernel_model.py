@@ -31,10 +31,14 @@ def _pow_mode(Sf):
     return 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
 
 
-def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
+def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
     """Per-frame maps of every (radius, sensitivity), computed as one block of the batched kernel does: the lattice
     is built for half-width KL (default: the largest K of the radii) and every wave runs the ring of its own radius
-    at offset KL - K into it. Returns {(R, S): (N, HM, WM) float32}."""
+    at offset KL - K into it. Returns {(R, S): (N, HM, WM) float32}.
+
+    arrays = (gx, gy, img, pad_edges): the kernel's arrays mode (npx_rgc): the lattice is gathered from the caller's
+    (N, 2HM, 2WM) gradients, the weight is the caller's (N, HM, WM) image and pad_edges sets the margin to 0; the
+    stack then only gives (N, H, W)."""
     stack = np.asarray(stack)
     N, H, W = stack.shape
     tabs = [_hip.plan_tables(H, W, M, R) for R in radii]
@@ -67,82 +71,101 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                 gx = x0 - HI + np.arange(TIC)
                 oky, okx = (gy >= 0) & (gy < H), (gx >= 0) & (gx < W)
                 s_in[np.ix_(oky, okx)] = img[np.ix_(gy[oky], gx[okx])]
-                # (2) Roberts
-                s_gc = np.zeros((TGR, TGC), np.float32)
-                s_gr = np.zeros((TGR, TGC), np.float32)
-                for r in range(TGR):
-                    yy = y0 - HG + r
-                    if yy < 0 or yy >= H:
-                        continue
-                    cc = np.arange(TGC)
-                    xx = x0 - HG + cc
-                    ok = (xx >= 0) & (xx < W)
-                    r1 = yy - (y0 - HI)
-                    r0 = (yy - 1 if yy > 0 else 0) - (y0 - HI)
-                    c1 = xx - (x0 - HI)
-                    c0 = np.where(xx > 0, xx - 1, 0) - (x0 - HI)
-                    assert 0 <= r0 < TIR and 0 <= r1 < TIR
-                    assert (c0[ok] >= 0).all() and (c1[ok] < TIC).all()
-                    a_r0c1 = s_in[r0, c1[ok]]
-                    a_r1c0 = s_in[r1, c0[ok]]
-                    a_r0c0 = s_in[r0, c0[ok]]
-                    a_r1c1 = s_in[r1, c1[ok]]
-                    s_gc[r, ok] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0
-                    s_gr[r, ok] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0
-                # (3a) rows: Catmull-Rom of the gradient tile along rows for every lattice row
-                a_lo, b_lo = 2 * y0 - KL, 2 * x0 - KL
-                s_rx = np.zeros((LR, TGC), np.float32)
-                s_ry = np.zeros((LR, TGC), np.float32)
-                for la in range(LR):
-                    aa = a_lo + la
-                    if not (1 <= aa <= 2 * HM - 2):
-                        continue
-                    ai = min(aa, 2 * H)
-                    wr = t["lrw"][ai]
-                    rb = t["lrb"][ai] - (y0 - HG)
-                    sx = np.zeros(TGC, np.float32)
-                    sy = np.zeros(TGC, np.float32)
-                    for u in range(4):
-                        tr = min(max(rb + u, 0), TGR - 1)
-                        if wr[u] != 0:
-                            assert 0 <= rb + u < TGR, ("lattice row tap outside tile", aa)
-                        sx = (sx + s_gc[tr] * wr[u]).astype(np.float32)
-                        sy = (sy + s_gr[tr] * wr[u]).astype(np.float32)
-                    s_rx[la], s_ry[la] = sx, sy
-                # (3b) columns + normalisation into the two column-parity planes: one lattice column per thread
-                # (plane, position), every rg_n-th lattice row of it
-                s_lat = np.zeros((2 * LPL, 2), np.float32)
-                seen = np.zeros(2 * LPL, bool)
-                NCOL = 2 * LP
-                NTHR = 64 * min(16, nphase * len(radii))  # threads of a block (up to 16 wave slots)
-                rg_n = max(1, NTHR // NCOL)
-                for tt in range(NCOL * rg_n):
-                    col, rg = tt % NCOL, tt // NCOL
-                    plane = 1 if col >= LP else 0
-                    pos = col - plane * LP
-                    bb = b_lo + 2 * pos + plane
-                    for la in range(rg, LR, rg_n):
-                        idx = plane * LPL + la * LP + pos
-                        assert 0 <= idx < 2 * LPL and not seen[idx]
-                        seen[idx] = True
-                        if not (1 <= bb <= 2 * WM - 2):
+                if arrays is None:
+                    # (2) Roberts
+                    s_gc = np.zeros((TGR, TGC), np.float32)
+                    s_gr = np.zeros((TGR, TGC), np.float32)
+                    for r in range(TGR):
+                        yy = y0 - HG + r
+                        if yy < 0 or yy >= H:
                             continue
-                        bi = min(bb, 2 * W)
-                        wc = t["lcw"][bi]
-                        cb = t["lcb"][bi] - (x0 - HG)
-                        gxv = f32(0)
-                        gyv = f32(0)
-                        for v in range(4):
-                            tc = min(max(cb + v, 0), TGC - 1)
-                            if wc[v] != 0:
-                                assert 0 <= cb + v < TGC, ("lattice column tap outside tile", bb)
-                            gxv = f32(gxv + s_rx[la, tc] * wc[v])
-                            gyv = f32(gyv + s_ry[la, tc] * wc[v])
+                        cc = np.arange(TGC)
+                        xx = x0 - HG + cc
+                        ok = (xx >= 0) & (xx < W)
+                        r1 = yy - (y0 - HI)
+                        r0 = (yy - 1 if yy > 0 else 0) - (y0 - HI)
+                        c1 = xx - (x0 - HI)
+                        c0 = np.where(xx > 0, xx - 1, 0) - (x0 - HI)
+                        assert 0 <= r0 < TIR and 0 <= r1 < TIR
+                        assert (c0[ok] >= 0).all() and (c1[ok] < TIC).all()
+                        a_r0c1 = s_in[r0, c1[ok]]
+                        a_r1c0 = s_in[r1, c0[ok]]
+                        a_r0c0 = s_in[r0, c0[ok]]
+                        a_r1c1 = s_in[r1, c1[ok]]
+                        s_gc[r, ok] = a_r0c1 - a_r1c0 + a_r1c1 - a_r0c0
+                        s_gr[r, ok] = -a_r0c1 + a_r1c0 + a_r1c1 - a_r0c0
+                    # (3a) rows: Catmull-Rom of the gradient tile along rows for every lattice row
+                    a_lo, b_lo = 2 * y0 - KL, 2 * x0 - KL
+                    s_rx = np.zeros((LR, TGC), np.float32)
+                    s_ry = np.zeros((LR, TGC), np.float32)
+                    for la in range(LR):
+                        aa = a_lo + la
+                        if not (1 <= aa <= 2 * HM - 2):
+                            continue
+                        ai = min(aa, 2 * H)
+                        wr = t["lrw"][ai]
+                        rb = t["lrb"][ai] - (y0 - HG)
+                        sx = np.zeros(TGC, np.float32)
+                        sy = np.zeros(TGC, np.float32)
+                        for u in range(4):
+                            tr = min(max(rb + u, 0), TGR - 1)
+                            if wr[u] != 0:
+                                assert 0 <= rb + u < TGR, ("lattice row tap outside tile", aa)
+                            sx = (sx + s_gc[tr] * wr[u]).astype(np.float32)
+                            sy = (sy + s_gr[tr] * wr[u]).astype(np.float32)
+                        s_rx[la], s_ry[la] = sx, sy
+                    # (3b) columns + normalisation into the two column-parity planes: one lattice column per thread
+                    # (plane, position), every rg_n-th lattice row of it
+                    s_lat = np.zeros((2 * LPL, 2), np.float32)
+                    seen = np.zeros(2 * LPL, bool)
+                    NCOL = 2 * LP
+                    NTHR = 64 * min(16, nphase * len(radii))  # threads of a block (up to 16 wave slots)
+                    rg_n = max(1, NTHR // NCOL)
+                    for tt in range(NCOL * rg_n):
+                        col, rg = tt % NCOL, tt // NCOL
+                        plane = 1 if col >= LP else 0
+                        pos = col - plane * LP
+                        bb = b_lo + 2 * pos + plane
+                        for la in range(rg, LR, rg_n):
+                            idx = plane * LPL + la * LP + pos
+                            assert 0 <= idx < 2 * LPL and not seen[idx]
+                            seen[idx] = True
+                            if not (1 <= bb <= 2 * WM - 2):
+                                continue
+                            bi = min(bb, 2 * W)
+                            wc = t["lcw"][bi]
+                            cb = t["lcb"][bi] - (x0 - HG)
+                            gxv = f32(0)
+                            gyv = f32(0)
+                            for v in range(4):
+                                tc = min(max(cb + v, 0), TGC - 1)
+                                if wc[v] != 0:
+                                    assert 0 <= cb + v < TGC, ("lattice column tap outside tile", bb)
+                                gxv = f32(gxv + s_rx[la, tc] * wc[v])
+                                gyv = f32(gyv + s_ry[la, tc] * wc[v])
+                            n2 = f32(gxv * gxv + gyv * gyv)
+                            if n2 > 0:
+                                inv = f32(1.0 / np.sqrt(np.float64(n2)))
+                                s_lat[idx] = (gxv * inv, gyv * inv)
+                    assert seen.all(), "lattice points left unwritten"
+                else:
+                    # lattice point (a, b): the caller's gradient at (min(a M, 2 HM - 1), min(b M, 2 WM - 1)),
+                    # normalised; 0 outside a in [1, 2 HM - 2], b in [1, 2 WM - 2]
+                    agx, agy = arrays[0], arrays[1]
+                    s_lat = np.zeros((2 * LPL, 2), np.float32)
+                    for idx in range(2 * LPL):
+                        plane, rem = divmod(idx, LPL)
+                        la, pos = divmod(rem, LP)
+                        aa, bb = 2 * y0 - KL + la, 2 * x0 - KL + 2 * pos + plane
+                        if not (1 <= aa <= 2 * HM - 2 and 1 <= bb <= 2 * WM - 2):
+                            continue
+                        ra, ca = min(aa * M, 2 * HM - 1), min(bb * M, 2 * WM - 1)
+                        assert 0 <= ra < 2 * HM and 0 <= ca < 2 * WM
+                        gxv, gyv = f32(agx[f, ra, ca]), f32(agy[f, ra, ca])
                         n2 = f32(gxv * gxv + gyv * gyv)
                         if n2 > 0:
                             inv = f32(1.0 / np.sqrt(np.float64(n2)))
                             s_lat[idx] = (gxv * inv, gyv * inv)
-                assert seen.all(), "lattice points left unwritten"
                 # (4)+(5) waves: one per (radius, phase)
                 for ri, R in enumerate(radii):
                     tr_ = tabs[ri]
@@ -150,7 +173,7 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                     DK = KL - K
                     NW = 2 * K + 1
                     NR = 2 * Q + 2 * K - 1
-                    margin = infos[ri]["margin"]
+                    margin = 0 if (arrays is not None and arrays[3]) else infos[ri]["margin"]
                     dyt, dxt, dtab, ttab = tr_["dy"], tr_["dx"], tr_["d"], tr_["t"]
                     wsum = tr_["wsum"]
                     for ph in range(nphase):
@@ -210,7 +233,9 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None):
                             with np.errstate(invalid="ignore", divide="ignore"):
                                 rgc0 = np.where(den > 0, acc[q] / den, 0).astype(np.float32)
                             inten = None
-                            if iw:
+                            if iw and arrays is not None:
+                                inten = arrays[2][f, min(rM, HM - 1), cMs].astype(np.float32)
+                            elif iw:
                                 rMs = min(rM, HM - 1)
                                 wrow = t["crw"][rMs]
                                 crow = t["crb"][rMs] - (y0 - HI)

@@ -77,3 +77,23 @@ def test_phase_row_sets_fit(M):
     for nwaves, nrad, ngroups in shapes:
         for g in range(ngroups):
             assert len(_group_rows(nphase, M, nwaves, nrad, g)) <= 4, (M, nwaves, nrad, g)
+
+
+@pytest.mark.parametrize("N,H,W,M,R,S,iw,pad", [
+    (1, 20, 35, 2, 1.0, 3.0, False, False),
+    (2, 10, 9, 5, 1.5, 1.0, True, False),
+    (1, 16, 34, 4, 2.0, 2.0, True, True),
+    (1, 10, 5, 7, 2.5, 1.7, True, True),      # 49 phases
+])
+def test_model_arrays_mode(oracle_built, N, H, W, M, R, S, iw, pad):
+    """The kernel's arrays mode (npx_rgc on caller gradients) block by block: the lattice gathered from the x2M
+    arrays and the caller's weight image reproduce the oracle's restatement of the reference loop."""
+    from kernel_model import model_multi
+    rng = np.random.default_rng(H * W + M)
+    HM, WM = H * M, W * M
+    gx = rng.normal(0, 10, (N, 2 * HM, 2 * WM)).astype(np.float32)
+    gy = rng.normal(0, 10, (N, 2 * HM, 2 * WM)).astype(np.float32)
+    img = (rng.random((N, HM, WM)) * 100 + 1).astype(np.float32)
+    out = model_multi(np.zeros((N, H, W), np.float32), M, [R], [S], iw=iw, arrays=(gx, gy, img, pad))
+    ref = oracle_built.rgc(gx, gy, img, M, R, S, iw, pad)
+    assert rel_err(out[(float(R), float(S))], ref)[0] <= 1e-5
