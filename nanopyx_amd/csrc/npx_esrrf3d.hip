@@ -135,18 +135,28 @@ __global__ __launch_bounds__(64 * kE3MagRows) void e3_magnify_xy(E3Args a) {
 constexpr int kE3Batch = 16;
 constexpr float kE3Scale = 65536.f;  // s of the indicator clamp(-s g . d, 0, 1) (a power of two)
 
-// Voxels of a block: 64 consecutive columns (the lanes) x 4 waves; the waves take the Zb output slices of one
-// source slice (Zb = Mz when Mz is 2 or 4, else 1) times 4 / Zb output rows of one phase (stride M), so their window
-// samples come from the same source slices and largely the same magnified rows and meet in L1.
+// Voxels of a block: 64 consecutive columns (the lanes) x kE3Waves rows of one phase (stride M), one row per wave, so
+// the waves' window samples come from largely the same magnified rows and meet in L1. A thread takes the ZB output
+// slices of one source slice (ZB = Mz when Mz is 2 or 4, else 1): with Mz a power of two their centres zc = sM / Mz
+// and sample positions vz = zc + k are exact, so every sub-slice sees the same offsets (dz = k), distances, weights
+// and source records and they differ only in which z offsets fall inside the volume. The thread gathers each record
+// once and adds its term to the sums of the sub-slices it is valid for: the sums are those of separate threads, bit
+// for bit.
 constexpr int kE3Waves = 4;
 __host__ __device__ inline int e3_zb(int Mz) { return (Mz == 2 || Mz == 4) ? Mz : 1; }
 
-// Window sums of one output voxel for NB frames of a batch (rg[0..NB)) and the weight sum. The NB gathers of a
-// window sample are issued together, before any of them is used.
-template <int NB>
+// Window sums of the ZB sub-slices of one output column and row for NB frames of a batch (rg[r * NBM + q], NBM =
+// kE3Batch / ZB frames at most) and their weight sums. The NB gathers of a window sample are issued together, before
+// any of them is used.
+template <int ZB, int NB>
 __device__ __forceinline__ void e3_window(const E3Args& a, const float4* m, size_t volM, size_t planeM, float xc,
-                                          float yc, float zc, float ylim, float xlim, float zlim,
-                                          float (&rg)[kE3Batch], float& wsum) {
+                                          float yc, const float (&zc)[ZB], float ylim, float xlim, float zlim,
+                                          float (&rg)[kE3Batch], float (&wsum)[ZB]) {
+    // no contraction of a * b + c into FMAs beyond the explicit fmaf calls: every batch size (a separate inlined
+    // copy of this body) then rounds alike, so results do not depend on how the frames are batched or chunked
+#pragma clang fp contract(off)
+    constexpr int NBM = kE3Batch / ZB;
+    static_assert(NB <= NBM, "batch larger than the sums kept per sub-slice");
     for (int j = a.j0; j <= a.j1; ++j) {
         const float vy = yc + j;
         if (!(0.f < vy && vy <= ylim)) continue;
@@ -161,23 +171,35 @@ __device__ __forceinline__ void e3_window(const E3Args& a, const float4* m, size
             const size_t oyx = oy + static_cast<int>(a.Mxy * vx);
             const float wxy = __expf(-dxy2 / a.tSS);
             for (int k = a.k0; k <= a.k1; ++k) {
-                const float vz = zc + k;
-                if (!(0.f < vz && vz <= zlim)) continue;
-                const float dz = vz - zc;
+                bool ok[ZB];
+                int first = -1;
+#pragma unroll
+                for (int r = 0; r < ZB; ++r) {
+                    const float vz = zc[r] + k;
+                    ok[r] = 0.f < vz && vz <= zlim;
+                    if (ok[r] && first < 0) first = r;
+                }
+                if (first < 0) continue;
+                float zf = zc[0];
+#pragma unroll
+                for (int r = 1; r < ZB; ++r) zf = first == r ? zc[r] : zf;
+                const float vz = zf + k;
+                const float dz = vz - zf;  // the same for every sub-slice
                 const float d2 = dxy2 + dz * dz;
                 if (d2 == 0.f || !(dz <= a.tSO_z)) continue;
                 const float4* g = m + static_cast<size_t>(static_cast<int>(vz * a.Mz) / a.Mz) * planeM +
-                                  oyx;  // source slice of the repeated volume
-                float3 G[NB];  // (g, I) records with g = G / |G|; only g is read here
+                                  oyx;  // source slice of the repeated volume (the same for every sub-slice)
+                float3 G[NB];  // g = G / |G| of the batch's frames
 #pragma unroll
                 for (int q = 0; q < NB; ++q) {
-                    const float4 r = g[q * volM];
-                    G[q] = make_float3(r.x, r.y, r.z);
+                    const float4 rr = g[q * volM];
+                    G[q] = make_float3(rr.x, rr.y, rr.z);
                 }
                 const float dist = sqrtf(d2);
                 float w = dist * (wxy * __expf(-dz * dz / a.tSS_z));
                 w = (w * w) * (w * w);
-                wsum += w;
+#pragma unroll
+                for (int r = 0; r < ZB; ++r) wsum[r] += ok[r] ? w : 0.f;
                 const float wd = w * __builtin_amdgcn_rcpf(dist);  // Dk w = w - |g x d| * w / |d|
                 // [g . d < 0] as clamp(-s g . d, 0, 1) on the clamp bit of the last FMA: exact for |g . d| >= 1/s;
                 // below that the term it scales is under w (g . d)^2 / |d|^2
@@ -191,18 +213,21 @@ __device__ __forceinline__ void e3_window(const E3Args& a, const float4* m, size
                                 c3 = fmaf(gq.x, dy, -gq.y * dx);
                     const float cr2 = fmaf(c1, c1, fmaf(c2, c2, c3 * c3));
                     const float term = fmaf(-__builtin_amdgcn_sqrtf(cr2), wd, w);
-                    rg[q] = fmaf(neg, term, rg[q]);
+#pragma unroll
+                    for (int r = 0; r < ZB; ++r) rg[r * NBM + q] = fmaf(ok[r] ? neg : 0.f, term, rg[r * NBM + q]);
                 }
             }
         }
     }
 }
 
+template <int ZB>
 __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
+#pragma clang fp contract(off)
+    constexpr int NBM = kE3Batch / ZB;
     const size_t volM = static_cast<size_t>(a.S) * a.RM * a.CM, planeM = static_cast<size_t>(a.RM) * a.CM;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int Zb = e3_zb(a.Mz), Rb = kE3Waves / Zb;
-    const int ncx = (a.CM + 63) / 64, nG = (a.R + Rb - 1) / Rb, nz = (a.SM + Zb - 1) / Zb;
+    const int ncx = (a.CM + 63) / 64, nG = (a.R + kE3Waves - 1) / kE3Waves, nz = (a.SM + ZB - 1) / ZB;
     const size_t ntiles = static_cast<size_t>(ncx) * a.Mxy * nG * nz;
     for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         size_t rest = tile;
@@ -212,32 +237,44 @@ __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
         rest /= a.Mxy;
         const int g = static_cast<int>(rest % nG);
         const int sz = static_cast<int>(rest / nG);
-        const int cM = cx * 64 + lane, rM = p + a.Mxy * (Rb * g + wv / Zb), sM = sz * Zb + wv % Zb;
-        if (cM >= a.CM || rM >= a.RM || sM >= a.SM) continue;
-        const size_t t = static_cast<size_t>(sM) * planeM + static_cast<size_t>(rM) * a.CM + cM;
-        const bool in = sM >= a.margin_z && sM < a.SM - a.margin_z && rM >= a.margin && rM < a.RM - a.margin &&
-                        cM >= a.margin && cM < a.CM - a.margin;
-        float acc = a.f0 == 0 ? 0.f : a.out[t];
-        float mu = (a.mode == 1 && a.f0 > 0) ? a.mean[t] : 0.f;
-        if (in) {
-            // window coordinates exactly as the reference forms them (float32)
+        const int cM = cx * 64 + lane, rM = p + a.Mxy * (kE3Waves * g + wv), sM0 = sz * ZB;
+        if (cM >= a.CM || rM >= a.RM) continue;
+        const bool in_xy = rM >= a.margin && rM < a.RM - a.margin && cM >= a.margin && cM < a.CM - a.margin;
+        bool in[ZB];
+        float acc[ZB], mu[ZB], zc[ZB];
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < ZB; ++r) {
+            const int sM = sM0 + r;
+            const size_t t = static_cast<size_t>(sM) * planeM + static_cast<size_t>(rM) * a.CM + cM;
+            in[r] = sM < a.SM && in_xy && sM >= a.margin_z && sM < a.SM - a.margin_z;
+            any = any || in[r];
+            acc[r] = (a.f0 == 0 || sM >= a.SM) ? 0.f : a.out[t];
+            mu[r] = (a.mode == 1 && a.f0 > 0 && sM < a.SM) ? a.mean[t] : 0.f;
+            zc[r] = static_cast<float>(sM) / a.Mz;  // window coordinates exactly as the reference forms them
+        }
+        if (any) {
             const float xc = static_cast<float>(cM) / a.Mxy, yc = static_cast<float>(rM) / a.Mxy;
-            const float zc = static_cast<float>(sM) / a.Mz;
             const float ylim = static_cast<float>(a.RM / a.Mxy - 1), xlim = static_cast<float>(a.CM / a.Mxy - 1);
             const float zlim = static_cast<float>(a.SM / a.Mz - 1);
-            const size_t self = static_cast<size_t>(sM / a.Mz) * planeM + static_cast<size_t>(rM) * a.CM + cM;
-            // batches of equal size (at most kE3Batch frames)
-            const int nbat = (a.nf + kE3Batch - 1) / kE3Batch, bsz = (a.nf + nbat - 1) / nbat;
+            // intensity of the source slice, the same for every sub-slice
+            const size_t self = static_cast<size_t>(sM0 / a.Mz) * planeM + static_cast<size_t>(rM) * a.CM + cM;
+            // batches of equal size (at most NBM frames)
+            const int nbat = (a.nf + NBM - 1) / NBM, bsz = (a.nf + nbat - 1) / nbat;
             for (int fb = 0; fb < a.nf; fb += bsz) {
                 const int nb = min(bsz, a.nf - fb);
                 const float4* m = a.mag + fb * volM;
                 float rg[kE3Batch];
 #pragma unroll
                 for (int q = 0; q < kE3Batch; ++q) rg[q] = 0.f;
-                float wsum = 0.f;
+                float wsum[ZB];
+#pragma unroll
+                for (int r = 0; r < ZB; ++r) wsum[r] = 0.f;
                 switch (nb) {
 #define NPX_E3_CASE(n) \
-    case n: e3_window<n>(a, m, volM, planeM, xc, yc, zc, ylim, xlim, zlim, rg, wsum); break;
+    case n:            \
+        if constexpr (n <= NBM) e3_window<ZB, (n <= NBM ? n : 1)>(a, m, volM, planeM, xc, yc, zc, ylim, xlim, zlim, rg, wsum); \
+        break;
                     NPX_E3_CASE(1) NPX_E3_CASE(2) NPX_E3_CASE(3) NPX_E3_CASE(4)
                     NPX_E3_CASE(5) NPX_E3_CASE(6) NPX_E3_CASE(7) NPX_E3_CASE(8)
                     NPX_E3_CASE(9) NPX_E3_CASE(10) NPX_E3_CASE(11) NPX_E3_CASE(12)
@@ -246,26 +283,37 @@ __global__ __launch_bounds__(64 * kE3Waves) void e3_rgc3d(E3Args a) {
                     default: break;
                 }
 #pragma unroll
-                for (int q = 0; q < kE3Batch; ++q) {
-                    if (q >= nb) break;
-                    float rgc = rg[q] / wsum;
-                    if (rgc >= 0.f && a.pow_on) rgc = powf(rgc, a.S_pow);
-                    else if (rgc < 0.f) rgc = 0.f;
-                    if (a.iw) rgc *= m[q * volM + self].w;
-                    const float fn = static_cast<float>(a.f0 + fb + q + 1);
-                    if (a.mode == 0) {
-                        acc = acc + (rgc - acc) / fn;
-                    } else {
-                        const float delta = rgc - mu;
-                        mu += delta / fn;
-                        acc += delta * (rgc - mu);
+                for (int r = 0; r < ZB; ++r) {
+                    if (!in[r]) continue;
+#pragma unroll
+                    for (int q = 0; q < NBM; ++q) {
+                        if (q >= nb) break;
+                        float rgc = rg[r * NBM + q] / wsum[r];
+                        if (rgc >= 0.f && a.pow_on) rgc = powf(rgc, a.S_pow);
+                        else if (rgc < 0.f) rgc = 0.f;
+                        if (a.iw) rgc *= m[q * volM + self].w;
+                        const float fn = static_cast<float>(a.f0 + fb + q + 1);
+                        if (a.mode == 0) {
+                            acc[r] = acc[r] + (rgc - acc[r]) / fn;
+                        } else {
+                            const float delta = rgc - mu[r];
+                            mu[r] += delta / fn;
+                            acc[r] += delta * (rgc - mu[r]);
+                        }
                     }
                 }
             }
         }
-        if (a.mode == 1 && !a.last) a.mean[t] = mu;
-        if (a.mode == 1 && a.last) acc = sqrtf(acc / static_cast<float>(a.F));
-        a.out[t] = acc;
+#pragma unroll
+        for (int r = 0; r < ZB; ++r) {
+            const int sM = sM0 + r;
+            if (sM >= a.SM) continue;
+            const size_t t = static_cast<size_t>(sM) * planeM + static_cast<size_t>(rM) * a.CM + cM;
+            float v = acc[r];
+            if (a.mode == 1 && !a.last) a.mean[t] = mu[r];
+            if (a.mode == 1 && a.last) v = sqrtf(v / static_cast<float>(a.F));
+            a.out[t] = v;
+        }
     }
 }
 
@@ -346,8 +394,8 @@ int e3_chunk(int F, int S, int R, int C, int Mxy) {
 }
 
 size_t e3_tiles(const E3Args& a) {
-    const int Zb = e3_zb(a.Mz), Rb = kE3Waves / Zb;
-    return static_cast<size_t>((a.CM + 63) / 64) * a.Mxy * ((a.R + Rb - 1) / Rb) * ((a.SM + Zb - 1) / Zb);
+    const int Zb = e3_zb(a.Mz);
+    return static_cast<size_t>((a.CM + 63) / 64) * a.Mxy * ((a.R + kE3Waves - 1) / kE3Waves) * ((a.SM + Zb - 1) / Zb);
 }
 
 int e3_grid(size_t n) { return static_cast<int>(std::min<size_t>((n + 255) / 256, size_t(65536))); }
@@ -451,8 +499,12 @@ int npx_esrrf3d(const void* d_img, int dtype, int F, int S, int R, int C, int Mx
         hipLaunchKernelGGL(e3_magnify_xy,
                            dim3((a.CM + 63) / 64, (a.RM + kE3MagRows - 1) / kE3MagRows, std::min(S * a.nf, 65535)),
                            dim3(64, kE3MagRows), 0, st, a);
-        hipLaunchKernelGGL(e3_rgc3d, dim3(static_cast<int>(std::min<size_t>(e3_tiles(a), size_t(1) << 20))),
-                           dim3(64 * kE3Waves), 0, st, a);
+        const dim3 grid(static_cast<int>(std::min<size_t>(e3_tiles(a), size_t(1) << 20))), block(64 * kE3Waves);
+        switch (e3_zb(Mz)) {
+            case 2: hipLaunchKernelGGL(e3_rgc3d<2>, grid, block, 0, st, a); break;
+            case 4: hipLaunchKernelGGL(e3_rgc3d<4>, grid, block, 0, st, a); break;
+            default: hipLaunchKernelGGL(e3_rgc3d<1>, grid, block, 0, st, a); break;
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return host::hip_fail(e, "eSRRF 3D launch");
     }
