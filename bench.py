@@ -231,10 +231,13 @@ def esrrf3d_bench(reps, cpu_frames=1):
 def fht_bench(reps, cpu_frames=1, frames=20):
     """eSRRF with use_fht=True (Fourier interpolation branch) on the first `frames` frames of the srrf_256 scene
     (256 x 256 float32), magnification 5, radius 1.5, sensitivity 1, intensity weighting, per-frame maps: Fourier x5
-    of the image, Roberts cross, Fourier x10 of both gradients (2560 x 2560 float64 spectra per frame), RGC on those
-    arrays. HIP events on the launch stream around the whole chain. HBM lower bound of the Fourier steps: every
-    enlarged complex grid written once by the spectrum placement and read and written once by the inverse FFT
-    (48 B per enlarged point)."""
+    of the image, Roberts cross, Fourier x10 of both gradients (2560 x 2560 outputs per frame), RGC on those arrays.
+    HIP events on the launch stream around the whole chain. HBM bytes of the Fourier steps as they run, per enlarged
+    S x S grid of a 256 x 256 working tile: the Ww/2+1 = 129 non-zero spectrum columns (16 B per entry, S entries
+    each) written by the placement, read and written by the column FFT and read by the fused row inverse (4 passes),
+    the float32 grid written by the row inverse, read and written by the rescale and crop (12 B per point); the
+    padding and forward FFT (about 1.6 MB per grid) are left out. The dominant kernel, the row inverse, computes
+    ceil(M/2) float64 FFTs of length 256 per row in LDS and is not bound by these bytes."""
     import torch
     from nanopyx_amd import _hip, synthetic
     from nanopyx_amd.core.transform._le_esrrf import _esrrf_fht
@@ -258,8 +261,7 @@ def fht_bench(reps, cpu_frames=1, frames=20):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     U = N * H * M * W * M
-    big_points = N * ((256 * M) ** 2 + 2 * (256 * 2 * M) ** 2)
-    hbm_min = big_points * 48.0
+    hbm_min = float(N * sum(12 * S * S + 4 * 16 * 129 * S for S in (256 * M, 256 * 2 * M, 256 * 2 * M)))
     res = {"workload": f"fht_256: {N} frames x {H}x{W} float32, eSRRF use_fht M{M} R{R} S1, per-frame maps "
                        f"{H * M}x{W * M}",
            "ms_per_call": round(ms, 3), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mpx·frames/s",

@@ -8,11 +8,14 @@
 // real-to-complex 2D FFT, then the reference's irfft2 of the M-times larger half spectrum whose row quadrants sit at
 // its top and bottom. Only the first Ww/2+1 columns of that spectrum are non-zero, so the inverse runs as: those
 // columns, stored column-major with the quadrants placed (fht_embed_cols), through a batched complex inverse FFT along
-// the rows; a transpose into the enlarged half spectrum, zero beyond those columns, with the imaginary parts of the
-// DC and Nyquist columns set to 0 as irfft ignores them (fht_to_half); a batched complex-to-real inverse FFT along
-// the columns. Then per-frame min/max of the float32 result (fht_minmax), and one pass that rescales to the tile's
-// input range, crops the padding and writes (or, for tiles, adds the Hann-weighted tile into the output and its
-// window into the weight; fht_finish); tiled outputs are then divided by the summed window (fht_normalise).
+// the rows (one column of the spectrum each); then along the columns, for a power-of-two Ww >= 2 and M >= 2, one
+// kernel (fht_rows) evaluates each output row as M polyphase inverse DFTs of length Ww, in pairs as complex FFTs in
+// LDS, and writes the float32 result with its per-frame min / max; for other shapes a transpose into the enlarged
+// half spectrum, zero beyond those columns, with the imaginary parts of the DC and Nyquist columns set to 0 as irfft
+// ignores them (fht_to_half), a batched complex-to-real inverse FFT and a min / max pass (fht_minmax). Then one pass
+// rescales to the tile's input range, crops the padding and writes (or, for tiles, adds the Hann-weighted tile into
+// the output and its window into the weight; fht_finish); tiled outputs are then divided by the summed window
+// (fht_normalise).
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
@@ -62,6 +65,7 @@ struct FhtTile {
     double2* cols;       // (B, Ww/2+1, HwM): the non-zero columns of the enlarged spectrum, column-major
     double2* half;       // (B, HwM, WwM/2+1): enlarged half spectrum after the row pass
     double* real;        // (B, HwM, WwM): the inverse
+    float* realf;        // (B, HwM, WwM) float32 of the inverse / (HwM WwM), in place of `real` (fht_rows), or null
     float* mm;           // (B, 4): input min, input max, output min, output max (as ordered ints for atomics)
     const float* win_r;  // [th*M] window rows (tiled)
     const float* win_c;  // [tw*M]
@@ -152,80 +156,350 @@ __global__ __launch_bounds__(256) void fht_embed_cols(FhtTile a) {
 }
 
 // transposed into the enlarged half spectrum (HwM x WwM/2+1): columns beyond Ww/2 zero, imaginary parts of the DC
-// and (even WwM) Nyquist columns zero
+// and (even WwM) Nyquist columns zero. A block covers kHalfRows rows of one frame (grid: frames x row groups): the
+// column-major source is read kHalfRows contiguous entries per column into LDS, kHalfCols columns at a time, and
+// each row is written as contiguous spans; past the non-zero columns only zeros are written.
+constexpr int kHalfRows = 16, kHalfCols = 64;
 __global__ __launch_bounds__(256) void fht_to_half(FhtTile a) {
     const int HM = a.Hw * a.M, WM = a.Ww * a.M, hc = a.Ww / 2 + 1, wh = WM / 2 + 1;
-    const size_t plane = static_cast<size_t>(HM) * wh, n = plane * a.B;
-    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
-         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const size_t fb = t / plane, p = t % plane;
-        const int y = static_cast<int>(p / wh), c = static_cast<int>(p % wh);
-        double2 v = make_double2(0.0, 0.0);
-        if (c < hc) v = a.cols[fb * static_cast<size_t>(hc) * HM + static_cast<size_t>(c) * HM + y];
-        if (c == 0 || (WM % 2 == 0 && c == WM / 2)) v.y = 0.0;
-        a.half[t] = v;
+    const int ngrp = (HM + kHalfRows - 1) / kHalfRows;
+    const int fb = blockIdx.x / ngrp, y0 = (blockIdx.x % ngrp) * kHalfRows, nr = min(kHalfRows, HM - y0);
+    const double2* src = a.cols + static_cast<size_t>(fb) * hc * HM + y0;
+    double2* dst = a.half + (static_cast<size_t>(fb) * HM + y0) * wh;
+    __shared__ double2 s[kHalfCols][kHalfRows + 1];
+    for (int c0 = 0; c0 < wh; c0 += kHalfCols) {
+        const bool filled = c0 < hc;  // block-uniform
+        if (filled) {
+            __syncthreads();  // the previous span's reads of s are done
+            for (int t = threadIdx.x; t < kHalfCols * kHalfRows; t += blockDim.x) {
+                const int c = t / kHalfRows, r = t % kHalfRows;
+                s[c][r] = (c0 + c < hc && r < nr) ? src[static_cast<size_t>(c0 + c) * HM + r] : make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < kHalfRows * kHalfCols; t += blockDim.x) {
+            const int r = t / kHalfCols, c = t % kHalfCols, C = c0 + c;
+            if (r < nr && C < wh) {
+                double2 v = filled ? s[c][r] : make_double2(0.0, 0.0);
+                if (C == 0 || (WM % 2 == 0 && C == WM / 2)) v.y = 0.0;
+                dst[static_cast<size_t>(r) * wh + C] = v;
+            }
+        }
     }
 }
 
-// float32 result of the inverse (real part / (HwM WwM)) and its per-frame min / max
+// Row inverse of the enlarged half spectrum without forming it (power-of-two Ww >= 2, M >= 2). Row y holds X[k] for
+// k <= K = Ww/2 (the column pass's output) and zeros up to WwM/2, so irfft's unnormalised value at n = M m + r is
+//   x[M m + r] = Re X[0] + 2 sum_{k=1..K} Re(X[k] e^{2 pi i k (M m + r) / (Ww M)}),
+// a length-Ww inverse DFT over m of G_r: G_r[0] = Re X[0], G_r[k] = Y_r[k] and G_r[Ww-k] = conj(Y_r[k]) for 0 < k < K,
+// G_r[K] = 2 Re Y_r[K], with Y_r[k] = X[k] e^{2 pi i k r / (Ww M)}. Phases r = 2p and 2p+1 share one complex FFT
+// (G_2p + i G_2p+1, real outputs), so a row is ceil(M/2) complex FFTs of length Ww: mixed radix-4/2 Stockham in LDS
+// (each stage reads its butterflies into registers, then writes them back in place). The result is written as the
+// float32 of x / (HwM WwM) with its per-frame min / max, replacing the half spectrum, the complex-to-real FFT and a
+// separate min / max pass (the float64 grid is never written). A block covers kRowsPerBlock rows of one frame, kRowPair
+// rows at a time; twiddles e^{2 pi i t / Ww} and e^{2 pi i j / (Ww M)} (split as j = 64 h + l) sit in LDS.
+constexpr int kRowsPerBlock = 8;
+constexpr int kRowLdsPoints = 3072;  // double2 of FFT data per block (48 KiB); tables and X come on top
+constexpr int kRowLdsLimit = 64 * 1024;
+constexpr int kRowThreads = 256;
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+struct RowShape {
+    int rb;        // rows per pass (1 or 2)
+    int npt;       // FFT entries per pass: rb * ceil(M/2) FFTs of Ww points, Ww + 1 apart
+    int nhi;       // entries of the coarse phase-twiddle table
+    size_t bytes;  // dynamic LDS
+};
+
+__host__ __device__ inline RowShape row_shape(int Ww, int M, int rb) {
+    RowShape r{};
+    const int K = Ww / 2, P = (M + 1) / 2;
+    r.rb = rb;
+    r.npt = rb * P * (Ww + 1);  // one pad entry per FFT staggers the pairs' LDS banks
+    r.nhi = (K * (M - 1)) / 64 + 1;
+    r.bytes = sizeof(double2) * (static_cast<size_t>(r.npt) + Ww + 64 + r.nhi + static_cast<size_t>(rb) * (K + 1));
+    return r;
+}
+
+__global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
+    extern __shared__ double2 lds[];
+    const int HM = a.Hw * a.M, Ww = a.Ww, M = a.M, N = Ww * M, K = Ww / 2, hc = K + 1, P = (M + 1) / 2;
+    const RowShape sh = row_shape(Ww, M, rb);
+    double2* F = lds;                  // [rb][P][Ww + 1]
+    double2* tw = F + sh.npt;          // e^{2 pi i t / Ww}, t < Ww
+    double2* tlo = tw + Ww;            // e^{2 pi i l / N}, l < 64
+    double2* thi = tlo + 64;           // e^{2 pi i 64 h / N}, h < nhi
+    double2* X = thi + sh.nhi;         // [rb][K + 1]
+    const int ngrp = (HM + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int fb = blockIdx.x / ngrp, ybeg = (blockIdx.x % ngrp) * kRowsPerBlock;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < Ww + 64 + sh.nhi; t += kRowThreads) {
+        double s, c;
+        if (t < Ww) sincospi(2.0 * t / Ww, &s, &c);
+        else if (t < Ww + 64) sincospi(2.0 * (t - Ww) / N, &s, &c);
+        else sincospi(2.0 * 64.0 * (t - Ww - 64) / N, &s, &c);
+        tw[t] = make_double2(c, s);
+    }
+    const int lgW = __builtin_ctz(Ww);
+    const double norm = 1.0 / (static_cast<double>(HM) * N);
+    const double2* src = a.cols + static_cast<size_t>(fb) * hc * HM;
+    float lo = INFINITY, hi = -INFINITY;
+    const int nf = rb * P;
+    for (int y0 = ybeg; y0 < ybeg + kRowsPerBlock; y0 += rb) {
+        __syncthreads();  // tables written; the previous pass's reads of F and X are done
+        for (int t = tid; t < rb * hc; t += kRowThreads) {
+            const int rr = t % rb, c = t / rb;
+            X[rr * hc + c] = y0 + rr < HM ? src[static_cast<size_t>(c) * HM + y0 + rr] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        // Z = G_2p + i G_2p+1 for every (row, pair): one thread per kk < K writes Z[kk] and Z[Ww - kk], the first nf
+        // threads of a second step write Z[K]
+        const int lgK = lgW - 1;
+        for (int t = tid; t < nf << lgK; t += kRowThreads) {
+            const int kk = t & (K - 1), f = t >> lgK, rr = f / P, p = f - rr * P, r1 = 2 * p;
+            const double2* x = X + rr * hc;
+            double2* z = F + f * (Ww + 1);
+            const bool two = r1 + 1 < M;
+            if (kk == 0) {
+                z[0] = make_double2(x[0].x, two ? x[0].x : 0.0);
+            } else {
+                const int j1 = kk * r1, j2 = j1 + kk;
+                const double2 y1 = cmul(x[kk], cmul(thi[j1 >> 6], tlo[j1 & 63]));
+                const double2 y2 = two ? cmul(x[kk], cmul(thi[j2 >> 6], tlo[j2 & 63])) : make_double2(0.0, 0.0);
+                z[kk] = make_double2(y1.x - y2.y, y1.y + y2.x);         // y1 + i y2
+                z[Ww - kk] = make_double2(y1.x + y2.y, -y1.y + y2.x);   // conj(y1) + i conj(y2)
+            }
+        }
+        for (int f = tid; f < nf; f += kRowThreads) {
+            const int rr = f / P, p = f - rr * P, r1 = 2 * p, j1 = K * r1, j2 = j1 + K;
+            const double2 xk = X[rr * hc + K];
+            const double2 y1 = cmul(xk, cmul(thi[j1 >> 6], tlo[j1 & 63]));
+            const double2 y2 = r1 + 1 < M ? cmul(xk, cmul(thi[j2 >> 6], tlo[j2 & 63])) : make_double2(0.0, 0.0);
+            F[f * (Ww + 1) + K] = make_double2(2.0 * y1.x, 2.0 * y2.x);
+        }
+        // Stockham stages: Ns points already combined; radix 4 while it fits, then one radix-2 stage
+        int Ns = 1, lgNs = 0;
+        while (Ns < Ww) {
+            __syncthreads();
+            if (Ns * 4 <= Ww) {
+                const int lgnb = lgW - 2, nb = 1 << lgnb, total = nf << lgnb, step = Ww / (Ns * 4);
+                double2 v[3][4];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int b = tid + u * kRowThreads;
+                    if (b < total) {
+                        const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
+                        const double2* in = F + f * (Ww + 1) + j;
+                        const double2 v0 = in[0], v1 = cmul(in[nb], tw[k * step]),
+                                      v2 = cmul(in[2 * nb], tw[2 * k * step]), v3 = cmul(in[3 * nb], tw[3 * k * step]);
+                        const double2 a0 = make_double2(v0.x + v2.x, v0.y + v2.y);
+                        const double2 a1 = make_double2(v0.x - v2.x, v0.y - v2.y);
+                        const double2 b0 = make_double2(v1.x + v3.x, v1.y + v3.y);
+                        const double2 b1 = make_double2(-(v1.y - v3.y), v1.x - v3.x);  // i (v1 - v3)
+                        v[u][0] = make_double2(a0.x + b0.x, a0.y + b0.y);
+                        v[u][1] = make_double2(a1.x + b1.x, a1.y + b1.y);
+                        v[u][2] = make_double2(a0.x - b0.x, a0.y - b0.y);
+                        v[u][3] = make_double2(a1.x - b1.x, a1.y - b1.y);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int b = tid + u * kRowThreads;
+                    if (b < total) {
+                        const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
+                        double2* out = F + f * (Ww + 1) + ((j >> lgNs) << (lgNs + 2)) + k;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) out[r * Ns] = v[u][r];
+                    }
+                }
+                Ns *= 4;
+                lgNs += 2;
+            } else {
+                const int lgnb = lgW - 1, nb = 1 << lgnb, total = nf << lgnb, step = Ww / (Ns * 2);
+                double2 v[6][2];
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const int b = tid + u * kRowThreads;
+                    if (b < total) {
+                        const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
+                        const double2* in = F + f * (Ww + 1) + j;
+                        const double2 v0 = in[0], v1 = cmul(in[nb], tw[k * step]);
+                        v[u][0] = make_double2(v0.x + v1.x, v0.y + v1.y);
+                        v[u][1] = make_double2(v0.x - v1.x, v0.y - v1.y);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const int b = tid + u * kRowThreads;
+                    if (b < total) {
+                        const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
+                        double2* out = F + f * (Ww + 1) + ((j >> lgNs) << (lgNs + 1)) + k;
+                        out[0] = v[u][0];
+                        out[Ns] = v[u][1];
+                    }
+                }
+                Ns *= 2;
+                lgNs += 1;
+            }
+        }
+        __syncthreads();
+        // pair p at m holds phases 2p (real part) and 2p + 1 (imaginary part): outputs n = M m + 2p and n + 1.
+        // m = t / P through a float reciprocal: t < Ww P <= 2^12, so the error stays far below 1 / (2P).
+        const float invP = 1.f / P;
+        for (int rr = 0; rr < rb; ++rr) {
+            const int y = y0 + rr;
+            if (y >= HM) break;
+            float* dst = a.realf + (static_cast<size_t>(fb) * HM + y) * N;
+            const double2* fr = F + rr * P * (Ww + 1);
+            for (int t = tid; t < Ww * P; t += kRowThreads) {
+                const int m = static_cast<int>((t + 0.5f) * invP), p = t - m * P, n = M * m + 2 * p;
+                const double2 z = fr[p * (Ww + 1) + m];
+                const float v0 = static_cast<float>(z.x * norm);
+                dst[n] = v0;
+                lo = fminf(lo, v0);
+                hi = fmaxf(hi, v0);
+                if (2 * p + 1 < M) {
+                    const float v1 = static_cast<float>(z.y * norm);
+                    dst[n + 1] = v1;
+                    lo = fminf(lo, v1);
+                    hi = fmaxf(hi, v1);
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    __shared__ float s_lo[kRowThreads / 64], s_hi[kRowThreads / 64];
+    if (tid % 64 == 0) {
+        s_lo[tid / 64] = lo;
+        s_hi[tid / 64] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < kRowThreads / 64; ++k) {
+            lo = fminf(lo, s_lo[k]);
+            hi = fmaxf(hi, s_hi[k]);
+        }
+        int* m = reinterpret_cast<int*>(a.mm) + 4 * fb;
+        atomicMin(m + 2, f2o(lo));
+        atomicMax(m + 3, f2o(hi));
+    }
+}
+
+__device__ inline void mm_update(double d, double norm, float& lo, float& hi) {
+    const float v = static_cast<float>(d * norm);
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+}
+
+// float32 result of the inverse (real part / (HwM WwM)) and its per-frame min / max (grid: row = frame). Even
+// planes are read as 16-byte pairs, four loads in flight per thread; min and max are exact, so the reduction order
+// does not change them.
 __global__ __launch_bounds__(256) void fht_minmax(FhtTile a) {
     const int HM = a.Hw * a.M, WM = a.Ww * a.M;
     const size_t plane = static_cast<size_t>(HM) * WM;
     const int fb = blockIdx.y;
     const double norm = 1.0 / (static_cast<double>(HM) * WM);
+    const double* src = a.real + fb * plane;
     float lo = INFINITY, hi = -INFINITY;
-    for (size_t p = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; p < plane;
-         p += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const float v = static_cast<float>(a.real[fb * plane + p] * norm);
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-    }
-    __shared__ float s_lo[256], s_hi[256];
-    s_lo[threadIdx.x] = lo;
-    s_hi[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_lo[threadIdx.x] = fminf(s_lo[threadIdx.x], s_lo[threadIdx.x + s]);
-            s_hi[threadIdx.x] = fmaxf(s_hi[threadIdx.x], s_hi[threadIdx.x + s]);
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    size_t p = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (plane % 2 == 0) {
+        const double2* s2 = reinterpret_cast<const double2*>(src);
+        const size_t n2 = plane / 2;
+        for (; p + 3 * stride < n2; p += 4 * stride) {
+            double2 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = s2[p + k * stride];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                mm_update(v[k].x, norm, lo, hi);
+                mm_update(v[k].y, norm, lo, hi);
+            }
         }
-        __syncthreads();
+        for (; p < n2; p += stride) {
+            const double2 v = s2[p];
+            mm_update(v.x, norm, lo, hi);
+            mm_update(v.y, norm, lo, hi);
+        }
+    } else {
+        for (; p < plane; p += stride) mm_update(src[p], norm, lo, hi);
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    __shared__ float s_lo[4], s_hi[4];
+    const int wv = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int k = 1; k < static_cast<int>(blockDim.x / 64); ++k) {
+            lo = fminf(lo, s_lo[k]);
+            hi = fmaxf(hi, s_hi[k]);
+        }
         int* m = reinterpret_cast<int*>(a.mm) + 4 * fb;
-        atomicMin(m + 2, f2o(s_lo[0]));
-        atomicMax(m + 3, f2o(s_hi[0]));
+        atomicMin(m + 2, f2o(lo));
+        atomicMax(m + 3, f2o(hi));
     }
 }
 
-// rescale to the input range (float32 arithmetic, interpolation.py:90-97), crop, write or blend
+// rescale to the input range in float32 as three rounded steps (interpolation.py:90-97), crop, write or blend. A
+// block covers one output row of one frame (grid: frames x tile rows); four loads in flight per thread.
 __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
+#pragma clang fp contract(off)
     const int HM = a.Hw * a.M, WM = a.Ww * a.M;
     const int oh = a.th * a.M, ow = a.tw * a.M;
-    const size_t plane = static_cast<size_t>(HM) * WM, tplane = static_cast<size_t>(oh) * ow, n = tplane * a.B;
+    const int fb = blockIdx.x / oh, y = blockIdx.x % oh;
     const double norm = 1.0 / (static_cast<double>(HM) * WM);
-    const size_t oplane = static_cast<size_t>(a.h) * a.M * a.w * a.M;
-    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
-         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const size_t fb = t / tplane, p = t % tplane;
-        const int y = static_cast<int>(p / ow), x = static_cast<int>(p % ow);
-        const int* m = reinterpret_cast<const int*>(a.mm) + 4 * fb;
-        const float lo = o2f(m[0]), hi = o2f(m[1]), omin = o2f(m[2]), omax = o2f(m[3]);
-        float v = static_cast<float>(a.real[fb * plane + static_cast<size_t>(y + a.yroi) * WM + (x + a.xroi)] * norm);
-        if (omax != omin) {
-            const float scale = static_cast<float>((static_cast<double>(hi) - lo) / (static_cast<double>(omax) - omin));
-            v = v - omin;
-            v = v * scale;
-            v = v + lo;
+    const int* m = reinterpret_cast<const int*>(a.mm) + 4 * fb;
+    const float lo = o2f(m[0]), hi = o2f(m[1]), omin = o2f(m[2]), omax = o2f(m[3]);
+    const bool rescale = omax != omin;
+    const float scale = rescale ? static_cast<float>((static_cast<double>(hi) - lo) / (static_cast<double>(omax) - omin))
+                                : 1.f;
+    const size_t row = (static_cast<size_t>(fb) * HM + y + a.yroi) * WM + a.xroi;
+    const double* src = a.real + row;
+    const float* srcf = a.realf ? a.realf + row : nullptr;  // float32 grid written by fht_rows
+    const size_t wo = static_cast<size_t>(a.w) * a.M;
+    const size_t o0 = static_cast<size_t>(a.f0 + fb) * a.h * a.M * wo + (static_cast<size_t>(a.r0) * a.M + y) * wo +
+                      static_cast<size_t>(a.c0) * a.M;
+    float* out = a.out + o0;
+    const float wr = a.tiled ? a.win_r[y] : 0.f;
+    constexpr int U = 4;
+    for (int x0 = threadIdx.x; x0 < ow; x0 += U * blockDim.x) {
+        float d[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int x = x0 + k * static_cast<int>(blockDim.x);
+            d[k] = x >= ow ? 0.f : (srcf ? srcf[x] : static_cast<float>(src[x] * norm));
         }
-        const size_t o = static_cast<size_t>(a.f0 + fb) * oplane +
-                         static_cast<size_t>(a.r0 * a.M + y) * (a.w * a.M) + (a.c0 * a.M + x);
-        if (a.tiled) {
-            const float wv = a.win_r[y] * a.win_c[x];
-            a.out[o] += v * wv;
-            a.weight[o] += wv;
-        } else {
-            a.out[o] = v;
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int x = x0 + k * static_cast<int>(blockDim.x);
+            if (x >= ow) break;
+            float v = d[k];
+            if (rescale) {
+                v = v - omin;
+                v = v * scale;
+                v = v + lo;
+            }
+            if (a.tiled) {
+                const float wv = wr * a.win_c[x];
+                out[x] += v * wv;
+                a.weight[o0 + x] += wv;
+            } else {
+                out[x] = v;
+            }
         }
     }
 }
@@ -348,6 +622,17 @@ Layout layout_of(int N, int h, int w, int M, int th, int tw, bool mirror, bool t
 }
 
 bool tiled_of(int h, int w, int tile) { return h > tile || w > tile; }
+
+// Rows per pass of fht_rows for this width and magnification (0: the shape takes the complex-to-real FFT path):
+// a power-of-two Ww >= 2, M >= 2, and the pass's FFT points and tables within the kernel's register and LDS budget.
+int rows_pass(int Ww, int M) {
+    if (M < 2 || Ww < 2 || (Ww & (Ww - 1)) != 0) return 0;
+    for (int rb = 2; rb >= 1; --rb) {
+        const RowShape r = row_shape(Ww, M, rb);
+        if (r.npt <= kRowLdsPoints && r.bytes <= static_cast<size_t>(kRowLdsLimit)) return rb;
+    }
+    return 0;
+}
 
 }  // namespace
 
@@ -491,21 +776,34 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
                                   reinterpret_cast<hipfftDoubleComplex*>(a.spec)) != HIPFFT_SUCCESS)
                     return host::fail(NPX_ERR_HIP, "hipfftExecD2Z");
                 const size_t ncol = static_cast<size_t>(a.Ww / 2 + 1) * a.Hw * M * a.B;
-                const size_t nhalf = static_cast<size_t>(a.Hw) * M * (a.Ww * M / 2 + 1) * a.B;
                 hipLaunchKernelGGL(fht_embed_cols, dim3(grid_of(ncol)), dim3(256), 0, st, a);
                 if (hipfftSetStream(pl.cols, st) != HIPFFT_SUCCESS ||
                     hipfftExecZ2Z(pl.cols, reinterpret_cast<hipfftDoubleComplex*>(a.cols),
                                   reinterpret_cast<hipfftDoubleComplex*>(a.cols), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
                     return host::fail(NPX_ERR_HIP, "hipfftExecZ2Z (columns)");
-                hipLaunchKernelGGL(fht_to_half, dim3(grid_of(nhalf)), dim3(256), 0, st, a);
+                const int rb = rows_pass(a.Ww, M);
+                if (rb > 0) {
+                    a.realf = reinterpret_cast<float*>(a.real);
+                    const int ngrp = (a.Hw * M + kRowsPerBlock - 1) / kRowsPerBlock;
+                    hipLaunchKernelGGL(fht_rows, dim3(ngrp * a.B), dim3(kRowThreads), row_shape(a.Ww, M, rb).bytes, st,
+                                       a, rb);
+                    hipLaunchKernelGGL(fht_finish, dim3(a.th * M * a.B), dim3(256), 0, st, a);
+                    hipError_t e = hipGetLastError();
+                    if (e != hipSuccess) return host::hip_fail(e, "Fourier interpolation launch");
+                    continue;
+                }
+                a.realf = nullptr;
+                const int ngrp = (a.Hw * M + kHalfRows - 1) / kHalfRows;
+                hipLaunchKernelGGL(fht_to_half, dim3(ngrp * a.B), dim3(256), 0, st, a);
                 if (hipfftSetStream(pl.rows, st) != HIPFFT_SUCCESS ||
                     hipfftExecZ2D(pl.rows, reinterpret_cast<hipfftDoubleComplex*>(a.half),
                                   reinterpret_cast<hipfftDoubleReal*>(a.real)) != HIPFFT_SUCCESS)
                     return host::fail(NPX_ERR_HIP, "hipfftExecZ2D (rows)");
-                hipLaunchKernelGGL(fht_minmax, dim3(std::min<size_t>((bigp + 255) / 256, 1024), a.B), dim3(256), 0,
-                                   st, a);
-                const size_t nt = static_cast<size_t>(a.th) * M * a.tw * M * a.B;
-                hipLaunchKernelGGL(fht_finish, dim3(grid_of(nt)), dim3(256), 0, st, a);
+                // about 8192 blocks over the batch, each sweeping 1024 pairs per step
+                const size_t mmx = std::min<size_t>((bigp / 2 + 1023) / 1024, std::max(1, 8192 / a.B));
+                hipLaunchKernelGGL(fht_minmax, dim3(static_cast<unsigned>(std::max<size_t>(1, mmx)), a.B), dim3(256),
+                                   0, st, a);
+                hipLaunchKernelGGL(fht_finish, dim3(a.th * M * a.B), dim3(256), 0, st, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return host::hip_fail(e, "Fourier interpolation launch");
             }

@@ -36,6 +36,9 @@ def _report(tag, x, ref):
     ((2, 24, 24), 10, True, 0, 16),     # x2M of M = 5 gradients
     ((2, 150, 170), 2, True, 64, 8),    # Hann-blended tiles
     ((1, 300, 280), 2, True, 0, 16),    # default 256 tiles
+    ((2, 24, 32), 4, False, 0, 16),     # no padding, power-of-two width: the fused row inverse
+    ((2, 20, 20), 1, True, 0, 16),      # M = 1: the complex-to-real FFT path
+    ((1, 260, 400), 5, True, 512, 16),  # 512-wide working grid: one row per pass of the fused row inverse
 ])
 def test_fht_matches_oracle(torch, shape, M, mirror, tile, overlap):
     from nanopyx_amd import _hip
