@@ -97,44 +97,50 @@ __global__ __launch_bounds__(256) void fht_init_mm(FhtTile a) {
     }
 }
 
-// mirror padding (interpolation.py:22-55) into float64, and the tile's input min / max
+// mirror padding (interpolation.py:22-55) into float64, and the tile's input min / max. A block covers kPadRows rows
+// of one frame (grid: row groups x frames): each thread finds its columns' sources once and reuses them per row.
+constexpr int kPadRows = 8;
 __global__ __launch_bounds__(256) void fht_pad(FhtTile a) {
-    const size_t plane = static_cast<size_t>(a.Hw) * a.Ww, n = plane * a.B;
-    const int fb = blockIdx.y;  // one frame per grid row keeps the min/max reduction per frame
+    const int fb = blockIdx.y, y0 = blockIdx.x * kPadRows, y1 = min(y0 + kPadRows, a.Hw);
+    const float* in = a.in + static_cast<size_t>(a.f0 + fb) * a.h * a.w;
+    double* out = a.pad + static_cast<size_t>(fb) * a.Hw * a.Ww;
     float lo = INFINITY, hi = -INFINITY;
-    for (size_t p = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; p < plane;
-         p += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const int y = static_cast<int>(p / a.Ww), x = static_cast<int>(p % a.Ww);
-        int sy = y, sx = x;
+    for (int x = threadIdx.x; x < a.Ww; x += blockDim.x) {
+        int sx = x;
         if (a.mirror) {
-            const int Y = y + a.off_h, X = x + a.off_w;  // position in the tiles_h x tiles_w mirror grid
-            const int ty = Y / a.th, tx = X / a.tw, qy = Y % a.th, qx = X % a.tw;
-            sy = fmod2(ty - a.mid_h) == 0 ? qy : a.th - 1 - qy;
+            const int X = x + a.off_w, tx = X / a.tw, qx = X - tx * a.tw;  // position in the mirror grid
             sx = fmod2(tx - a.mid_w) == 0 ? qx : a.tw - 1 - qx;
         }
-        const float v = a.in[static_cast<size_t>(a.f0 + fb) * a.h * a.w + static_cast<size_t>(a.r0 + sy) * a.w +
-                             (a.c0 + sx)];
-        a.pad[fb * plane + p] = static_cast<double>(v);
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-    }
-    (void)n;
-    // block reduction then one atomic per block
-    __shared__ float s_lo[256], s_hi[256];
-    s_lo[threadIdx.x] = lo;
-    s_hi[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_lo[threadIdx.x] = fminf(s_lo[threadIdx.x], s_lo[threadIdx.x + s]);
-            s_hi[threadIdx.x] = fmaxf(s_hi[threadIdx.x], s_hi[threadIdx.x + s]);
+        for (int y = y0; y < y1; ++y) {
+            int sy = y;
+            if (a.mirror) {
+                const int Y = y + a.off_h, ty = Y / a.th, qy = Y - ty * a.th;
+                sy = fmod2(ty - a.mid_h) == 0 ? qy : a.th - 1 - qy;
+            }
+            const float v = in[static_cast<size_t>(a.r0 + sy) * a.w + (a.c0 + sx)];
+            out[static_cast<size_t>(y) * a.Ww + x] = static_cast<double>(v);
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
         }
-        __syncthreads();
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    __shared__ float s_lo[4], s_hi[4];
+    if (threadIdx.x % 64 == 0) {
+        s_lo[threadIdx.x / 64] = lo;
+        s_hi[threadIdx.x / 64] = hi;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int k = 1; k < static_cast<int>(blockDim.x / 64); ++k) {
+            lo = fminf(lo, s_lo[k]);
+            hi = fmaxf(hi, s_hi[k]);
+        }
         int* m = reinterpret_cast<int*>(a.mm) + 4 * fb;
-        atomicMin(m + 0, f2o(s_lo[0]));
-        atomicMax(m + 1, f2o(s_hi[0]));
+        atomicMin(m + 0, f2o(lo));
+        atomicMax(m + 1, f2o(hi));
     }
 }
 
@@ -766,11 +772,9 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
                 Plans pl;
                 int rc = plans_for(a.Hw, a.Ww, M, a.B, &pl);
                 if (rc != NPX_OK) return rc;
-                const size_t plane = static_cast<size_t>(a.Hw) * a.Ww;
                 const size_t bigp = static_cast<size_t>(a.Hw) * M * a.Ww * M;
                 hipLaunchKernelGGL(fht_init_mm, dim3((a.B + 255) / 256), dim3(256), 0, st, a);
-                hipLaunchKernelGGL(fht_pad, dim3(std::min<size_t>((plane + 255) / 256, 1024), a.B), dim3(256), 0, st,
-                                   a);
+                hipLaunchKernelGGL(fht_pad, dim3((a.Hw + kPadRows - 1) / kPadRows, a.B), dim3(256), 0, st, a);
                 if (hipfftSetStream(pl.fwd, st) != HIPFFT_SUCCESS ||
                     hipfftExecD2Z(pl.fwd, reinterpret_cast<hipfftDoubleReal*>(a.pad),
                                   reinterpret_cast<hipfftDoubleComplex*>(a.spec)) != HIPFFT_SUCCESS)
