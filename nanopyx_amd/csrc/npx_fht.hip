@@ -215,16 +215,21 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 
 struct RowShape {
     int rb;        // rows per pass (1 or 2)
-    int npt;       // FFT entries per pass: rb * ceil(M/2) FFTs of Ww points, Ww + 1 apart
+    int npt;       // FFT entries per pass: rb * ceil(M/2) FFTs of Ww points, fft_stride(Ww) apart
     int nhi;       // entries of the coarse phase-twiddle table
     size_t bytes;  // dynamic LDS
 };
+
+// LDS entries of one FFT of Ww points: entry e sits at e + e / 16 (a 16-byte skew every 16 entries spreads the
+// strided stage writes over the banks), plus one entry that staggers consecutive FFTs.
+__host__ __device__ inline int fft_stride(int Ww) { return Ww + (Ww >> 4) + 1; }
+__device__ inline int sk(int e) { return e + (e >> 4); }
 
 __host__ __device__ inline RowShape row_shape(int Ww, int M, int rb) {
     RowShape r{};
     const int K = Ww / 2, P = (M + 1) / 2;
     r.rb = rb;
-    r.npt = rb * P * (Ww + 1);  // one pad entry per FFT staggers the pairs' LDS banks
+    r.npt = rb * P * fft_stride(Ww);
     r.nhi = (K * (M - 1)) / 64 + 1;
     r.bytes = sizeof(double2) * (static_cast<size_t>(r.npt) + Ww + 64 + r.nhi + static_cast<size_t>(rb) * (K + 1));
     return r;
@@ -234,7 +239,8 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
     extern __shared__ double2 lds[];
     const int HM = a.Hw * a.M, Ww = a.Ww, M = a.M, N = Ww * M, K = Ww / 2, hc = K + 1, P = (M + 1) / 2;
     const RowShape sh = row_shape(Ww, M, rb);
-    double2* F = lds;                  // [rb][P][Ww + 1]
+    const int FS = fft_stride(Ww);
+    double2* F = lds;                  // [rb][P][fft_stride(Ww)], skewed entries
     double2* tw = F + sh.npt;          // e^{2 pi i t / Ww}, t < Ww
     double2* tlo = tw + Ww;            // e^{2 pi i l / N}, l < 64
     double2* thi = tlo + 64;           // e^{2 pi i 64 h / N}, h < nhi
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
         for (int t = tid; t < nf << lgK; t += kRowThreads) {
             const int kk = t & (K - 1), f = t >> lgK, rr = f / P, p = f - rr * P, r1 = 2 * p;
             const double2* x = X + rr * hc;
-            double2* z = F + f * (Ww + 1);
+            double2* z = F + f * FS;
             const bool two = r1 + 1 < M;
             if (kk == 0) {
                 z[0] = make_double2(x[0].x, two ? x[0].x : 0.0);
@@ -275,8 +281,8 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                 const int j1 = kk * r1, j2 = j1 + kk;
                 const double2 y1 = cmul(x[kk], cmul(thi[j1 >> 6], tlo[j1 & 63]));
                 const double2 y2 = two ? cmul(x[kk], cmul(thi[j2 >> 6], tlo[j2 & 63])) : make_double2(0.0, 0.0);
-                z[kk] = make_double2(y1.x - y2.y, y1.y + y2.x);         // y1 + i y2
-                z[Ww - kk] = make_double2(y1.x + y2.y, -y1.y + y2.x);   // conj(y1) + i conj(y2)
+                z[sk(kk)] = make_double2(y1.x - y2.y, y1.y + y2.x);         // y1 + i y2
+                z[sk(Ww - kk)] = make_double2(y1.x + y2.y, -y1.y + y2.x);   // conj(y1) + i conj(y2)
             }
         }
         for (int f = tid; f < nf; f += kRowThreads) {
@@ -284,7 +290,7 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
             const double2 xk = X[rr * hc + K];
             const double2 y1 = cmul(xk, cmul(thi[j1 >> 6], tlo[j1 & 63]));
             const double2 y2 = r1 + 1 < M ? cmul(xk, cmul(thi[j2 >> 6], tlo[j2 & 63])) : make_double2(0.0, 0.0);
-            F[f * (Ww + 1) + K] = make_double2(2.0 * y1.x, 2.0 * y2.x);
+            F[f * FS + sk(K)] = make_double2(2.0 * y1.x, 2.0 * y2.x);
         }
         // Stockham stages: Ns points already combined; radix 4 while it fits, then one radix-2 stage
         int Ns = 1, lgNs = 0;
@@ -298,9 +304,10 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     const int b = tid + u * kRowThreads;
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
-                        const double2* in = F + f * (Ww + 1) + j;
-                        const double2 v0 = in[0], v1 = cmul(in[nb], tw[k * step]),
-                                      v2 = cmul(in[2 * nb], tw[2 * k * step]), v3 = cmul(in[3 * nb], tw[3 * k * step]);
+                        const double2* in = F + f * FS;
+                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tw[k * step]),
+                                      v2 = cmul(in[sk(j + 2 * nb)], tw[2 * k * step]),
+                                      v3 = cmul(in[sk(j + 3 * nb)], tw[3 * k * step]);
                         const double2 a0 = make_double2(v0.x + v2.x, v0.y + v2.y);
                         const double2 a1 = make_double2(v0.x - v2.x, v0.y - v2.y);
                         const double2 b0 = make_double2(v1.x + v3.x, v1.y + v3.y);
@@ -317,9 +324,10 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     const int b = tid + u * kRowThreads;
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
-                        double2* out = F + f * (Ww + 1) + ((j >> lgNs) << (lgNs + 2)) + k;
+                        double2* out = F + f * FS;
+                        const int o = ((j >> lgNs) << (lgNs + 2)) + k;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) out[r * Ns] = v[u][r];
+                        for (int r = 0; r < 4; ++r) out[sk(o + r * Ns)] = v[u][r];
                     }
                 }
                 Ns *= 4;
@@ -332,8 +340,8 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     const int b = tid + u * kRowThreads;
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
-                        const double2* in = F + f * (Ww + 1) + j;
-                        const double2 v0 = in[0], v1 = cmul(in[nb], tw[k * step]);
+                        const double2* in = F + f * FS;
+                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tw[k * step]);
                         v[u][0] = make_double2(v0.x + v1.x, v0.y + v1.y);
                         v[u][1] = make_double2(v0.x - v1.x, v0.y - v1.y);
                     }
@@ -344,9 +352,10 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     const int b = tid + u * kRowThreads;
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
-                        double2* out = F + f * (Ww + 1) + ((j >> lgNs) << (lgNs + 1)) + k;
-                        out[0] = v[u][0];
-                        out[Ns] = v[u][1];
+                        double2* out = F + f * FS;
+                        const int o = ((j >> lgNs) << (lgNs + 1)) + k;
+                        out[sk(o)] = v[u][0];
+                        out[sk(o + Ns)] = v[u][1];
                     }
                 }
                 Ns *= 2;
@@ -361,10 +370,10 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
             const int y = y0 + rr;
             if (y >= HM) break;
             float* dst = a.realf + (static_cast<size_t>(fb) * HM + y) * N;
-            const double2* fr = F + rr * P * (Ww + 1);
+            const double2* fr = F + rr * P * FS;
             for (int t = tid; t < Ww * P; t += kRowThreads) {
                 const int m = static_cast<int>((t + 0.5f) * invP), p = t - m * P, n = M * m + 2 * p;
-                const double2 z = fr[p * (Ww + 1) + m];
+                const double2 z = fr[p * FS + sk(m)];
                 const float v0 = static_cast<float>(z.x * norm);
                 dst[n] = v0;
                 lo = fminf(lo, v0);
