@@ -215,6 +215,17 @@ int npx_roberts(const void *d_img, int dtype, int N, int H, int W, float *d_gc, 
 size_t npx_fht_workspace(int N, int h, int w, int M, int mirror, int tile_size, int overlap);
 int npx_fht_interpolate(const float *d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
                         float *d_out, void *d_work, size_t work_bytes, void *stream);
+/*
+ * npx_fht_interpolate that stores only output rows and columns that are multiples of `stride`, plus the last row and
+ * column (stride 0 or 1: every position); the rest of d_out is left as it was (zero when the frame is tiled, as the
+ * tile blend clears the output first). The rescale still uses the min / max
+ * of the whole interpolated grid, so every stored value equals npx_fht_interpolate's. For eSRRF's use_fht branch
+ * (_le_esrrf.pyx:183-188): its radial gradient convergence reads the x2M gradients only at positions k*M clamped to
+ * the last row / column (_c_sr_radial_gradient_convergence.c:31-32,45-61), so stride = M leaves out ~(1 - 1/M^2) of
+ * the writes. Same workspace as npx_fht_interpolate.
+ */
+int npx_fht_interpolate_lattice(const float *d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
+                                int stride, float *d_out, void *d_work, size_t work_bytes, void *stream);
 
 /*
  * Catmull-Rom shift and magnify of a device (N, H, W) stack (f32 or u16) into d_out (N, int(H*mag_row),

@@ -110,6 +110,8 @@ def lib():
         L.npx_fht_workspace.restype = _c_size_t
         L.npx_fht_interpolate.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                           _c_void_p, _c_void_p, _c_size_t, _c_void_p]
+        L.npx_fht_interpolate_lattice.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                                  _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]
         L.npx_release.restype = None
         _lib = L
     return _lib
@@ -453,8 +455,10 @@ def roberts(stack, stream=None):
     return gc, gr
 
 
-def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, stream=None):
-    """Fourier interpolation of an (N, h, w) stack by an integer factor: (N, h*M, w*M) float32 device tensor."""
+def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, lattice=0, stream=None):
+    """Fourier interpolation of an (N, h, w) stack by an integer factor: (N, h*M, w*M) float32 device tensor. With
+    lattice >= 2 only rows and columns that are multiples of it (and the last ones) are written, the rest of the
+    tensor is left uninitialised (npx_fht_interpolate_lattice)."""
     torch = _torch()
     x = _dev_stack(stack).to(torch.float32).contiguous()
     N, h, w = x.shape
@@ -464,8 +468,13 @@ def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, 
         raise ValueError("bad Fourier interpolation shape")
     work = torch.empty(need, dtype=torch.uint8, device=x.device)
     out = torch.empty((N, h * M, w * M), dtype=torch.float32, device=x.device)
-    _check(lib().npx_fht_interpolate(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size), int(overlap),
-                                     out.data_ptr(), work.data_ptr(), need, _stream(torch, stream)))
+    if lattice:
+        _check(lib().npx_fht_interpolate_lattice(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size),
+                                                 int(overlap), int(lattice), out.data_ptr(), work.data_ptr(), need,
+                                                 _stream(torch, stream)))
+    else:
+        _check(lib().npx_fht_interpolate(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size), int(overlap),
+                                         out.data_ptr(), work.data_ptr(), need, _stream(torch, stream)))
     return out
 
 

@@ -27,13 +27,15 @@ def _is_tensor(x) -> bool:
 def _esrrf_fht(image, M, radius, sensitivity, iw, pad_edges):
     """The use_fht branch (_le_esrrf.pyx:183-188): Fourier xM of the image, Roberts cross of the image, Fourier x2M
     of both gradients (npx_fht_interpolate, npx_roberts), then RGC on those arrays with pad_edges forwarded
-    (npx_rgc); returns the (N, H*M, W*M) device tensor."""
+    (npx_rgc); returns the (N, H*M, W*M) device tensor. The RGC reads the x2M gradients only at multiples of M
+    (and the last row / column), so only those positions of them are written (npx_fht_interpolate_lattice)."""
     x = _hip._dev_stack(image)
     mag = _hip.fht_interpolate(x, M)
     gc, gr = _hip.roberts(x)
-    gci = _hip.fht_interpolate(gc, 2 * M)
+    lat = M if M >= 2 else 0
+    gci = _hip.fht_interpolate(gc, 2 * M, lattice=lat)
     del gc
-    gri = _hip.fht_interpolate(gr, 2 * M)
+    gri = _hip.fht_interpolate(gr, 2 * M, lattice=lat)
     del gr
     return _hip.rgc(gci, gri, mag, M, radius, sensitivity, iw, pad_edges)
 

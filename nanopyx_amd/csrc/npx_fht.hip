@@ -75,7 +75,25 @@ struct FhtTile {
     int Hw, Ww;          // padded working size
     int yroi, xroi;      // crop offsets of the tile inside the enlarged grid (output pixels)
     int tiles_h, tiles_w, mid_h, mid_w, off_h, off_w;  // mirror tiling: copies per axis, middle copy, crop offsets
+    int lat;             // >= 2: write only output rows / columns that are multiples of lat, and the last ones
 };
+
+// Output position g of an axis of n positions is written: all of them, or (lattice output) the multiples of lat and
+// the last position, which are what a radial-gradient-convergence pass reads from a x2M gradient (its sample
+// positions are k M, clamped to the last row / column).
+__device__ inline bool on_lattice(int g, int lat, int n) { return lat < 2 || g % lat == 0 || g == n - 1; }
+
+// Kept positions of a span [g0, g0 + len) of an axis of n positions: all len of them (lat < 2), or the multiples of
+// lat in it and the last position of the axis when it falls inside; kept_index gives the t-th one's offset in the span.
+__host__ __device__ inline int kept_count(int g0, int len, int n, int lat) {
+    if (lat < 2) return len;
+    const int k0 = (g0 + lat - 1) / lat, nk = (g0 + len - 1) / lat - k0 + 1;
+    return nk + (n - 1 >= g0 && n - 1 < g0 + len && (n - 1) % lat != 0 ? 1 : 0);
+}
+__device__ inline int kept_index(int g0, int len, int n, int lat, int t) {
+    const int k0 = (g0 + lat - 1) / lat, nk = (g0 + len - 1) / lat - k0 + 1;
+    return (t < nk ? (k0 + t) * lat : n - 1) - g0;
+}
 
 __device__ inline int fmod2(int v) { return ((v % 2) + 2) % 2; }
 
@@ -366,23 +384,36 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
         // pair p at m holds phases 2p (real part) and 2p + 1 (imaginary part): outputs n = M m + 2p and n + 1.
         // m = t / P through a float reciprocal: t < Ww P <= 2^12, so the error stays far below 1 / (2P).
         const float invP = 1.f / P;
+        // only positions inside the crop that fht_finish reads are stored; min / max covers the whole grid
+        const int oh = a.th * M, ow = a.tw * M, gh = a.h * M, gw = a.w * M;
         for (int rr = 0; rr < rb; ++rr) {
             const int y = y0 + rr;
             if (y >= HM) break;
+            const int yo = y - a.yroi;
+            const bool row_kept = yo >= 0 && yo < oh && on_lattice(a.r0 * M + yo, a.lat, gh);
             float* dst = a.realf + (static_cast<size_t>(fb) * HM + y) * N;
             const double2* fr = F + rr * P * FS;
+            const bool store_all = row_kept && a.lat < 2;
             for (int t = tid; t < Ww * P; t += kRowThreads) {
                 const int m = static_cast<int>((t + 0.5f) * invP), p = t - m * P, n = M * m + 2 * p;
                 const double2 z = fr[p * FS + sk(m)];
-                const float v0 = static_cast<float>(z.x * norm);
-                dst[n] = v0;
-                lo = fminf(lo, v0);
-                hi = fmaxf(hi, v0);
-                if (2 * p + 1 < M) {
-                    const float v1 = static_cast<float>(z.y * norm);
-                    dst[n + 1] = v1;
-                    lo = fminf(lo, v1);
-                    hi = fmaxf(hi, v1);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && 2 * p + 1 >= M) break;
+                    const float v = static_cast<float>((h ? z.y : z.x) * norm);
+                    lo = fminf(lo, v);
+                    hi = fmaxf(hi, v);
+                    const int xo = n + h - a.xroi;
+                    if (store_all && xo >= 0 && xo < ow) dst[n + h] = v;
+                }
+            }
+            if (row_kept && a.lat >= 2) {  // lattice output: only its columns inside the crop, same values as above
+                const int nkc = kept_count(a.c0 * M, ow, gw, a.lat);
+                for (int t = tid; t < nkc; t += kRowThreads) {
+                    const int n = kept_index(a.c0 * M, ow, gw, a.lat, t) + a.xroi;
+                    const int m = n / M, r = n - m * M;
+                    const double2 z = fr[(r >> 1) * FS + sk(m)];
+                    dst[n] = static_cast<float>((r & 1 ? z.y : z.x) * norm);
                 }
             }
         }
@@ -475,7 +506,11 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
 #pragma clang fp contract(off)
     const int HM = a.Hw * a.M, WM = a.Ww * a.M;
     const int oh = a.th * a.M, ow = a.tw * a.M;
-    const int fb = blockIdx.x / oh, y = blockIdx.x % oh;
+    // lattice output: blocks cover only the kept rows (multiples of lat inside the tile's rows, and the last row)
+    const int nry = kept_count(a.r0 * a.M, oh, a.h * a.M, a.lat);
+    const int fb = blockIdx.x / nry, t = blockIdx.x % nry;
+    const int y = a.lat < 2 ? t : kept_index(a.r0 * a.M, oh, a.h * a.M, a.lat, t);
+    const int gw = a.w * a.M;
     const double norm = 1.0 / (static_cast<double>(HM) * WM);
     const int* m = reinterpret_cast<const int*>(a.mm) + 4 * fb;
     const float lo = o2f(m[0]), hi = o2f(m[1]), omin = o2f(m[2]), omax = o2f(m[3]);
@@ -490,6 +525,26 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
                       static_cast<size_t>(a.c0) * a.M;
     float* out = a.out + o0;
     const float wr = a.tiled ? a.win_r[y] : 0.f;
+    if (a.lat >= 2) {  // lattice output: the multiples of lat inside this tile's columns, and the last column
+        const int nkc = kept_count(a.c0 * a.M, ow, gw, a.lat);
+        for (int t = threadIdx.x; t < nkc; t += blockDim.x) {
+            const int x = kept_index(a.c0 * a.M, ow, gw, a.lat, t);
+            float v = srcf ? srcf[x] : static_cast<float>(src[x] * norm);
+            if (rescale) {
+                v = v - omin;
+                v = v * scale;
+                v = v + lo;
+            }
+            if (a.tiled) {
+                const float wv = wr * a.win_c[x];
+                out[x] += v * wv;
+                a.weight[o0 + x] += wv;
+            } else {
+                out[x] = v;
+            }
+        }
+        return;
+    }
     constexpr int U = 4;
     for (int x0 = threadIdx.x; x0 < ow; x0 += U * blockDim.x) {
         float d[U];
@@ -694,9 +749,10 @@ size_t npx_fht_workspace(int N, int h, int w, int M, int mirror, int tile_size, 
     return layout_of(N, h, w, M, th, tw, tiled ? true : mirror != 0, tiled).total;
 }
 
-int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
-                        float* d_out, void* d_work, size_t work_bytes, void* stream) {
+int npx_fht_interpolate_lattice(const float* d_in, int N, int h, int w, int M, int mirror, int tile_size,
+                                int overlap, int stride, float* d_out, void* d_work, size_t work_bytes, void* stream) {
     using namespace npx;
+    if (stride < 0) return host::fail(NPX_ERR_ARG, "lattice stride must be >= 0");
     if (!d_in || !d_out || !d_work) return host::fail(NPX_ERR_ARG, "NULL array pointer");
     if (N < 1 || h < 1 || w < 1 || M < 1) return host::fail(NPX_ERR_ARG, "bad shape");
     const int tile = tile_size > 0 ? tile_size : 256;
@@ -730,6 +786,7 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
     a.weight = tiled ? reinterpret_cast<float*>(wb + off) : nullptr;
     a.tiled = tiled ? 1 : 0;
     a.mirror = tiled ? 1 : (mirror ? 1 : 0);
+    a.lat = stride;
     const size_t nout = static_cast<size_t>(N) * h * M * w * M;
     if (tiled) {
         hipLaunchKernelGGL(fill_zero, dim3(grid_of(nout)), dim3(256), 0, st, d_out, nout);
@@ -800,7 +857,8 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
                     const int ngrp = (a.Hw * M + kRowsPerBlock - 1) / kRowsPerBlock;
                     hipLaunchKernelGGL(fht_rows, dim3(ngrp * a.B), dim3(kRowThreads), row_shape(a.Ww, M, rb).bytes, st,
                                        a, rb);
-                    hipLaunchKernelGGL(fht_finish, dim3(a.th * M * a.B), dim3(256), 0, st, a);
+                    if (const int ny = kept_count(a.r0 * M, a.th * M, h * M, a.lat); ny > 0)
+                    hipLaunchKernelGGL(fht_finish, dim3(ny * a.B), dim3(256), 0, st, a);
                     hipError_t e = hipGetLastError();
                     if (e != hipSuccess) return host::hip_fail(e, "Fourier interpolation launch");
                     continue;
@@ -816,7 +874,8 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
                 const size_t mmx = std::min<size_t>((bigp / 2 + 1023) / 1024, std::max(1, 8192 / a.B));
                 hipLaunchKernelGGL(fht_minmax, dim3(static_cast<unsigned>(std::max<size_t>(1, mmx)), a.B), dim3(256),
                                    0, st, a);
-                hipLaunchKernelGGL(fht_finish, dim3(a.th * M * a.B), dim3(256), 0, st, a);
+                if (const int ny = kept_count(a.r0 * M, a.th * M, h * M, a.lat); ny > 0)
+                    hipLaunchKernelGGL(fht_finish, dim3(ny * a.B), dim3(256), 0, st, a);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return host::hip_fail(e, "Fourier interpolation launch");
             }
@@ -826,6 +885,12 @@ int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirro
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return host::hip_fail(e, "Fourier interpolation launch");
     return NPX_OK;
+}
+
+int npx_fht_interpolate(const float* d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
+                        float* d_out, void* d_work, size_t work_bytes, void* stream) {
+    return npx_fht_interpolate_lattice(d_in, N, h, w, M, mirror, tile_size, overlap, 0, d_out, d_work, work_bytes,
+                                       stream);
 }
 
 }  // extern "C"

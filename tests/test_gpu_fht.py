@@ -81,3 +81,32 @@ def test_esrrf_workflow_use_fht(torch, oracle_built):
         maps = fht.esrrf_fht(x[3 * t:3 * t + 3], 2, 1.5, 1, True, False)
         ref = oracle_built.macro_pixel(oracle_built.temporal(maps, "VAR")[None], 2)[0]
         assert _report(f"eSRRF(use_fht) timepoint {t}", got[t], ref) <= TOL
+
+
+@pytest.mark.parametrize("shape,M,stride,mirror,tile", [
+    ((2, 32, 40), 10, 5, True, 0),      # eSRRF M5's x10 gradients, whole frames
+    ((1, 150, 170), 4, 2, True, 64),    # Hann-blended tiles
+    ((2, 31, 29), 6, 3, False, 0),      # no padding, odd sizes: the complex-to-real path
+])
+def test_fht_lattice_output(torch, shape, M, stride, mirror, tile):
+    """npx_fht_interpolate_lattice stores exactly the rows and columns that are multiples of the stride plus the last
+    ones, each bit-identical to npx_fht_interpolate's value (the rescale uses the whole grid's min / max); every other
+    position keeps what the caller's buffer held (NaN here), or is zero when the frame is tiled (the blend clears the
+    output first)."""
+    from nanopyx_amd import _hip
+    print(f"fht lattice case {shape} M{M} stride {stride} tile {tile}", flush=True)
+    x = torch.from_numpy(_stack(shape, seed=M + stride)).cuda()
+    N, h, w = shape
+    full = _hip.fht_interpolate(x, M, mirror, tile, 8).cpu().numpy()
+    need = int(_hip.lib().npx_fht_workspace(N, h, w, M, int(mirror), tile, 8))
+    work = torch.empty(need, dtype=torch.uint8, device="cuda")
+    out = torch.full((N, h * M, w * M), float("nan"), dtype=torch.float32, device="cuda")
+    _hip._check(_hip.lib().npx_fht_interpolate_lattice(x.data_ptr(), N, h, w, M, int(mirror), tile, 8, stride,
+                                                       out.data_ptr(), work.data_ptr(), need, None))
+    got = out.cpu().numpy()
+    keep_r = (np.arange(h * M) % stride == 0) | (np.arange(h * M) == h * M - 1)
+    keep_c = (np.arange(w * M) % stride == 0) | (np.arange(w * M) == w * M - 1)
+    keep = np.broadcast_to(keep_r[:, None] & keep_c[None, :], got.shape)
+    assert np.array_equal(got[keep], full[keep])
+    tiled = tile > 0 and (h > tile or w > tile)
+    assert (not got[~keep].any()) if tiled else np.isnan(got[~keep]).all()
