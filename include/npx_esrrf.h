@@ -74,6 +74,15 @@ int npx_esrrf_temporal(const void *d_in, int dtype, int N, int H, int W, int M, 
  */
 int npx_rgc(const float *d_gx, const float *d_gy, const float *d_img, int N, int HM, int WM, int M, double R,
             double S, int iw, int pad_edges, float *d_out, void *stream);
+/*
+ * npx_rgc with the gradients given only at the lattice it reads, compact: d_gx, d_gy (N, 2H + 1, 2W + 1) with
+ * H = HM / M, W = WM / M, entry (i, j) = the x2M array's value at row min(i M, 2 HM - 1), column min(j M, 2 WM - 1)
+ * (npx_fht_interpolate_lattice with stride M writes this layout). Maps bit-identical to npx_rgc on the full arrays.
+ * Runs on the fused kernel only: NPX_ERR_UNSUPPORTED when M < 2, HM or WM is not a multiple of M, M > 64 or the
+ * radius is outside the eSRRF kernels' range. For eSRRF's use_fht branch (_le_esrrf.pyx:183-188).
+ */
+int npx_rgc_lattice(const float *d_gx, const float *d_gy, const float *d_img, int N, int HM, int WM, int M,
+                    double R, double S, int iw, int pad_edges, float *d_out, void *stream);
 
 /*
  * AVG / VAR / TAC2 over axis 0 of a device (N, P) float32 stack. Replaces calculate_eSRRF_temporal_correlations,
@@ -216,13 +225,13 @@ size_t npx_fht_workspace(int N, int h, int w, int M, int mirror, int tile_size, 
 int npx_fht_interpolate(const float *d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
                         float *d_out, void *d_work, size_t work_bytes, void *stream);
 /*
- * npx_fht_interpolate that stores only output rows and columns that are multiples of `stride`, plus the last row and
- * column (stride 0 or 1: every position); the rest of d_out is left as it was (zero when the frame is tiled, as the
- * tile blend clears the output first). The rescale still uses the min / max
- * of the whole interpolated grid, so every stored value equals npx_fht_interpolate's. For eSRRF's use_fht branch
- * (_le_esrrf.pyx:183-188): its radial gradient convergence reads the x2M gradients only at positions k*M clamped to
- * the last row / column (_c_sr_radial_gradient_convergence.c:31-32,45-61), so stride = M leaves out ~(1 - 1/M^2) of
- * the writes. Same workspace as npx_fht_interpolate.
+ * npx_fht_interpolate at the lattice positions only, compact: d_out (N, LH, LW) holds the interpolated (N, h*M, w*M)
+ * result at output rows that are multiples of `stride` (>= 2) and the last row, and likewise for columns, in order;
+ * LH = ceil(h*M / stride) + (1 if (h*M - 1) % stride != 0 else 0), LW likewise. The rescale still uses the min / max
+ * of the whole interpolated grid, so every value equals npx_fht_interpolate's at that position. For eSRRF's use_fht
+ * branch (_le_esrrf.pyx:183-188): its radial gradient convergence reads the x2M gradients only at positions k*M
+ * clamped to the last row / column (_c_sr_radial_gradient_convergence.c:31-32,45-61); stride = M gives the
+ * (2H + 1, 2W + 1) lattice npx_rgc_lattice reads. Same workspace as npx_fht_interpolate.
  */
 int npx_fht_interpolate_lattice(const float *d_in, int N, int h, int w, int M, int mirror, int tile_size, int overlap,
                                 int stride, float *d_out, void *d_work, size_t work_bytes, void *stream);

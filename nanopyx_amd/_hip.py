@@ -55,6 +55,7 @@ def lib():
                                          _c_size_t, _c_void_p]
         L.npx_rgc.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_double,
                               _c_double, _c_int, _c_int, _c_void_p, _c_void_p]
+        L.npx_rgc_lattice.argtypes = L.npx_rgc.argtypes
         L.npx_temporal.argtypes = [_c_void_p, _c_int, ctypes.c_longlong, ctypes.c_uint, _c_void_p, _c_void_p,
                                    _c_void_p, _c_void_p]
         L.npx_macro_pixel.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]
@@ -382,6 +383,30 @@ def rgc(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=N
     return out
 
 
+def lattice_shape(n, stride):
+    """Kept positions of an axis of n points for a lattice stride (>= 2): the multiples of it and the last point."""
+    return -(-n // stride) + (1 if (n - 1) % stride else 0)
+
+
+def rgc_lattice(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=None):
+    """rgc with the gradients given only at the lattice it reads, compact (N, 2H+1, 2W+1) (npx_rgc_lattice);
+    raises NpxError for shapes or radii only the per-pixel kernel covers."""
+    torch = _torch()
+    gx, gy, img = (_dev_stack(a).to(torch.float32).contiguous() for a in (gx, gy, img))
+    N, HM, WM = img.shape
+    M = int(magnification)
+    if M < 2 or HM % M or WM % M:
+        raise NpxError("compact lattice needs an image of H x W pixels magnified M >= 2 times")
+    shape = (N, 2 * (HM // M) + 1, 2 * (WM // M) + 1)
+    if tuple(gx.shape) != shape or tuple(gy.shape) != shape:
+        raise ValueError(f"compact gradient lattices must be {shape}")
+    out = torch.empty_like(img)
+    _check(lib().npx_rgc_lattice(gx.data_ptr(), gy.data_ptr(), img.data_ptr(), N, HM, WM, M, float(radius),
+                                 float(sensitivity), int(bool(iw)), int(bool(pad_edges)), out.data_ptr(),
+                                 _stream(torch, stream)))
+    return out
+
+
 def temporal(stack, modes, stream=None):
     """AVG / VAR / TAC2 over axis 0 of an (N, ...) stack; returns {mode: device tensor}."""
     torch = _torch()
@@ -457,8 +482,8 @@ def roberts(stack, stream=None):
 
 def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, lattice=0, stream=None):
     """Fourier interpolation of an (N, h, w) stack by an integer factor: (N, h*M, w*M) float32 device tensor. With
-    lattice >= 2 only rows and columns that are multiples of it (and the last ones) are written, the rest of the
-    tensor is left uninitialised (npx_fht_interpolate_lattice)."""
+    lattice >= 2 only the rows and columns that are multiples of it and the last ones, compact:
+    (N, lattice_shape(h*M, lattice), lattice_shape(w*M, lattice)) (npx_fht_interpolate_lattice)."""
     torch = _torch()
     x = _dev_stack(stack).to(torch.float32).contiguous()
     N, h, w = x.shape
@@ -467,7 +492,11 @@ def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, 
     if need == 0:
         raise ValueError("bad Fourier interpolation shape")
     work = torch.empty(need, dtype=torch.uint8, device=x.device)
-    out = torch.empty((N, h * M, w * M), dtype=torch.float32, device=x.device)
+    if lattice and lattice >= 2:
+        shape = (N, lattice_shape(h * M, lattice), lattice_shape(w * M, lattice))
+    else:
+        shape = (N, h * M, w * M)
+    out = torch.empty(shape, dtype=torch.float32, device=x.device)
     if lattice:
         _check(lib().npx_fht_interpolate_lattice(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size),
                                                  int(overlap), int(lattice), out.data_ptr(), work.data_ptr(), need,

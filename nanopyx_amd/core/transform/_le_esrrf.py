@@ -28,14 +28,22 @@ def _esrrf_fht(image, M, radius, sensitivity, iw, pad_edges):
     """The use_fht branch (_le_esrrf.pyx:183-188): Fourier xM of the image, Roberts cross of the image, Fourier x2M
     of both gradients (npx_fht_interpolate, npx_roberts), then RGC on those arrays with pad_edges forwarded
     (npx_rgc); returns the (N, H*M, W*M) device tensor. The RGC reads the x2M gradients only at multiples of M
-    (and the last row / column), so only those positions of them are written (npx_fht_interpolate_lattice)."""
+    (and the last row / column), so for M >= 2 only that lattice of them is computed into compact arrays
+    (npx_fht_interpolate_lattice) and read by npx_rgc_lattice; radii that kernel does not cover take the full
+    arrays and npx_rgc."""
     x = _hip._dev_stack(image)
     mag = _hip.fht_interpolate(x, M)
     gc, gr = _hip.roberts(x)
-    lat = M if M >= 2 else 0
-    gci = _hip.fht_interpolate(gc, 2 * M, lattice=lat)
+    if M >= 2:
+        gci = _hip.fht_interpolate(gc, 2 * M, lattice=M)
+        gri = _hip.fht_interpolate(gr, 2 * M, lattice=M)
+        try:
+            return _hip.rgc_lattice(gci, gri, mag, M, radius, sensitivity, iw, pad_edges)
+        except _hip.NpxError:
+            del gci, gri
+    gci = _hip.fht_interpolate(gc, 2 * M)
     del gc
-    gri = _hip.fht_interpolate(gr, 2 * M, lattice=lat)
+    gri = _hip.fht_interpolate(gr, 2 * M)
     del gr
     return _hip.rgc(gci, gri, mag, M, radius, sensitivity, iw, pad_edges)
 

@@ -640,8 +640,10 @@ int npx_graph_destroy(void* graph) {
     return NPX_OK;
 }
 
-int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int HM, int WM, int M, double R,
-            double S, int iw, int pad_edges, float* d_out, void* stream) {
+// npx_rgc and npx_rgc_lattice: `compact` gradients hold only the lattice the RGC reads, (N, 2H + 1, 2W + 1), and
+// run only on the fused kernel (NPX_ERR_UNSUPPORTED where the per-pixel kernel would be needed).
+static int rgc_impl(const float* d_gx, const float* d_gy, const float* d_img, int N, int HM, int WM, int M, double R,
+                    double S, int iw, int pad_edges, float* d_out, void* stream, bool compact) {
     if (!d_gx || !d_gy || !d_img || !d_out) return fail(NPX_ERR_ARG, "NULL array pointer");
     if (N < 1 || HM < 1 || WM < 1 || M < 1) return fail(NPX_ERR_ARG, "bad shape");
     const float sigma = static_cast<float>(R / 2.355);
@@ -653,12 +655,14 @@ int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int
     // An image of H x W input pixels magnified M times: the fused kernel runs on the gradient lattice (the caller's
     // arrays at every M-th position) with the plan's ring tables, as the eSRRF path does on its own lattice.
     // Other shapes, and radii or sizes the plan does not cover, take the per-pixel kernel below.
+    if (compact && M < 2) return fail(NPX_ERR_UNSUPPORTED, "compact lattice: needs M >= 2");
     if (HM % M == 0 && WM % M == 0 && M <= 64) {
         const int H = HM / M, W = WM / M;
         const npx::DevTables* t = nullptr;
         Plan* p = nullptr;
         const std::string prev = g_err;
-        if (plan_and_tables(H, W, M, R, &p, &t, st) == NPX_OK) {
+        const int rc = plan_and_tables(H, W, M, R, &p, &t, st);
+        if (rc == NPX_OK) {
             const LaunchCfg c = launch_cfg(H, W, M, R);
             Shape s = fused_shape(H, W, M, c);
             s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
@@ -666,6 +670,7 @@ int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int
             a.agx = d_gx;
             a.agy = d_gy;
             a.aimg = d_img;
+            a.alat = compact ? 1 : 0;
             a.margin[0] = margin;
             const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
             long long want = std::max<long long>(1, (c.target_blocks + spatial - 1) / spatial);
@@ -678,12 +683,24 @@ int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int
             if (e != hipSuccess) return hip_fail(e, "esrrf_fused(arrays) launch");
             return NPX_OK;
         }
+        if (compact) return rc;  // g_err holds the plan's reason
         g_err = prev;
     }
+    if (compact) return fail(NPX_ERR_UNSUPPORTED, "compact lattice: needs H x W pixels magnified M <= 64 times");
     hipError_t e = npx::launch_rgc_arrays(d_gx, d_gy, d_img, N, HM, WM, M, fwhm, tSO, tSS, pow_mode_of(S),
                                           static_cast<float>(S), iw ? 1 : 0, margin, d_out, st);
     if (e != hipSuccess) return hip_fail(e, "rgc_arrays launch");
     return NPX_OK;
+}
+
+int npx_rgc(const float* d_gx, const float* d_gy, const float* d_img, int N, int HM, int WM, int M, double R,
+            double S, int iw, int pad_edges, float* d_out, void* stream) {
+    return rgc_impl(d_gx, d_gy, d_img, N, HM, WM, M, R, S, iw, pad_edges, d_out, stream, false);
+}
+
+int npx_rgc_lattice(const float* d_gx, const float* d_gy, const float* d_img, int N, int HM, int WM, int M,
+                    double R, double S, int iw, int pad_edges, float* d_out, void* stream) {
+    return rgc_impl(d_gx, d_gy, d_img, N, HM, WM, M, R, S, iw, pad_edges, d_out, stream, true);
 }
 
 int npx_temporal(const float* d_stack, int N, long long P, unsigned mode_mask, float* d_avg, float* d_var,

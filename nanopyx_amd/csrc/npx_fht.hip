@@ -75,7 +75,7 @@ struct FhtTile {
     int Hw, Ww;          // padded working size
     int yroi, xroi;      // crop offsets of the tile inside the enlarged grid (output pixels)
     int tiles_h, tiles_w, mid_h, mid_w, off_h, off_w;  // mirror tiling: copies per axis, middle copy, crop offsets
-    int lat;             // >= 2: write only output rows / columns that are multiples of lat, and the last ones
+    int lat;             // >= 2: write only output rows / columns that are multiples of lat and the last ones, compact
 };
 
 // Output position g of an axis of n positions is written: all of them, or (lattice output) the multiples of lat and
@@ -508,8 +508,8 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
     const int oh = a.th * a.M, ow = a.tw * a.M;
     // lattice output: blocks cover only the kept rows (multiples of lat inside the tile's rows, and the last row)
     const int nry = kept_count(a.r0 * a.M, oh, a.h * a.M, a.lat);
-    const int fb = blockIdx.x / nry, t = blockIdx.x % nry;
-    const int y = a.lat < 2 ? t : kept_index(a.r0 * a.M, oh, a.h * a.M, a.lat, t);
+    const int fb = blockIdx.x / nry, ty = blockIdx.x % nry;
+    const int y = a.lat < 2 ? ty : kept_index(a.r0 * a.M, oh, a.h * a.M, a.lat, ty);
     const int gw = a.w * a.M;
     const double norm = 1.0 / (static_cast<double>(HM) * WM);
     const int* m = reinterpret_cast<const int*>(a.mm) + 4 * fb;
@@ -520,15 +520,17 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
     const size_t row = (static_cast<size_t>(fb) * HM + y + a.yroi) * WM + a.xroi;
     const double* src = a.real + row;
     const float* srcf = a.realf ? a.realf + row : nullptr;  // float32 grid written by fht_rows
-    const size_t wo = static_cast<size_t>(a.w) * a.M;
-    const size_t o0 = static_cast<size_t>(a.f0 + fb) * a.h * a.M * wo + (static_cast<size_t>(a.r0) * a.M + y) * wo +
-                      static_cast<size_t>(a.c0) * a.M;
-    float* out = a.out + o0;
     const float wr = a.tiled ? a.win_r[y] : 0.f;
-    if (a.lat >= 2) {  // lattice output: the multiples of lat inside this tile's columns, and the last column
+    if (a.lat >= 2) {
+        // lattice output, compact (N, LH, LW): kept row gy goes to gy / lat (the last row, when not a multiple, to
+        // LH - 1); the tile's kept columns likewise
+        const int gh = a.h * a.M, LH = kept_count(0, gh, gh, a.lat), LW = kept_count(0, gw, gw, a.lat);
+        const int gy = a.r0 * a.M + y, ci = gy % a.lat == 0 ? gy / a.lat : LH - 1;
+        const size_t orow = (static_cast<size_t>(a.f0 + fb) * LH + ci) * LW;
         const int nkc = kept_count(a.c0 * a.M, ow, gw, a.lat);
         for (int t = threadIdx.x; t < nkc; t += blockDim.x) {
-            const int x = kept_index(a.c0 * a.M, ow, gw, a.lat, t);
+            const int x = kept_index(a.c0 * a.M, ow, gw, a.lat, t), gx = a.c0 * a.M + x;
+            const size_t o = orow + (gx % a.lat == 0 ? gx / a.lat : LW - 1);
             float v = srcf ? srcf[x] : static_cast<float>(src[x] * norm);
             if (rescale) {
                 v = v - omin;
@@ -537,14 +539,18 @@ __global__ __launch_bounds__(256) void fht_finish(FhtTile a) {
             }
             if (a.tiled) {
                 const float wv = wr * a.win_c[x];
-                out[x] += v * wv;
-                a.weight[o0 + x] += wv;
+                a.out[o] += v * wv;
+                a.weight[o] += wv;
             } else {
-                out[x] = v;
+                a.out[o] = v;
             }
         }
         return;
     }
+    const size_t wo = static_cast<size_t>(a.w) * a.M;
+    const size_t o0 = static_cast<size_t>(a.f0 + fb) * a.h * a.M * wo + (static_cast<size_t>(a.r0) * a.M + y) * wo +
+                      static_cast<size_t>(a.c0) * a.M;
+    float* out = a.out + o0;
     constexpr int U = 4;
     for (int x0 = threadIdx.x; x0 < ow; x0 += U * blockDim.x) {
         float d[U];
@@ -787,7 +793,9 @@ int npx_fht_interpolate_lattice(const float* d_in, int N, int h, int w, int M, i
     a.tiled = tiled ? 1 : 0;
     a.mirror = tiled ? 1 : (mirror ? 1 : 0);
     a.lat = stride;
-    const size_t nout = static_cast<size_t>(N) * h * M * w * M;
+    const size_t nout = stride >= 2 ? static_cast<size_t>(N) * kept_count(0, h * M, h * M, stride) *
+                                          kept_count(0, w * M, w * M, stride)
+                                    : static_cast<size_t>(N) * h * M * w * M;  // compact lattice or every point
     if (tiled) {
         hipLaunchKernelGGL(fill_zero, dim3(grid_of(nout)), dim3(256), 0, st, d_out, nout);
         hipLaunchKernelGGL(fill_zero, dim3(grid_of(nout)), dim3(256), 0, st, a.weight, nout);

@@ -481,6 +481,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             // its x2M arrays, normalised (_c_sr_radial_gradient_convergence.c:45-52 reads those positions); points
             // outside a in [1, 2 HM - 2], b in [1, 2 WM - 2] are outside every window and hold 0
             const size_t P4 = static_cast<size_t>(4) * a.HM * a.WM;
+            const int CW = 2 * a.W + 1;  // compact lattice: row min(a, 2H), column min(b, 2W)
+            const size_t CP = static_cast<size_t>(2 * a.H + 1) * CW;
             for (int idx = tid; idx < 2 * LPL; idx += nthr) {
                 const int plane = idx / LPL;
                 const int rem = idx - plane * LPL;
@@ -489,9 +491,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 const int aa = a_lo + la, bb = b_lo + 2 * pos + plane;
                 float2 g = make_float2(0.f, 0.f);
                 if (aa >= 1 && aa <= 2 * a.HM - 2 && bb >= 1 && bb <= 2 * a.WM - 2) {
-                    const size_t o = static_cast<size_t>(f) * P4 +
-                                     static_cast<size_t>(min(aa * M, 2 * a.HM - 1)) * (2 * a.WM) +
-                                     min(bb * M, 2 * a.WM - 1);
+                    const size_t o = a.alat ? static_cast<size_t>(f) * CP + static_cast<size_t>(min(aa, 2 * a.H)) * CW +
+                                                  min(bb, 2 * a.W)
+                                            : static_cast<size_t>(f) * P4 +
+                                                  static_cast<size_t>(min(aa * M, 2 * a.HM - 1)) * (2 * a.WM) +
+                                                  min(bb * M, 2 * a.WM - 1);
                     const float gxv = a.agx[o], gyv = a.agy[o];
                     const float n2 = gxv * gxv + gyv * gyv;
                     if (n2 > 0.f) {

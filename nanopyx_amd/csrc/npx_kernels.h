@@ -98,10 +98,12 @@ struct FusedArgs {
     float* var;
     float* tac2;
     float* sweep_avg[kMaxRad * kMaxSens];  // sweep: AVG map of (radius r, sensitivity s) at [r * kMaxSens + s]
-    // arrays mode: column and row gradients (N, 2 HM, 2 WM) and the intensity image (N, HM, WM) of the caller
+    // arrays mode: column and row gradients (N, 2 HM, 2 WM) and the intensity image (N, HM, WM) of the caller;
+    // alat = 1: the gradients hold only the lattice, compact (N, 2H + 1, 2W + 1): rows / columns k M and the last one
     const float* agx;
     const float* agy;
     const float* aimg;
+    int alat;
 };
 
 // Several jobs in one launch: job j owns blocks [start[j], start[j + 1]), laid out as

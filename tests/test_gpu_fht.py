@@ -87,26 +87,48 @@ def test_esrrf_workflow_use_fht(torch, oracle_built):
     ((2, 32, 40), 10, 5, True, 0),      # eSRRF M5's x10 gradients, whole frames
     ((1, 150, 170), 4, 2, True, 64),    # Hann-blended tiles
     ((2, 31, 29), 6, 3, False, 0),      # no padding, odd sizes: the complex-to-real path
+    ((2, 20, 22), 3, 2, True, 0),       # odd x M: the last row and column are not multiples of the stride
 ])
 def test_fht_lattice_output(torch, shape, M, stride, mirror, tile):
-    """npx_fht_interpolate_lattice stores exactly the rows and columns that are multiples of the stride plus the last
-    ones, each bit-identical to npx_fht_interpolate's value (the rescale uses the whole grid's min / max); every other
-    position keeps what the caller's buffer held (NaN here), or is zero when the frame is tiled (the blend clears the
-    output first)."""
+    """npx_fht_interpolate_lattice writes, compact, exactly the rows and columns that are multiples of the stride
+    plus the last ones, each bit-identical to npx_fht_interpolate's value at that position (the rescale uses the
+    whole grid's min / max); every entry of the compact output is written."""
     from nanopyx_amd import _hip
     print(f"fht lattice case {shape} M{M} stride {stride} tile {tile}", flush=True)
     x = torch.from_numpy(_stack(shape, seed=M + stride)).cuda()
     N, h, w = shape
     full = _hip.fht_interpolate(x, M, mirror, tile, 8).cpu().numpy()
+    LH, LW = _hip.lattice_shape(h * M, stride), _hip.lattice_shape(w * M, stride)
     need = int(_hip.lib().npx_fht_workspace(N, h, w, M, int(mirror), tile, 8))
     work = torch.empty(need, dtype=torch.uint8, device="cuda")
-    out = torch.full((N, h * M, w * M), float("nan"), dtype=torch.float32, device="cuda")
+    out = torch.full((N, LH, LW), float("nan"), dtype=torch.float32, device="cuda")
     _hip._check(_hip.lib().npx_fht_interpolate_lattice(x.data_ptr(), N, h, w, M, int(mirror), tile, 8, stride,
                                                        out.data_ptr(), work.data_ptr(), need, None))
     got = out.cpu().numpy()
-    keep_r = (np.arange(h * M) % stride == 0) | (np.arange(h * M) == h * M - 1)
-    keep_c = (np.arange(w * M) % stride == 0) | (np.arange(w * M) == w * M - 1)
-    keep = np.broadcast_to(keep_r[:, None] & keep_c[None, :], got.shape)
-    assert np.array_equal(got[keep], full[keep])
-    tiled = tile > 0 and (h > tile or w > tile)
-    assert (not got[~keep].any()) if tiled else np.isnan(got[~keep]).all()
+    rows = [g for g in range(h * M) if g % stride == 0 or g == h * M - 1]
+    cols = [g for g in range(w * M) if g % stride == 0 or g == w * M - 1]
+    assert (LH, LW) == (len(rows), len(cols))
+    assert np.array_equal(got, full[:, rows][:, :, cols])
+    assert np.array_equal(_hip.fht_interpolate(x, M, mirror, tile, 8, lattice=stride).cpu().numpy(), got)
+
+
+@pytest.mark.parametrize("M,R,S,iw,pad", [(2, 1.5, 1, True, False), (3, 1.0, 2, True, True), (5, 1.5, 1, False, False),
+                                          (4, 2.5, 3, True, False)])
+def test_rgc_lattice_matches_full_arrays(torch, M, R, S, iw, pad):
+    """npx_rgc_lattice on the compact lattice of x2M gradient arrays gives the maps of npx_rgc on the full arrays
+    bit for bit (the fused kernel gathers the same values); a radius outside the fused kernels' range is refused."""
+    from nanopyx_amd import _hip
+    print(f"rgc lattice case M{M} R{R} S{S} iw {iw} pad {pad}", flush=True)
+    N, H, W = 3, 14, 17
+    rng = np.random.default_rng(M)
+    gx = torch.from_numpy(rng.standard_normal((N, 2 * H * M, 2 * W * M)).astype(np.float32)).cuda()
+    gy = torch.from_numpy(rng.standard_normal((N, 2 * H * M, 2 * W * M)).astype(np.float32)).cuda()
+    img = torch.from_numpy((rng.random((N, H * M, W * M)) * 50 + 10).astype(np.float32)).cuda()
+    rows = [min(i * M, 2 * H * M - 1) for i in range(2 * H + 1)]
+    cols = [min(j * M, 2 * W * M - 1) for j in range(2 * W + 1)]
+    cgx, cgy = (g[:, rows][:, :, cols].contiguous() for g in (gx, gy))
+    ref = _hip.rgc(gx, gy, img, M, R, S, iw, pad)
+    got = _hip.rgc_lattice(cgx, cgy, img, M, R, S, iw, pad)
+    assert torch.equal(got, ref)
+    with pytest.raises(_hip.NpxError):
+        _hip.rgc_lattice(cgx, cgy, img, M, 3.5, S, iw, pad)
