@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
     const RowShape sh = row_shape(Ww, M, rb);
     const int FS = fft_stride(Ww);
     double2* F = lds;                  // [rb][P][fft_stride(Ww)], skewed entries
-    double2* tw = F + sh.npt;          // e^{2 pi i t / Ww}, t < Ww
+    double2* tw = F + sh.npt;          // the Stockham stages' twiddles, stage after stage (Ww - 1 of Ww entries)
     double2* tlo = tw + Ww;            // e^{2 pi i l / N}, l < 64
     double2* thi = tlo + 64;           // e^{2 pi i 64 h / N}, h < nhi
     double2* X = thi + sh.nhi;         // [rb][K + 1]
@@ -267,10 +267,24 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
     const int fb = blockIdx.x / ngrp, ybeg = (blockIdx.x % ngrp) * kRowsPerBlock;
     const int tid = threadIdx.x;
     for (int t = tid; t < Ww + 64 + sh.nhi; t += kRowThreads) {
-        double s, c;
-        if (t < Ww) sincospi(2.0 * t / Ww, &s, &c);
-        else if (t < Ww + 64) sincospi(2.0 * (t - Ww) / N, &s, &c);
-        else sincospi(2.0 * 64.0 * (t - Ww - 64) / N, &s, &c);
+        double s = 0.0, c = 1.0;
+        if (t < Ww - 1) {  // stage tables: stage (Ns, R) holds e^{2 pi i r k / (Ns R)} at (r - 1) Ns + k
+            int base = 0;
+            for (int ns = 1; ns < Ww;) {
+                const int R = ns * 4 <= Ww ? 4 : 2, cnt = (R - 1) * ns;
+                if (t < base + cnt) {
+                    const int l = t - base, r = l / ns + 1, k = l - (r - 1) * ns;
+                    sincospi(2.0 * (r * k * (Ww / (ns * R))) / Ww, &s, &c);
+                    break;
+                }
+                base += cnt;
+                ns *= R;
+            }
+        } else if (t >= Ww && t < Ww + 64) {
+            sincospi(2.0 * (t - Ww) / N, &s, &c);
+        } else if (t >= Ww + 64) {
+            sincospi(2.0 * 64.0 * (t - Ww - 64) / N, &s, &c);
+        }
         tw[t] = make_double2(c, s);
     }
     const int lgW = __builtin_ctz(Ww);
@@ -312,10 +326,11 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
         }
         // Stockham stages: Ns points already combined; radix 4 while it fits, then one radix-2 stage
         int Ns = 1, lgNs = 0;
+        const double2* tws = tw;  // this stage's table: twiddle of (r, k) at (r - 1) Ns + k
         while (Ns < Ww) {
             __syncthreads();
             if (Ns * 4 <= Ww) {
-                const int lgnb = lgW - 2, nb = 1 << lgnb, total = nf << lgnb, step = Ww / (Ns * 4);
+                const int lgnb = lgW - 2, nb = 1 << lgnb, total = nf << lgnb;
                 double2 v[3][4];
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
@@ -323,9 +338,9 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
                         const double2* in = F + f * FS;
-                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tw[k * step]),
-                                      v2 = cmul(in[sk(j + 2 * nb)], tw[2 * k * step]),
-                                      v3 = cmul(in[sk(j + 3 * nb)], tw[3 * k * step]);
+                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tws[k]),
+                                      v2 = cmul(in[sk(j + 2 * nb)], tws[Ns + k]),
+                                      v3 = cmul(in[sk(j + 3 * nb)], tws[2 * Ns + k]);
                         const double2 a0 = make_double2(v0.x + v2.x, v0.y + v2.y);
                         const double2 a1 = make_double2(v0.x - v2.x, v0.y - v2.y);
                         const double2 b0 = make_double2(v1.x + v3.x, v1.y + v3.y);
@@ -348,10 +363,11 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                         for (int r = 0; r < 4; ++r) out[sk(o + r * Ns)] = v[u][r];
                     }
                 }
+                tws += 3 * Ns;
                 Ns *= 4;
                 lgNs += 2;
             } else {
-                const int lgnb = lgW - 1, nb = 1 << lgnb, total = nf << lgnb, step = Ww / (Ns * 2);
+                const int lgnb = lgW - 1, nb = 1 << lgnb, total = nf << lgnb;
                 double2 v[6][2];
 #pragma unroll
                 for (int u = 0; u < 6; ++u) {
@@ -359,7 +375,7 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                     if (b < total) {
                         const int f = b >> lgnb, j = b & (nb - 1), k = j & (Ns - 1);
                         const double2* in = F + f * FS;
-                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tw[k * step]);
+                        const double2 v0 = in[sk(j)], v1 = cmul(in[sk(j + nb)], tws[k]);
                         v[u][0] = make_double2(v0.x + v1.x, v0.y + v1.y);
                         v[u][1] = make_double2(v0.x - v1.x, v0.y - v1.y);
                     }
@@ -376,6 +392,7 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
                         out[sk(o + Ns)] = v[u][1];
                     }
                 }
+                tws += Ns;
                 Ns *= 2;
                 lgNs += 1;
             }
