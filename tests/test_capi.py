@@ -34,6 +34,14 @@ def test_version_and_error_paths():
     assert rc == -1
     with pytest.raises(_hip.NpxError, match="radius"):
         _hip.plan_info(8, 8, 4, 5.0)  # window wider than the kernels are built for
+    # the lattice entry points of the use_fht branch refuse bad arguments before touching a device
+    p = ctypes.c_void_p(16)
+    rc = lib.npx_rgc_lattice(None, p, p, 1, 8, 8, 2, 1.5, 1.0, 1, 0, p, None)
+    assert rc == -1 and b"NULL" in lib.npx_last_error()
+    rc = lib.npx_rgc_lattice(p, p, p, 1, 8, 8, 1, 1.5, 1.0, 1, 0, p, None)
+    assert rc == -3 and b"M >= 2" in lib.npx_last_error()
+    rc = lib.npx_fht_interpolate_lattice(p, 1, 8, 8, 2, 1, 0, 16, -1, p, p, ctypes.c_size_t(1 << 20), None)
+    assert rc == -1 and b"stride" in lib.npx_last_error()
 
 
 @pytest.mark.parametrize("R,M", [(1.0, 2), (1.5, 5), (2.0, 4), (2.5, 8), (1.2345, 3)])
