@@ -59,7 +59,11 @@ def test_roberts_bit_exact(torch, oracle_built):
     assert np.array_equal(gc, rc) and np.array_equal(gr, rr)
 
 
-@pytest.mark.parametrize("M,R,S,pad_edges", [(2, 1.5, 1, False), (3, 1.0, 2, True), (5, 1.5, 1, False)])
+@pytest.mark.parametrize("M,R,S,pad_edges", [
+    (2, 1.5, 1, False), (3, 1.0, 2, True), (5, 1.5, 1, False),
+    (1, 1.5, 1, True),   # M = 1: full gradient arrays (no lattice to compact)
+    (2, 3.5, 1, True),   # radius beyond the fused kernels: compact lattice refused, full arrays and the per-pixel RGC
+])
 def test_esrrf_use_fht(torch, oracle_built, M, R, S, pad_edges):
     from nanopyx_amd.core.transform import eSRRF_ST
     from oracle import fht
