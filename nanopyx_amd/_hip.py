@@ -27,6 +27,10 @@ class NpxError(RuntimeError):
     """A libnpx_hip call failed (HIP error, unsupported parameters, or bad workspace)."""
 
 
+class NpxUnsupported(NpxError):
+    """Parameters valid for the reference but outside what these kernels build (NPX_ERR_UNSUPPORTED)."""
+
+
 def library_path() -> str:
     return _LIB_PATH
 
@@ -123,6 +127,8 @@ def _check(rc: int) -> None:
         msg = lib().npx_last_error().decode(errors="replace")
         if rc == -1:
             raise ValueError(msg)
+        if rc == -3:
+            raise NpxUnsupported(f"libnpx_hip error {rc}: {msg}")
         raise NpxError(f"libnpx_hip error {rc}: {msg}")
 
 
@@ -390,13 +396,13 @@ def lattice_shape(n, stride):
 
 def rgc_lattice(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=None):
     """rgc with the gradients given only at the lattice it reads, compact (N, 2H+1, 2W+1) (npx_rgc_lattice);
-    raises NpxError for shapes or radii only the per-pixel kernel covers."""
+    raises NpxUnsupported for shapes or radii only the per-pixel kernel covers."""
     torch = _torch()
     gx, gy, img = (_dev_stack(a).to(torch.float32).contiguous() for a in (gx, gy, img))
     N, HM, WM = img.shape
     M = int(magnification)
     if M < 2 or HM % M or WM % M:
-        raise NpxError("compact lattice needs an image of H x W pixels magnified M >= 2 times")
+        raise NpxUnsupported("compact lattice needs an image of H x W pixels magnified M >= 2 times")
     shape = (N, 2 * (HM // M) + 1, 2 * (WM // M) + 1)
     if tuple(gx.shape) != shape or tuple(gy.shape) != shape:
         raise ValueError(f"compact gradient lattices must be {shape}")

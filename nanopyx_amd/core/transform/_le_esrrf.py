@@ -39,7 +39,7 @@ def _esrrf_fht(image, M, radius, sensitivity, iw, pad_edges):
         gri = _hip.fht_interpolate(gr, 2 * M, lattice=M)
         try:
             return _hip.rgc_lattice(gci, gri, mag, M, radius, sensitivity, iw, pad_edges)
-        except _hip.NpxError:
+        except _hip.NpxUnsupported:
             del gci, gri
     gci = _hip.fht_interpolate(gc, 2 * M)
     del gc
