@@ -46,6 +46,9 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             raise NpxError(f"{_LIB_PATH} is missing: build it with `python -m nanopyx_amd._build` "
                            "(or __graft_entry__.build()); there is no CPU fallback")
+        # torch first: its bundled HIP runtime (SONAME libamdhip64.so.7) then satisfies this library's dependency,
+        # so one runtime serves both; loaded the other way round, torch would bring a second runtime instance
+        import torch  # noqa: F401
         L = ctypes.CDLL(_LIB_PATH)
         L.npx_version.restype = _c_int
         L.npx_last_error.restype = ctypes.c_char_p
