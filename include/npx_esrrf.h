@@ -23,6 +23,7 @@ extern "C" {
 #define NPX_ERR_HIP (-2)         /* HIP runtime error (launch, allocation, copy)                     */
 #define NPX_ERR_UNSUPPORTED (-3) /* valid for the reference but outside what these kernels build      */
 #define NPX_ERR_WORKSPACE (-4)   /* workspace pointer given but smaller than required                 */
+#define NPX_ERR_INTERNAL (-5)    /* an internal consistency check failed (a library bug)              */
 
 #define NPX_DTYPE_F32 0
 #define NPX_DTYPE_U16 1 /* uint16 camera counts, converted in-kernel exactly like astype(float32) */
@@ -154,6 +155,14 @@ int npx_session_submit(void *session, const void *src, int src_is_host, int dtyp
                        long long *job_id);
 /* Pool bytes npx_session_submit would need for such a job (0 for invalid parameters). */
 size_t npx_session_job_bytes(int N, int H, int W, int M, double R, int dtype, int src_is_host);
+/* Host-only check of a job's slicing: lays the job out as npx_session_submit would and slices it as npx_session_step
+ * would when it runs alone with px_frames_per_step per step, twice: stepped as each upload piece of a host stack
+ * lands (src_is_host 1; all frames at once from device memory), and stepped only after every piece has landed.
+ * info[6]: frame chunks a slice may run as (partial slots), frames per chunk, most chunks any slice ran as (never
+ * above info[0]), slices of both passes, slices not starting on a chunk boundary (0), upload pieces. No device
+ * calls. */
+int npx_session_slice_plan(int N, int H, int W, int M, double R, int dtype, int src_is_host, long long px_frames_per_step,
+                           int *info);
 int npx_session_step(void *session, void *stream, int *launches);
 int npx_session_poll(void *session, long long job_id);
 int npx_session_forget(void *session, long long job_id);

@@ -88,6 +88,8 @@ def lib():
                                          _c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.npx_session_job_bytes.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
         L.npx_session_job_bytes.restype = _c_size_t
+        L.npx_session_slice_plan.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int,
+                                             ctypes.c_longlong, _ip]
         L.npx_session_step.argtypes = [_c_void_p, _c_void_p, _ip]
         L.npx_session_poll.argtypes = [_c_void_p, ctypes.c_longlong]
         L.npx_session_forget.argtypes = [_c_void_p, ctypes.c_longlong]
@@ -142,9 +144,30 @@ def _torch():
     return torch
 
 
+_CAPTURING = set()  # raw handles of the streams Graph is capturing on
+
+
 def _stream(torch, stream=None) -> int:
     s = stream if stream is not None else torch.cuda.current_stream()
     return int(s.cuda_stream)
+
+
+def _on(torch, stream, *tensors) -> int:
+    """Raw handle of the stream a library call launches on. A caller-given stream other than the current one first
+    waits for the work queued on the current stream (the input's dtype cast or copy made by these wrappers, the
+    allocation of outputs and workspaces), and every device tensor the call reads or writes is recorded on it, so the
+    caching allocator does not hand their memory to other work before the call has finished. A stream under graph
+    capture is taken as it is: inputs are prepared before the capture."""
+    cur = torch.cuda.current_stream()
+    if stream is None:
+        return int(cur.cuda_stream)
+    h = int(stream.cuda_stream)
+    if h != int(cur.cuda_stream) and h not in _CAPTURING:
+        stream.wait_stream(cur)
+        for t in tensors:
+            if t is not None and isinstance(t, torch.Tensor) and t.is_cuda:
+                t.record_stream(stream)
+    return h
 
 
 def _dev_stack(x):
@@ -182,7 +205,7 @@ def esrrf_frames(stack, magnification, radius, sensitivity, iw, stream=None):
     M = int(magnification)
     out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
     _check(lib().npx_esrrf_frames(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
-                                  int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+                                  int(bool(iw)), out.data_ptr(), _on(torch, stream, x, out)))
     return out
 
 
@@ -208,9 +231,8 @@ def esrrf_temporal(stack, magnification, radius, sensitivity, iw, modes=("AVG",)
     ptr = lambda k: outs[k].data_ptr() if k in outs else None
     _check(lib().npx_esrrf_temporal(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
                                     int(bool(iw)), mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"),
-                                    work.data_ptr() if work is not None else None, nbytes, _stream(torch, stream)))
-    if work is not None:
-        work.record_stream(torch.cuda.current_stream() if stream is None else stream)
+                                    work.data_ptr() if work is not None else None, nbytes,
+                                    _on(torch, stream, x, work, *outs.values())))
     return outs
 
 
@@ -231,6 +253,9 @@ class SweepOutputs:
 
     def __init__(self, torch, x, magnifications, radii, sensitivities, workspace):
         N, H, W = x.shape
+        # what the pointer array and workspace were sized for: a later call reusing them must match
+        self.shape = (int(N), int(H), int(W))
+        self.lists = ([int(m) for m in magnifications], [float(r) for r in radii], [float(s) for s in sensitivities])
         self.key = [(int(m), float(r), float(s)) for m in magnifications for r in radii for s in sensitivities]
         self.maps = {}
         for (m, r, s) in self.key:
@@ -250,11 +275,21 @@ def esrrf_sweep(stack, magnifications, radii, sensitivities, iw=True, stream=Non
     x = _dev_stack(stack)
     N, H, W = x.shape
     mags, rads, sens = _sweep_lists(magnifications, radii, sensitivities)
+    if outputs is not None:
+        lists = ([int(m) for m in magnifications], [float(r) for r in radii], [float(s) for s in sensitivities])
+        if outputs.shape != (N, H, W) or outputs.lists != lists:
+            raise ValueError(f"outputs were made for a {outputs.shape} stack and settings {outputs.lists}, not "
+                             f"{(N, H, W)} and {lists}")
+        if len(outputs.ptrs) != len(mags) * len(rads) * len(sens):
+            raise ValueError("outputs hold a different number of maps than the settings need")
+        if outputs.work is not None and outputs.nbytes < sweep_workspace_bytes(N, H, W, magnifications, radii,
+                                                                                sensitivities):
+            raise ValueError("outputs' workspace is smaller than this sweep needs")
     out = outputs if outputs is not None else SweepOutputs(torch, x, magnifications, radii, sensitivities, workspace)
     _check(lib().npx_esrrf_sweep(x.data_ptr(), _dtype_code(x), N, H, W, len(mags), mags, len(rads), rads, len(sens),
                                  sens, int(bool(iw)), out.ptrs,
                                  out.work.data_ptr() if out.work is not None else None, out.nbytes,
-                                 _stream(torch, stream)))
+                                 _on(torch, stream, x, out.work, *out.maps.values())))
     return out
 
 
@@ -277,11 +312,13 @@ class Graph:
         s = _c_void_p()
         _check(lib().npx_capture_begin(ctypes.byref(s)))
         self._raw = s.value
+        _CAPTURING.add(int(self._raw))
         self.stream = torch.cuda.ExternalStream(self._raw)
         return self
 
     def __exit__(self, exc_type, exc, tb):
         g = _c_void_p()
+        _CAPTURING.discard(int(self._raw))
         rc = lib().npx_capture_end(self._raw, ctypes.byref(g))
         self.stream = None
         if exc_type is None:
@@ -346,6 +383,16 @@ class Session:
         return int(lib().npx_session_job_bytes(int(N), int(H), int(W), int(magnification), float(radius),
                                                DTYPE_U16 if dtype_u16 else DTYPE_F32, int(bool(host))))
 
+    @staticmethod
+    def slice_plan(N, H, W, magnification, radius, dtype_u16=False, host=True, px_frames_per_step=1 << 29):
+        """Host-only replay of how a lone job is laid out and sliced (npx_session_slice_plan): partial slots, frames
+        per chunk, most chunks in one slice, slices, slices off a chunk boundary, upload pieces."""
+        info = (_c_int * 6)()
+        _check(lib().npx_session_slice_plan(int(N), int(H), int(W), int(magnification), float(radius),
+                                            DTYPE_U16 if dtype_u16 else DTYPE_F32, int(bool(host)),
+                                            int(px_frames_per_step), info))
+        return dict(zip(("cmax", "cfpc", "max_chunks", "slices", "off_boundary", "pieces"), list(info)))
+
     def step(self, stream=None) -> int:
         torch = _torch()
         n = ctypes.c_int(0)
@@ -388,7 +435,7 @@ def rgc(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, stream=N
     out = torch.empty_like(img)
     _check(lib().npx_rgc(gx.data_ptr(), gy.data_ptr(), img.data_ptr(), N, HM, WM, int(magnification), float(radius),
                          float(sensitivity), int(bool(iw)), int(bool(pad_edges)), out.data_ptr(),
-                         _stream(torch, stream)))
+                         _on(torch, stream, gx, gy, img, out)))
     return out
 
 
@@ -412,7 +459,7 @@ def rgc_lattice(gx, gy, img, magnification, radius, sensitivity, iw, pad_edges, 
     out = torch.empty_like(img)
     _check(lib().npx_rgc_lattice(gx.data_ptr(), gy.data_ptr(), img.data_ptr(), N, HM, WM, M, float(radius),
                                  float(sensitivity), int(bool(iw)), int(bool(pad_edges)), out.data_ptr(),
-                                 _stream(torch, stream)))
+                                 _on(torch, stream, gx, gy, img, out)))
     return out
 
 
@@ -430,7 +477,8 @@ def temporal(stack, modes, stream=None):
         mask |= MODE_BITS[m]
         outs[m] = torch.empty(tuple(x.shape[1:]), dtype=torch.float32, device=x.device)
     ptr = lambda k: outs[k].data_ptr() if k in outs else None
-    _check(lib().npx_temporal(x.data_ptr(), N, P, mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"), _stream(torch, stream)))
+    _check(lib().npx_temporal(x.data_ptr(), N, P, mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"),
+                              _on(torch, stream, x, *outs.values())))
     return outs
 
 
@@ -440,7 +488,7 @@ def macro_pixel_(img, magnification, stream=None):
     S, HM, WM = img.shape
     M = int(magnification)
     cmap = torch.empty(S * M * M, dtype=torch.float32, device=img.device)
-    _check(lib().npx_macro_pixel(img.data_ptr(), S, HM, WM, M, cmap.data_ptr(), _stream(torch, stream)))
+    _check(lib().npx_macro_pixel(img.data_ptr(), S, HM, WM, M, cmap.data_ptr(), _on(torch, stream, img, cmap)))
     return img
 
 
@@ -460,7 +508,8 @@ def radiality(stack, interp, magnification, ringRadius, border, positivity, iw, 
     if out is None:
         out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
     _check(lib().npx_radiality(x.data_ptr(), _dtype_code(x), ip, N, H, W, M, float(ringRadius), int(border),
-                               int(bool(positivity)), int(bool(iw)), out.data_ptr(), _stream(torch, stream)))
+                               int(bool(positivity)), int(bool(iw)), out.data_ptr(),
+                               _on(torch, stream, x, interp, out)))
     return out
 
 
@@ -473,7 +522,7 @@ def shift_magnify(stack, shift_row, shift_col, mag_row, mag_col, stream=None):
     _check(lib().npx_shift_magnify_shape(H, W, float(mag_row), float(mag_col), ctypes.byref(hm), ctypes.byref(wm)))
     out = torch.empty((N, hm.value, wm.value), dtype=torch.float32, device=x.device)
     _check(lib().npx_shift_magnify(x.data_ptr(), _dtype_code(x), N, H, W, float(shift_row), float(shift_col),
-                                   float(mag_row), float(mag_col), out.data_ptr(), _stream(torch, stream)))
+                                   float(mag_row), float(mag_col), out.data_ptr(), _on(torch, stream, x, out)))
     return out
 
 
@@ -485,7 +534,7 @@ def roberts(stack, stream=None):
     gc = torch.empty((N, H, W), dtype=torch.float32, device=x.device)
     gr = torch.empty_like(gc)
     _check(lib().npx_roberts(x.data_ptr(), _dtype_code(x), N, H, W, gc.data_ptr(), gr.data_ptr(),
-                             _stream(torch, stream)))
+                             _on(torch, stream, x, gc, gr)))
     return gc, gr
 
 
@@ -509,10 +558,10 @@ def fht_interpolate(stack, magnification, mirror=True, tile_size=0, overlap=16, 
     if lattice:
         _check(lib().npx_fht_interpolate_lattice(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size),
                                                  int(overlap), int(lattice), out.data_ptr(), work.data_ptr(), need,
-                                                 _stream(torch, stream)))
+                                                 _on(torch, stream, x, out, work)))
     else:
         _check(lib().npx_fht_interpolate(x.data_ptr(), N, h, w, M, int(bool(mirror)), int(tile_size), int(overlap),
-                                         out.data_ptr(), work.data_ptr(), need, _stream(torch, stream)))
+                                         out.data_ptr(), work.data_ptr(), need, _on(torch, stream, x, out, work)))
     return out
 
 
@@ -548,7 +597,7 @@ def esrrf3d(stack, magnification_xy, magnification_z, radius, radius_z, sensitiv
     out = torch.empty((S * Mz, R * Mxy, C * Mxy), dtype=torch.float32, device=x.device)
     _check(lib().npx_esrrf3d(x.data_ptr(), _dtype_code(x), F, S, R, C, Mxy, Mz, float(radius), float(radius_z),
                              float(sensitivity), m, int(bool(iw)), out.data_ptr(), work.data_ptr(), need,
-                             _stream(torch, stream)))
+                             _on(torch, stream, x, out, work)))
     return out
 
 
@@ -638,7 +687,8 @@ def autotune(stack, magnification, radius, sensitivity, iw, modes=("AVG",), stre
     best = ctypes.c_double(0)
     _check(lib().npx_autotune(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
                               int(bool(iw)), mask, ptr("AVG"), ptr("VAR"), ptr("TAC2"),
-                              work.data_ptr() if work is not None else None, nbytes, _stream(torch, stream),
+                              work.data_ptr() if work is not None else None, nbytes,
+                              _on(torch, stream, x, work, *outs.values()),
                               ctypes.byref(best)))
     mw, tb, _ = tune_get(H, W, M, radius)
     return mw, tb, best.value

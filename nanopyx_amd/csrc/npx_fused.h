@@ -427,14 +427,26 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     // kPrefetch elements of each thread come through registers loaded a frame ahead. The tile of frame f + 1 is
     // stored while frame f's ring sums run (its last reader, the intensity row pass, is behind a barrier by then).
     float pf[kPrefetch];
-    auto fetch = [&](int f) {
+    // in-frame element offsets of the prefetched tile elements (-1: outside the image or the tile), fixed for the
+    // block; a frame's loads add them to the frame's base address (one VGPR per element instead of a 64-bit address)
+    int pf_off[kPrefetch];
 #pragma unroll
-        for (int k = 0; k < kPrefetch; ++k) {
-            const int idx = tid + k * nthr;
-            if (idx < TIR * TIC) {
-                const int r = idx / TIC, c = idx - r * TIC;
-                pf[k] = load_px(a, f, y0 - HI + r, x0 - HI + c);
-            }
+    for (int k = 0; k < kPrefetch; ++k) {
+        const int idx = tid + k * nthr;
+        const int r = idx / TIC, c = idx - r * TIC;
+        const int gy = y0 - HI + r, gx = x0 - HI + c;
+        pf_off[k] = (idx < TIR * TIC && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) ? gy * a.W + gx : -1;
+    }
+    auto fetch = [&](int f) {
+        const size_t fb = static_cast<size_t>(f) * a.H * a.W;
+        if (a.u16) {
+            const uint16_t* base = static_cast<const uint16_t*>(a.in) + fb;
+#pragma unroll
+            for (int k = 0; k < kPrefetch; ++k) pf[k] = pf_off[k] >= 0 ? static_cast<float>(base[pf_off[k]]) : 0.f;
+        } else {
+            const float* base = static_cast<const float*>(a.in) + fb;
+#pragma unroll
+            for (int k = 0; k < kPrefetch; ++k) pf[k] = pf_off[k] >= 0 ? base[pf_off[k]] : 0.f;
         }
     };
     auto store_tile = [&](int f) {

@@ -223,6 +223,19 @@ int admit(Session& s) {
     return NPX_OK;
 }
 
+// Frames of the next slice of job j given `left` output pixel-frames of the step's budget: at least
+// kMinSliceFrames, whole chunks when the slice runs as chunks (chunk boundaries fixed per job), at most cmax chunks
+// (the job's partials hold cmax chunk slots), and no more than the frames already on the device.
+long long slice_frames(const Job& j, long long left) {
+    const long long P = static_cast<long long>(j.plan->HM) * j.plan->WM;
+    long long f = std::max<long long>(kMinSliceFrames, left / P);
+    if (j.cmax > 1) {
+        f = std::max<long long>(1, f / j.cfpc) * j.cfpc;
+        f = std::min<long long>(f, static_cast<long long>(j.cmax) * j.cfpc);
+    }
+    return std::min<long long>(j.arrived - j.next_frame, f);
+}
+
 // One step: retire, admit, then one frame slice of every runnable job in batched launches on `st`.
 int step(Session& s, hipStream_t st, int* nlaunch) {
     *nlaunch = 0;
@@ -268,9 +281,7 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
         const bool small = remaining(&j) <= small_cut;
         if (!small && left <= 0) break;
         const long long P = static_cast<long long>(j.plan->HM) * j.plan->WM;
-        long long f = std::max<long long>(kMinSliceFrames, left / P);
-        if (j.cmax > 1) f = std::max<long long>(1, f / j.cfpc) * j.cfpc;  // whole chunks
-        f = std::min<long long>(j.arrived - j.next_frame, f);  // frames on the device (pieces end on chunk bounds)
+        const long long f = slice_frames(j, left);
         frames.push_back(static_cast<int>(f));
         left -= f * P;
         keep = k + 1;
@@ -317,6 +328,7 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
                 // frame chunks in parallel, each from a fresh start into its partial slot; merged after the launch
                 mg.fpc = j.cfpc;
                 mg.nchunk = (nf + mg.fpc - 1) / mg.fpc;
+                if (mg.nchunk > j.cmax) return fail(NPX_ERR_INTERNAL, "session slice has more chunks than partial slots");
                 a.fpc = mg.fpc;
                 a.nchunk = mg.nchunk;
                 a.part = reinterpret_cast<float*>(s.pool.at(j.off) + j.part_at);
@@ -481,6 +493,60 @@ size_t npx_session_job_bytes(int N, int H, int W, int M, double R, int dtype, in
     npx::host::pack_tables(*plan, &j.blob);
     npx::serve::layout(j, *plan);
     return npx::serve::align_up(j.bytes);
+}
+
+int npx_session_slice_plan(int N, int H, int W, int M, double R, int dtype, int src_is_host, long long px_frames_per_step,
+                           int* info) {
+    if (!info || N < 1 || H < 1 || W < 1 || M < 1 || M > 64 || px_frames_per_step < 1)
+        return npx::host::fail(NPX_ERR_ARG, "bad arguments");
+    const npx::PlanHost* plan = nullptr;
+    int rc = npx::host::host_plan(H, W, M, R, &plan);
+    if (rc) return rc;
+    Job j;
+    j.N = N;
+    j.H = H;
+    j.W = W;
+    j.M = M;
+    j.dtype = dtype;
+    j.src_host = src_is_host != 0;
+    j.plan = plan;
+    npx::host::pack_tables(*plan, &j.blob);
+    npx::serve::layout(j, *plan);
+    // a lone job whose frames arrive as admit() cuts the upload (every frame at once from device memory), given
+    // the whole budget each step until done
+    std::vector<int> ends;
+    if (j.src_host) {
+        const size_t fb = static_cast<size_t>(H) * W * (dtype == NPX_DTYPE_U16 ? 2 : 4);
+        int pf = static_cast<int>(std::max<size_t>(npx::serve::kMinSliceFrames, npx::serve::kPieceBytes / fb));
+        if (j.cmax > 1) pf = std::max(1, pf / j.cfpc) * j.cfpc;
+        for (int f0 = 0; f0 < N; f0 += pf) ends.push_back(std::min(N, f0 + pf));
+    } else {
+        ends.push_back(N);
+    }
+    // twice: stepped as each piece lands, and stepped only once the whole upload has landed
+    int slices = 0, max_chunks = 0, bad_bounds = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        j.next_frame = 0;
+        for (size_t k = pass ? ends.size() - 1 : 0; k < ends.size(); ++k) {
+            j.arrived = ends[k];
+            while (j.next_frame < j.arrived) {
+                const long long f = npx::serve::slice_frames(j, px_frames_per_step);
+                if (f < 1) return npx::host::fail(NPX_ERR_INTERNAL, "empty slice");
+                const int nchunk = j.cmax > 1 ? static_cast<int>((f + j.cfpc - 1) / j.cfpc) : 1;
+                max_chunks = std::max(max_chunks, nchunk);
+                if (j.cmax > 1 && j.next_frame % j.cfpc) ++bad_bounds;  // slices start on the job's chunk boundaries
+                j.next_frame += static_cast<int>(f);
+                ++slices;
+            }
+        }
+    }
+    info[0] = j.cmax;
+    info[1] = j.cfpc;
+    info[2] = max_chunks;
+    info[3] = slices;
+    info[4] = bad_bounds;
+    info[5] = static_cast<int>(ends.size());
+    return NPX_OK;
 }
 
 int npx_session_step(void* session, void* stream, int* launches) {

@@ -296,3 +296,26 @@ def test_rgc_arrays_paths(torch, oracle_built, N, HM, WM, M, R, S, iw, pad):
     out = RadialGradientConvergence(verbose=False).run(gx, gy, img, magnification=M, radius=R, sensitivity=S,
                                                        doIntensityWeighting=iw, pad_edges=pad)
     assert _report(f"rgc arrays {HM}x{WM} M{M}", out, ref) <= TOL
+
+
+def test_explicit_stream_waits_for_input_prep(torch):
+    """A call on a caller-given stream (not the current one) with an input that the wrapper must cast and upload
+    first: the kernels wait for that work, and the maps equal those of the same call on the current stream, bit for
+    bit. The temporaries are recorded on the given stream, so freeing them right away is safe."""
+    from nanopyx_amd import _hip
+    rng = np.random.default_rng(3)
+    x64 = rng.gamma(4.0, 30.0, size=(24, 40, 56))  # float64 on the host: cast and uploaded by the wrapper
+    ref_t = _hip.esrrf_temporal(x64, 3, 1.5, 2, True, ("AVG", "TAC2"))
+    ref_f = _hip.esrrf_frames(x64, 3, 1.5, 2, True)
+    ref_r = _hip.radiality(x64, None, 3, 1.0, 0, True, True)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    for _ in range(3):
+        got_t = _hip.esrrf_temporal(x64, 3, 1.5, 2, True, ("AVG", "TAC2"), stream=s)
+        got_f = _hip.esrrf_frames(x64, 3, 1.5, 2, True, stream=s)
+        got_r = _hip.radiality(x64, None, 3, 1.0, 0, True, True, stream=s)
+        torch.cuda.current_stream().wait_stream(s)
+        for k in ("AVG", "TAC2"):
+            assert torch.equal(got_t[k], ref_t[k]), k
+        assert torch.equal(got_f, ref_f) and torch.equal(got_r, ref_r)
+    torch.cuda.synchronize()

@@ -175,3 +175,48 @@ def test_host_stack_in_pieces(torch, N, H, W, M, R, mode, steps):
     assert np.array_equal(got[0], got[1]), "host-source maps differ from device-source maps"
     plain = _hip.esrrf_temporal(x, M, R, 1, True, (mode,), workspace=False)[mode].cpu().numpy()
     assert _report(f"host pieces {mode} vs plain", got[0], plain) <= 1e-6
+
+
+@pytest.mark.parametrize("N,H,W,M,mode", [
+    (300, 384, 384, 2, "TAC2"),   # 177 MB host stack in 6 pieces; 4 partial slots of 56 frames
+    (200, 512, 512, 3, "AVG"),    # 210 MB; the whole stack arrives before the first slice
+])
+def test_host_stack_stepped_after_upload(torch, N, H, W, M, mode):
+    """A host stack whose upload has fully landed before the session's next step: the step may take every frame at
+    once, and its slice must still run no more frame chunks than the job has partial slots (a longer slice would
+    write partial moments past its slots, into the job's own stack). Maps equal the device-source job and
+    the plain fused call (<= 1e-6); bit for bit when both have the same frame chunks."""
+    from nanopyx_amd import _hip, synthetic
+    print(f"host stack stepped after upload N{N} {H}x{W} M{M} {mode}", flush=True)
+    plan = _hip.Session.slice_plan(N, H, W, M, 1.5, False, True)
+    assert plan["max_chunks"] <= plan["cmax"]
+    sc = synthetic.Scene(N, H, W, max(1, int(0.02 * H * W)), 1.5, 500, 0.3, 100, 0.0, 2.0, 0.0, "float32", 78)
+    x = synthetic.generate_device(sc)
+    h = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    h.copy_(x)
+    torch.cuda.synchronize()
+    maps = []
+    for src in (h, x):
+        sess = _hip.Session(2 << 30, 4, 1 << 29)
+        out = {mode: torch.full((H * M, W * M), -7.0, device="cuda")}
+        jid = sess.submit(src, M, 1.5, 1, True, (mode,), out)
+        sess.step()                 # admits the job and queues its upload
+        torch.cuda.synchronize()    # every piece has landed before the next step
+        for _ in range(10000):
+            sess.step()
+            if sess.poll(jid) == _hip.Session.DONE:
+                break
+        else:
+            raise AssertionError("session did not finish")
+        torch.cuda.synchronize()
+        print("session stats", sess.stats(), flush=True)
+        sess.close()
+        maps.append(out[mode].cpu().numpy())
+    # the upload pieces may shorten a host job's chunks (different merge points, so different float32 rounding of the
+    # merged moments); with the same chunks the maps are the same bit for bit
+    dev_plan = _hip.Session.slice_plan(N, H, W, M, 1.5, False, False)
+    if dev_plan["cfpc"] == plan["cfpc"]:
+        assert np.array_equal(maps[0], maps[1]), "host-source maps differ from device-source maps"
+    assert _report(f"host after upload {mode} vs device source", maps[0], maps[1]) <= 1e-6
+    plain = _hip.esrrf_temporal(x, M, 1.5, 1, True, (mode,), workspace=False)[mode].cpu().numpy()
+    assert _report(f"host after upload {mode} vs plain", maps[0], plain) <= 1e-6

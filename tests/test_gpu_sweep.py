@@ -122,6 +122,22 @@ def test_sweep_errors(torch):
         _hip.esrrf_sweep(x, [2], [3.6], [1.0])  # ring half-width 7 > 6
 
 
+def test_sweep_outputs_reuse_checked(torch):
+    """Outputs (map pointers, workspace) made for one stack shape and setting list are refused for another before
+    anything launches."""
+    from nanopyx_amd import _hip
+    x = torch.from_numpy(_stack((12, 16, 24))).cuda()
+    outs = _hip.SweepOutputs(torch, x, [2], [1.0, 1.5], [1.0, 2.0], workspace=True)
+    _hip.esrrf_sweep(x, [2], [1.0, 1.5], [1.0, 2.0], outputs=outs)
+    with pytest.raises(ValueError, match="outputs were made"):
+        _hip.esrrf_sweep(torch.from_numpy(_stack((24, 16, 24))).cuda(), [2], [1.0, 1.5], [1.0, 2.0], outputs=outs)
+    with pytest.raises(ValueError, match="outputs were made"):
+        _hip.esrrf_sweep(x, [2], [1.0, 1.5, 2.0], [1.0, 2.0], outputs=outs)
+    with pytest.raises(ValueError, match="outputs were made"):
+        _hip.esrrf_sweep(x, [2, 4], [1.0, 1.5], [1.0, 2.0], outputs=outs)
+    torch.cuda.synchronize()
+
+
 def test_parameter_sweep_class(torch, oracle_built):
     from nanopyx_amd.core.analysis import ParameterSweep
     x = _stack((5, 16, 20), seed=8)
