@@ -66,6 +66,19 @@ int npx_esrrf_temporal(const void *d_in, int dtype, int N, int H, int W, int M, 
                        void *stream);
 
 /*
+ * npx_esrrf_temporal without any device allocation, for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. The geometry tables and
+ * the frame-chunk partial moments live in a static device array of 64 MiB (one per GPU, held until the process
+ * ends); the tables are copied into it on `stream` by each call (hipMemcpyAsync from host memory). The frames are
+ * split as npx_esrrf_temporal splits them when its partials fit beside the tables, otherwise into as many chunks as
+ * fit (maps then differ from npx_esrrf_temporal's only in the float32 rounding of the chunk merge). Image sizes
+ * whose tables exceed the array are refused (NPX_ERR_ARG). Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race and must be serialised by the caller.
+ */
+int npx_esrrf_temporal_static(const void *d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                              unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2, void *stream);
+
+/*
  * RadialGradientConvergence on caller arrays: d_gx, d_gy (N, 2*HM, 2*WM), d_img (N, HM, WM), indexed exactly as
  * _c_calculate_rgc does (src/include/_c_sr_radial_gradient_convergence.c:17-90). When HM and WM are multiples of M
  * (and the radius is within the eSRRF kernels' range) the fused eSRRF kernel runs on the lattice the reference reads

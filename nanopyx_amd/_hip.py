@@ -60,6 +60,9 @@ def lib():
         L.npx_esrrf_temporal.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_double,
                                          _c_int, ctypes.c_uint, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                          _c_size_t, _c_void_p]
+        L.npx_esrrf_temporal_static.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double,
+                                                _c_double, _c_int, ctypes.c_uint, _c_void_p, _c_void_p, _c_void_p,
+                                                _c_void_p]
         L.npx_rgc.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_double,
                               _c_double, _c_int, _c_int, _c_void_p, _c_void_p]
         L.npx_rgc_lattice.argtypes = L.npx_rgc.argtypes

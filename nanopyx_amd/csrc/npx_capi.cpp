@@ -40,6 +40,7 @@ struct DevCopy {
 struct Plan {
     npx::PlanHost host;
     std::map<int, DevCopy> dev;  // per device id
+    TableBlob blob;              // host copy of the packed tables, for npx_esrrf_temporal_static (built on its first use)
 };
 
 using Key = std::tuple<int, int, int, uint64_t>;
@@ -397,6 +398,35 @@ int tables_for(int H, int W, int M, double R, const PlanHost** h, const DevTable
     return NPX_OK;
 }
 
+// ------------------------------------------------------------------------------------------ static scratch
+
+// Device memory of npx_esrrf_temporal_static: its geometry tables and frame-chunk partials, for callers that cannot
+// allocate. Every GPU the library runs on holds one copy until the process ends; its address on a device comes from
+// hipGetSymbolAddress with that device current (in host code the array's name is not a device address).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device id -> address of `scratch` on that device
+
+// Address of the current device's scratch, looked up on the first call on each device.
+int scratch_on_device(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(NPX_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return NPX_OK;
+}
+
 }  // namespace host
 }  // namespace npx
 
@@ -507,6 +537,72 @@ int npx_esrrf_temporal(const void* d_in, int dtype, int N, int H, int W, int M, 
         if (e != hipSuccess) return hip_fail(e, "moments_merge launch");
     }
     if (timed && hipEventRecord(ev.e2, st) == hipSuccess) timing_end(ev);
+    return NPX_OK;
+}
+
+int npx_esrrf_temporal_static(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                              unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2, void* stream) {
+    int rc = check_common(d_in, dtype, N, H, W, M);
+    if (rc) return rc;
+    if ((mode_mask & (NPX_AVG | NPX_VAR | NPX_TAC2)) == 0) return fail(NPX_ERR_ARG, "empty temporal mode mask");
+    if (((mode_mask & NPX_AVG) && !d_avg) || ((mode_mask & NPX_VAR) && !d_var) || ((mode_mask & NPX_TAC2) && !d_tac2))
+        return fail(NPX_ERR_ARG, "output pointer of a requested temporal mode is NULL");
+    // host plan and its packed tables (host memory only)
+    const TableBlob* blob = nullptr;
+    const npx::PlanHost* h = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        Plan* p = get_plan(H, W, M, R);
+        if (!p) return NPX_ERR_UNSUPPORTED;
+        if (p->blob.bytes.empty()) pack_tables(p->host, &p->blob);
+        blob = &p->blob;
+        h = &p->host;
+    }
+    const size_t tab = align256(blob->bytes.size());
+    if (tab > kScratchBytes)
+        return fail(NPX_ERR_ARG, "geometry tables of this image size (" + std::to_string(tab) +
+                                     " bytes) exceed the static scratch of npx_esrrf_temporal_static (" +
+                                     std::to_string(kScratchBytes) + " bytes)");
+    // the frame chunks npx_esrrf_temporal would run, as many as the scratch holds partials for (each 5 HM WM floats)
+    const LaunchCfg c = launch_cfg(H, W, M, R);
+    const Shape sh = fused_shape(H, W, M, c);
+    int nchunk = 1, fpc = N;
+    temporal_chunks(N, sh, c.target_blocks, &nchunk, &fpc);
+    const size_t P = static_cast<size_t>(h->HM) * h->WM;
+    const size_t slot = 5 * P * sizeof(float);
+    const long long fit = static_cast<long long>((kScratchBytes - tab) / slot);
+    if (nchunk > 1 && nchunk > fit) {
+        if (fit < 2) {
+            nchunk = 1;
+            fpc = N;
+        } else {
+            fpc = static_cast<int>((N + fit - 1) / fit);
+            nchunk = (N + fpc - 1) / fpc;
+        }
+    }
+    void* base = nullptr;
+    rc = scratch_on_device(&base);
+    if (rc) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // tables into the scratch on the call's stream (stream-ordered before the kernels that read them)
+    hipError_t e = hipMemcpyAsync(base, blob->bytes.data(), blob->bytes.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "tables into the static scratch");
+    const npx::DevTables t = blob->tables_in(base);
+    npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, *h, &t, sh, S, iw);
+    a.fpc = fpc;
+    a.nchunk = nchunk;
+    a.direct = nchunk == 1;
+    a.avg = (mode_mask & NPX_AVG) ? d_avg : nullptr;
+    a.var = (mode_mask & NPX_VAR) ? d_var : nullptr;
+    a.tac2 = (mode_mask & NPX_TAC2) ? d_tac2 : nullptr;
+    a.part = nchunk > 1 ? reinterpret_cast<float*>(static_cast<uint8_t*>(base) + tab) : nullptr;
+    const int mode = a.var ? npx::kModeTemporalVar : npx::kModeTemporal;
+    e = npx::launch_fused(a, dim3(sh.nregx, sh.nregy, sh.ngroups * nchunk), dim3(64 * sh.nwaves), mode, st);
+    if (e != hipSuccess) return hip_fail(e, "esrrf_fused(temporal, static scratch) launch");
+    if (nchunk > 1) {
+        e = npx::launch_merge(a.part, nchunk, fpc, N, P, a.avg, a.var, a.tac2, st);
+        if (e != hipSuccess) return hip_fail(e, "moments_merge launch");
+    }
     return NPX_OK;
 }
 

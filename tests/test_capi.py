@@ -80,6 +80,24 @@ def test_plan_ring_tables_against_fp64_geometry():
                         assert abs(tt - dw / d) <= 2e-5 * dw / d
 
 
+def test_temporal_static_argument_errors():
+    """npx_esrrf_temporal_static checks its arguments and its scratch limit on the host, before any device call."""
+    lib = _hip.lib()
+    p = ctypes.c_void_p(16)
+    rc = lib.npx_esrrf_temporal_static(None, 0, 4, 8, 8, 2, 1.5, 1.0, 1, 1, p, None, None, None)
+    assert rc == -1 and b"NULL" in lib.npx_last_error()
+    rc = lib.npx_esrrf_temporal_static(p, 0, 4, 8, 8, 2, 1.5, 1.0, 1, 0, p, None, None, None)
+    assert rc == -1 and b"mode mask" in lib.npx_last_error()
+    rc = lib.npx_esrrf_temporal_static(p, 0, 4, 8, 8, 2, 1.5, 1.0, 1, 4, p, None, None, None)  # TAC2 output NULL
+    assert rc == -1
+    rc = lib.npx_esrrf_temporal_static(p, 0, 4, 8, 8, 2, 5.0, 1.0, 1, 1, p, None, None, None)  # radius too wide
+    assert rc == -3
+    # tables of a 30000 x 30000 image at M 64 (taps of 1.92 M output rows and as many columns, ~79 MB) exceed the
+    # 64 MiB scratch
+    rc = lib.npx_esrrf_temporal_static(p, 0, 1, 30000, 30000, 64, 1.0, 1.0, 1, 1, p, None, None, None)
+    assert rc == -1 and b"static scratch" in lib.npx_last_error()
+
+
 def test_lattice_taps_are_catmull_rom_at_half_integers():
     t = _hip.plan_tables(9, 9, 3, 1.5)
     for a in range(1, 18):
