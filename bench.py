@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 METRIC = "eSRRF output Mpx·frames/s vs plain GPU & CPU; sweep speedup; served p50/p99"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# the fused kernels are bound by FP32 vector ALU issue (a stencil: no MFMA on this path)
+BOUND = "fp32-valu"
 
 
 def log(*a):
@@ -99,16 +101,29 @@ def sweep_bench(reps):
             "frac_fp32_peak": round(flops / k_s / 1e12 / FP32_PEAK_TFLOPS, 4)}
 
 
-SRRF_FLOPS_PER_PX = 12 * 95 + 43   # per output pixel-frame: 12 ring samples x 95, + 43 per pixel (see srrf_bench)
+# per output pixel-frame, reference-equivalent: 12 ring samples x 95 (two direct 16-tap Catmull-Rom gradient
+# interpolations of 40 flops each + the convergence term 15), + 43 per pixel (see srrf_bench)
+SRRF_REF_FLOPS_PER_PX = 12 * 95 + 43
+
+
+def srrf_flops_per_px(H, W, M, ringRadius, border=0):
+    """FP32 flops the separable SRRF kernel executes per output pixel-frame: the row pass interpolates every
+    gradient-tile row (TH per band of TJ output rows) at each of the 12 sample x positions (4 taps x 2 gradient
+    components), the column pass combines 4 of those rows per sample (4 x 2) and forms the convergence term (15), and
+    each pixel adds 43 (ring mean and clamp, 16-tap intensity interpolation, weight)."""
+    from nanopyx_amd import _hip
+    d = _hip.radiality_info(H, W, M, ringRadius, border)
+    return 12 * d["TH"] * 4 * 2 * 2 / d["TJ"] + 12 * (4 * 2 * 2 + 15) + 43
 
 
 def srrf_bench(reps, cpu_frames=2, workload="srrf_256"):
     """SRRF radiality (npx_radiality: gradients, 12-sample Catmull-Rom ring, in-kernel xM intensity weighting) over
     the srrf_256 stack, per-frame maps out. Kernel time from HIP events on the launch stream over `reps` launches.
-    Algorithmic flops per output pixel-frame: per ring sample two 16-tap Catmull-Rom interpolations (4x4 separable:
-    20 multiply-adds = 40 flops each) and the convergence term (|G|^2, cross and dot products, Dk, accumulate: 15),
-    12 x 95; per pixel the ring mean and clamp (2), the 16-tap intensity interpolation (40) and the weight (1).
-    The CPU baseline is the oracle's restatement of the reference's threaded radiality on `cpu_frames` frames."""
+    `frac` counts the flops the separable kernel executes (srrf_flops_per_px); `frac_reference_equivalent` counts
+    the reference's direct evaluation: per ring sample two 16-tap Catmull-Rom interpolations (40 flops each) and the
+    convergence term (|G|^2, cross and dot products, Dk, accumulate: 15), 12 x 95; per pixel the ring mean and clamp
+    (2), the 16-tap intensity interpolation (40) and the weight (1). The CPU baseline is the reference's own compiled
+    radiality and Catmull-Rom kernels (oracle/_ref) driven like its threaded SRRF() on `cpu_frames` frames."""
     import torch
     from nanopyx_amd import _hip, synthetic
     sc = synthetic.CONFIGS[workload]
@@ -135,27 +150,38 @@ def srrf_bench(reps, cpu_frames=2, workload="srrf_256"):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     U = N * H * M * W * M
+    SRRF_FLOPS_PER_PX = srrf_flops_per_px(H, W, M, R, e["border"])
     flops = U * float(SRRF_FLOPS_PER_PX)
     achieved = flops / (ms / 1e3) / 1e12
+    ref_achieved = U * float(SRRF_REF_FLOPS_PER_PX) / (ms / 1e3) / 1e12
     res = {"workload": f"{workload}: {N} frames x {H}x{W} float32, SRRF M{M} ringRadius {R}, positivity, "
                        f"intensity-weighted, per-frame maps {H * M}x{W * M}",
            "kernel_ms": round(ms, 4), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mpx·frames/s",
-           "roofline": {"bound": "mfma", "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
+           "roofline": {"bound": BOUND, "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                         "flops_per_launch": flops, "flops_per_px": SRRF_FLOPS_PER_PX,
+                        "flops_per_px_reference_equivalent": SRRF_REF_FLOPS_PER_PX,
+                        "frac_reference_equivalent": round(ref_achieved / FP32_PEAK_TFLOPS, 4),
                         "algorithmic_bytes": N * H * W * 4 + U * 4,
                         "hbm_gbs_algorithmic": round((N * H * W * 4 + U * 4) / (ms / 1e3) / 1e9, 1)}}
     if cpu_frames:
         from oracle import ref32
         ref32.build()
+        kind = "port"
+        if ref32.reference_srrf_available() and ref32.reference_available():
+            ref32.use_reference_radiality(True)
+            ref32.use_reference_kernels(True)  # the Catmull-Rom magnification of the intensity
+            kind = "reference"
         xs = x_host[:cpu_frames]
         t0 = time.perf_counter()
         ref = ref32.srrf_workflow(xs, M, R, e["border"], e["radialityPositivityConstraint"],
                                   e["doIntensityWeighting"], macro_pixel_correction=False)
         dt = time.perf_counter() - t0
+        ref32.use_reference_radiality(False)
+        ref32.use_reference_kernels(False)
         Us = cpu_frames * H * M * W * M
         res["cpu_baseline"] = {"value": round(Us / 1e6 / dt, 2), "unit": "Mpx·frames/s",
-                               "cores": ref32.num_threads(), "kind": "port",
+                               "cores": ref32.num_threads(), "kind": kind,
                                "sample": f"first {cpu_frames} frames (CR xM + radiality)"}
         got = out[:cpu_frames].cpu().numpy()
         err = np.abs(got - ref).max() / np.abs(ref).max()
@@ -210,18 +236,25 @@ def esrrf3d_bench(reps, cpu_frames=1):
     res = {"workload": f"esrrf3d: {F} frames x {S}x{R}x{C} float32, Mxy {Mxy} Mz {Mz}, radius {rad} "
                        f"(radius_z {rad_z:.3f}), sensitivity 1, average -> {S * Mz}x{R * Mxy}x{C * Mxy}",
            "ms_per_call": round(ms, 4), "value": round(U / 1e6 / (ms / 1e3), 1), "unit": "Mvoxel·frames/s",
-           "roofline": {"bound": "mfma", "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
+           "roofline": {"bound": BOUND, "compute": "FP32 vector ALU", "achieved": round(achieved, 2),
                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                         "flops_per_call": flops, "samples_mean": s_bar}}
     if cpu_frames:
         from oracle import ref32
         ref32.build()
+        kind = "port"
+        if ref32.reference_available():
+            ref32.use_reference_3d(True)
+            ref32.use_reference_kernels(True)  # the xy Catmull-Rom magnification
+            kind = "reference"
         t0 = time.perf_counter()
         ref = ref32.esrrf3d(x_host[:cpu_frames], Mxy, Mz, rad, 2.8, 4.0, 1, "average", True)
         dt = time.perf_counter() - t0
+        ref32.use_reference_3d(False)
+        ref32.use_reference_kernels(False)
         Us = cpu_frames * S * Mz * R * Mxy * C * Mxy
         res["cpu_baseline"] = {"value": round(Us / 1e6 / dt, 3), "unit": "Mvoxel·frames/s",
-                               "cores": ref32.num_threads(), "kind": "port",
+                               "cores": ref32.num_threads(), "kind": kind,
                                "sample": f"first {cpu_frames} frame(s) of the same stack"}
         got = _hip.esrrf3d(x[:cpu_frames], Mxy, Mz, rad, rad_z, 1.0, "average", True).cpu().numpy()
         res["parity_max_rel_err"] = float(np.abs(got - ref).max() / np.abs(ref).max())
@@ -375,6 +408,69 @@ def parity_on_sample(x_dev, ref_maps, e):
     return out
 
 
+def config_leg(workload, reps, cpu_seconds, full_length_parity=False):
+    """A plain-eSRRF workload of the benchmark set other than the headline, measured as the headline is: the fused
+    eSRRF + temporal call over the whole stack resident in HBM (host wall clock per call, and the HIP-event time of
+    the fused kernel and of the moment merge on the launch stream, over `reps` calls), the fused kernel's FP32
+    roofline, the reference CPU chain (the reference's own C kernels in oracle/_ref) on a bounded frame sample of the
+    same stack, and parity of every temporal map against NumPy's reduction of the reference's per-frame maps on the
+    frames the CPU processed; with full_length_parity also against NumPy's reduction of the GPU per-frame maps of
+    every frame of the stack (the drift of the streamed moments at full length)."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    from oracle import ref32
+    sc = synthetic.CONFIGS[workload]
+    e = sc.esrrf
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    modes = tuple(e["temporal"])
+    x_host = synthetic.generate(sc)
+    x = torch.from_numpy(x_host).cuda()
+    N, H, W = x.shape
+    for _ in range(2):
+        _hip.esrrf_temporal(x, M, R, S, iw, modes)
+    torch.cuda.synchronize()
+    _hip.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _hip.esrrf_temporal(x, M, R, S, iw, modes)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    _hip.timing(False)
+    n, k_ms, m_ms = _hip.timing_read()
+    k_ms, m_ms = k_ms / max(n, 1), m_ms / max(n, 1)
+    U = N * H * M * W * M
+    s_bar = ring_samples_mean(H, W, M, R)
+    flops = U * (9.0 * s_bar + 40.0)
+    achieved = flops / (k_ms / 1e3) / 1e12
+    res = {"workload": f"{workload}: {N} frames x {H}x{W} {sc.dtype}, M{M} R{R} S{S} intensity-weighted, "
+                       f"{'+'.join(modes)} -> {H * M}x{W * M}",
+           "ms_per_call": round(dt * 1e3, 4), "kernel_ms": round(k_ms, 4), "merge_ms": round(m_ms, 4),
+           "value": round(U / 1e6 / dt, 1), "value_kernels": round(U / 1e6 / ((k_ms + m_ms) / 1e3), 1),
+           "unit": "Mpx·frames/s",
+           "roofline": {"bound": BOUND, "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops,
+                        "ring_samples_mean": s_bar}}
+    cpu, ref_maps = cpu_baseline(x_host, e, cpu_seconds)
+    res["cpu_baseline"] = cpu
+    res["parity"] = parity_on_sample(x, ref_maps, e)
+    del ref_maps
+    if full_length_parity:
+        ref32.build()
+        fused = _hip.esrrf_temporal(x, M, R, S, iw, modes)
+        frames = _hip.esrrf_frames(x, M, R, S, iw).cpu().numpy()
+        full = {"frames": int(N), "against": "NumPy float32 reduction of the GPU per-frame maps of every frame"}
+        for mode in modes:
+            ref = ref32.temporal(frames, mode).astype(np.float64)
+            g = fused[mode].cpu().numpy().astype(np.float64)
+            err = np.abs(g - ref) / max(np.abs(ref).max(), 1e-30)
+            full[mode] = {"max": float(err.max()), "mean": float(err.mean())}
+        res["parity_full_length"] = full
+        del frames
+    del x
+    torch.cuda.empty_cache()
+    return res
+
+
 def end_to_end(x_host, e, reps=2):
     """Host stack -> device copy -> fused eSRRF + temporal -> maps back on the host, wall clock per call."""
     import torch
@@ -446,6 +542,8 @@ def main():
     ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
     ap.add_argument("--no-esrrf3d", action="store_true")
     ap.add_argument("--no-fht", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the cpu_small and plain_256 legs")
+    ap.add_argument("--configs-only", action="store_true", help="run only the cpu_small and plain_256 legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one plain 1 GiB device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fht-only", action="store_true", help="run only the use_fht leg and print its JSON")
@@ -474,6 +572,10 @@ def main():
         return
     if args.fht_only:
         print(json.dumps(fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)), flush=True)
+        return
+    if args.configs_only:
+        print(json.dumps({"cpu_small": config_leg("cpu_small", max(5, args.steps), 10.0),
+                          "plain_256": config_leg("plain_256", max(5, args.steps), 10.0, True)}), flush=True)
         return
     if args.esrrf3d_only:
         print(json.dumps(esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)), flush=True)
@@ -551,7 +653,7 @@ def main():
                                f"intensity-weighted, {'+'.join(modes)} -> {HM}x{WM}",
                    "frames": N, "height": H, "width": W, "magnification": M, "radius": R, "sensitivity": S,
                    "temporal": list(modes), "input_dtype": sc.dtype},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": BOUND, "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profile(args.workload),
                      "compute": "FP32 vector ALU (no MFMA on this stencil path)",
                      "flops_per_launch": flops, "ring_samples_mean": s_bar,
@@ -560,12 +662,19 @@ def main():
                      "hbm_gbs_algorithmic": round(alg_bytes / k_s / 1e9, 1)},
         "launch_shape": tune_info,
         "cpu_baseline": None,
+        "configs": None,
         "sweep": None,
         "serving": None,
         "srrf": None,
         "esrrf3d": None,
         "use_fht": None,
     }
+    if not args.no_configs:
+        log("cpu_small and plain_256 workloads ...")
+        result["configs"] = {"cpu_small": config_leg("cpu_small", max(5, args.steps), 0.0 if args.no_cpu else 10.0),
+                             "plain_256": config_leg("plain_256", max(5, args.steps), 0.0 if args.no_cpu else 10.0,
+                                                     not args.no_cpu)}
+        log(f"configs: {result['configs']}")
     if not args.no_sweep:
         log("parameter sweep ...")
         result["sweep"] = sweep_bench(max(3, args.steps // 2))
