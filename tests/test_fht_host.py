@@ -1,4 +1,7 @@
-"""The oracle's restatement of the use_fht Fourier interpolation (oracle/fht.py), pinned by closed forms."""
+"""The oracle's restatement of the use_fht Fourier interpolation (oracle/fht.py), pinned by closed forms and by the
+reference's own interpolation.py (tests/golden/fht_golden.npz)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -105,3 +108,23 @@ def test_rows_inverse_scheme(Ww, M):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
     # a mistake planted in the model is seen: the K term taken once, as a Nyquist term would be
     assert np.abs(_rows_inverse_model(X, Ww, M, k_term=1.0) - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+FHT_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fht_golden.npz")
+FHT_CASES = ("whole_mirror_m4", "whole_nopad_m3", "tiled_m3_t32", "tiled_m2_t16")
+
+
+@pytest.mark.parametrize("name", FHT_CASES)
+def test_oracle_fht_against_reference_vectors(name):
+    """oracle/fht.py against the reference's own interpolation.py on non-constant frames (whole frames with and
+    without mirror padding, overlapping ragged tiles blended with the Hann window), written by
+    tests/golden/make_fht_golden.py."""
+    from oracle import fht
+    g = np.load(FHT_GOLDEN)
+    x, ref = g[f"{name}/input"], g[f"{name}/output"]
+    M, mirror, tile, overlap = (int(v) for v in g[f"{name}/params"])
+    got = fht.fht_space_interpolation(x, M, bool(mirror), tile, overlap)
+    assert got.shape == ref.shape
+    err = np.abs(got.astype(np.float64) - ref) / np.abs(ref).max()
+    print(f"fht {name}: max {err.max():.3e}")
+    assert err.max() <= 1e-6

@@ -136,3 +136,18 @@ def test_rgc_lattice_matches_full_arrays(torch, M, R, S, iw, pad):
     assert torch.equal(got, ref)
     with pytest.raises(_hip.NpxUnsupported):
         _hip.rgc_lattice(cgx, cgy, img, M, 3.5, S, iw, pad)
+
+
+@pytest.mark.parametrize("name", ["whole_mirror_m4", "whole_nopad_m3", "tiled_m3_t32", "tiled_m2_t16"])
+def test_fht_against_reference_vectors(torch, name):
+    """npx_fht_interpolate against the reference's own interpolation.py on non-constant frames (whole frames with and
+    without mirror padding, overlapping ragged tiles blended with the Hann window): tests/golden/fht_golden.npz."""
+    import os
+    from nanopyx_amd import _hip
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fht_golden.npz"))
+    x, ref = g[f"{name}/input"], g[f"{name}/output"]
+    M, mirror, tile, overlap = (int(v) for v in g[f"{name}/params"])
+    print(f"fht reference vectors {name}: {x.shape} M{M} mirror {mirror} tile {tile} overlap {overlap}", flush=True)
+    got = _hip.fht_interpolate(torch.from_numpy(x).cuda(), M, bool(mirror), tile, overlap).cpu().numpy()
+    assert got.shape == ref.shape
+    assert _report(f"fht {name} vs reference interpolation.py", got, ref) <= 1e-5
