@@ -166,6 +166,15 @@ int npx_session_create(size_t pool_bytes, int max_active, long long px_frames_pe
 int npx_session_submit(void *session, const void *src, int src_is_host, int dtype, int N, int H, int W, int M,
                        double R, double S, int iw, unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2,
                        long long *job_id);
+/* A job whose stack the caller copies into device memory itself, piece by piece (d_stack: the device stack, filled
+ * over time): npx_session_feed(session, job, frames, stream) announces that frames [0, frames) are on the device
+ * once the work queued so far on `stream` (the caller's copies) has completed; frame counts grow with each call and
+ * the last one is N. Slices follow the announcements (chunks of at most piece_frames frames). The caller keeps
+ * d_stack and the outputs alive until the job is done. */
+int npx_session_submit_fed(void *session, const void *d_stack, int dtype, int N, int H, int W, int M, double R,
+                           double S, int iw, unsigned mode_mask, float *d_avg, float *d_var, float *d_tac2,
+                           int piece_frames, long long *job_id);
+int npx_session_feed(void *session, long long job_id, int frames, void *stream);
 /* Pool bytes npx_session_submit would need for such a job (0 for invalid parameters). */
 size_t npx_session_job_bytes(int N, int H, int W, int M, double R, int dtype, int src_is_host);
 /* Host-only check of a job's slicing: lays the job out as npx_session_submit would and slices it as npx_session_step

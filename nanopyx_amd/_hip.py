@@ -89,6 +89,10 @@ def lib():
         L.npx_session_submit.argtypes = [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                          _c_double, _c_double, _c_int, ctypes.c_uint, _c_void_p, _c_void_p,
                                          _c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.npx_session_submit_fed.argtypes = [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                             _c_double, _c_double, _c_int, ctypes.c_uint, _c_void_p, _c_void_p,
+                                             _c_void_p, _c_int, ctypes.POINTER(ctypes.c_longlong)]
+        L.npx_session_feed.argtypes = [_c_void_p, ctypes.c_longlong, _c_int, _c_void_p]
         L.npx_session_job_bytes.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
         L.npx_session_job_bytes.restype = _c_size_t
         L.npx_session_slice_plan.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int,
@@ -379,6 +383,33 @@ class Session:
                                         W, int(magnification), float(radius), float(sensitivity), int(bool(iw)), mask,
                                         ptr("AVG"), ptr("VAR"), ptr("TAC2"), ctypes.byref(jid)))
         return jid.value
+
+    def submit_fed(self, stack, magnification, radius, sensitivity, iw, modes, outputs, piece_frames):
+        """Queues a job whose device stack (N, H, W) the caller fills itself, piece by piece; announce each piece
+        with feed(). piece_frames: frames the caller copies at a time (the job's frame chunks are no longer)."""
+        torch = _torch()
+        if stack.dim() != 3 or not stack.is_cuda or not stack.is_contiguous():
+            raise ValueError("a fed job's stack must be a contiguous (N, H, W) device tensor")
+        if stack.dtype not in (torch.float32, torch.uint16):
+            raise ValueError("served stacks must be float32 or uint16")
+        N, H, W = stack.shape
+        mask = 0
+        for m in modes:
+            if m not in MODE_BITS:
+                raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+            mask |= MODE_BITS[m]
+        ptr = lambda k: outputs[k].data_ptr() if k in outputs else None
+        jid = ctypes.c_longlong(0)
+        _check(lib().npx_session_submit_fed(self._h, stack.data_ptr(), _dtype_code(stack), N, H, W,
+                                            int(magnification), float(radius), float(sensitivity), int(bool(iw)), mask,
+                                            ptr("AVG"), ptr("VAR"), ptr("TAC2"), int(piece_frames), ctypes.byref(jid)))
+        return jid.value
+
+    def feed(self, job_id, frames, stream=None):
+        """Frames [0, frames) of a fed job's stack are on the device once the work queued so far on `stream` (the
+        current stream by default) has completed."""
+        torch = _torch()
+        _check(lib().npx_session_feed(self._h, int(job_id), int(frames), _stream(torch, stream)))
 
     @staticmethod
     def job_bytes(N, H, W, magnification, radius, dtype_u16=False, host=True) -> int:

@@ -96,6 +96,8 @@ struct Job {
     // request
     const void* src = nullptr;
     bool src_host = false;
+    bool fed = false;           // device stack filled by the caller, frames announced by npx_session_feed
+    int piece_frames = 0;       // fed jobs: frames the caller announces at a time (chunks are no longer)
     int dtype = 0, N = 0, H = 0, W = 0, M = 0;
     double R = 0, S = 0;
     int iw = 1;
@@ -183,7 +185,9 @@ int admit(Session& s) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(base, j.blob.bytes.data(), j.blob.bytes.size(), hipMemcpyHostToDevice, s.copy);
         if (e == hipSuccess) e = hipEventRecord(j.copied, s.copy);
-        if (j.src_host) {
+        if (j.fed) {
+            // the caller's device stack: frames arrive as npx_session_feed announces them
+        } else if (j.src_host) {
             // pieces of whole frames (and whole chunks when a slice runs as chunks)
             const size_t fb = static_cast<size_t>(j.H) * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4);
             int pf = static_cast<int>(std::max<size_t>(kMinSliceFrames, kPieceBytes / fb));
@@ -207,10 +211,12 @@ int admit(Session& s) {
             (void)hipStreamSynchronize(s.copy);
             if (j.copied) (void)hipEventDestroy(j.copied);
             if (j.done) (void)hipEventDestroy(j.done);
-            for (hipEvent_t ev : j.pieces) (void)hipEventDestroy(ev);
             j.copied = j.done = nullptr;
-            j.pieces.clear();
-            j.piece_end.clear();
+            if (!j.fed) {  // a fed job keeps the caller's announcements
+                for (hipEvent_t ev : j.pieces) (void)hipEventDestroy(ev);
+                j.pieces.clear();
+                j.piece_end.clear();
+            }
             s.pool.release(off, j.bytes);
             return hip_fail(e, "session admission copy");
         }
@@ -254,8 +260,12 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
     for (auto& kv : s.jobs) {
         Job& j = kv.second;
         if (j.state != kAdmitted || j.final_launched || hipEventQuery(j.copied) != hipSuccess) continue;
-        while (j.pieces_done < j.pieces.size() && hipEventQuery(j.pieces[j.pieces_done]) == hipSuccess)
+        while (j.pieces_done < j.pieces.size() && hipEventQuery(j.pieces[j.pieces_done]) == hipSuccess) {
             j.arrived = j.piece_end[j.pieces_done++];
+            // slices start on the job's chunk boundaries: a caller's announcement that ends inside a chunk counts
+            // only up to the chunk's start until the rest arrives
+            if (j.fed && j.cmax > 1 && j.arrived < j.N) j.arrived = j.arrived / j.cfpc * j.cfpc;
+        }
         if (j.arrived > j.next_frame) run.push_back(&j);
     }
     if (run.empty()) return NPX_OK;
@@ -403,6 +413,8 @@ void layout(Job& j, const PlanHost& plan) {
         const size_t fb = static_cast<size_t>(j.H) * j.W * (j.dtype == NPX_DTYPE_U16 ? 2 : 4);
         const int pf = static_cast<int>(std::max<size_t>(kMinSliceFrames, kPieceBytes / fb));
         j.cfpc = std::max(kMinSliceFrames, std::min(j.cfpc, pf));
+    } else if (j.fed) {
+        j.cfpc = std::max(kMinSliceFrames, std::min(j.cfpc, j.piece_frames));
     }
     j.state_at = align_up(j.blob.bytes.size());
     j.part_at = j.state_at + align_up(5 * P * sizeof(float));
@@ -436,9 +448,15 @@ int npx_session_create(size_t pool_bytes, int max_active, long long px_frames_pe
     return NPX_OK;
 }
 
-int npx_session_submit(void* session, const void* src, int src_is_host, int dtype, int N, int H, int W, int M,
-                       double R, double S, int iw, unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2,
-                       long long* job_id) {
+}  // extern "C"
+
+namespace npx {
+namespace serve {
+
+// Validates and queues a job; fed: a device stack the caller fills, announced frames at a time (piece_frames).
+int submit(void* session, const void* src, int src_is_host, int dtype, int N, int H, int W, int M, double R, double S,
+           int iw, unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2, bool fed, int piece_frames,
+           long long* job_id) {
     if (!session || !job_id) return npx::host::fail(NPX_ERR_ARG, "NULL session or job id output");
     if (!src) return npx::host::fail(NPX_ERR_ARG, "input pointer is NULL");
     if (dtype != NPX_DTYPE_F32 && dtype != NPX_DTYPE_U16) return npx::host::fail(NPX_ERR_ARG, "dtype must be f32 or u16");
@@ -467,6 +485,8 @@ int npx_session_submit(void* session, const void* src, int src_is_host, int dtyp
     j.var = d_var;
     j.tac2 = d_tac2;
     j.plan = plan;
+    j.fed = fed;
+    j.piece_frames = piece_frames;
     npx::host::pack_tables(*plan, &j.blob);
     npx::serve::layout(j, *plan);
     std::lock_guard<std::mutex> lk(s.mu);
@@ -476,6 +496,50 @@ int npx_session_submit(void* session, const void* src, int src_is_host, int dtyp
     *job_id = j.id;
     s.queue.push_back(j.id);
     s.jobs.emplace(j.id, std::move(j));
+    return NPX_OK;
+}
+
+}  // namespace serve
+}  // namespace npx
+
+extern "C" {
+
+int npx_session_submit(void* session, const void* src, int src_is_host, int dtype, int N, int H, int W, int M,
+                       double R, double S, int iw, unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2,
+                       long long* job_id) {
+    return npx::serve::submit(session, src, src_is_host, dtype, N, H, W, M, R, S, iw, mode_mask, d_avg, d_var, d_tac2,
+                              false, 0, job_id);
+}
+
+int npx_session_submit_fed(void* session, const void* d_stack, int dtype, int N, int H, int W, int M, double R,
+                           double S, int iw, unsigned mode_mask, float* d_avg, float* d_var, float* d_tac2,
+                           int piece_frames, long long* job_id) {
+    if (piece_frames < 1) return npx::host::fail(NPX_ERR_ARG, "piece_frames must be >= 1");
+    return npx::serve::submit(session, d_stack, 0, dtype, N, H, W, M, R, S, iw, mode_mask, d_avg, d_var, d_tac2, true,
+                              piece_frames, job_id);
+}
+
+int npx_session_feed(void* session, long long job_id, int frames, void* stream) {
+    if (!session) return npx::host::fail(NPX_ERR_ARG, "NULL session");
+    Session& s = *static_cast<Session*>(session);
+    std::lock_guard<std::mutex> lk(s.mu);
+    auto it = s.jobs.find(job_id);
+    if (it == s.jobs.end()) return npx::host::fail(NPX_ERR_ARG, "unknown job id");
+    Job& j = it->second;
+    if (!j.fed) return npx::host::fail(NPX_ERR_ARG, "job was not submitted with npx_session_submit_fed");
+    const int last = j.piece_end.empty() ? 0 : j.piece_end.back();
+    if (frames <= last || frames > j.N)
+        return npx::host::fail(NPX_ERR_ARG, "fed frame count must grow and stay within the stack");
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return npx::host::hip_fail(e, "session feed event");
+    e = hipEventRecord(ev, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return npx::host::hip_fail(e, "session feed event");
+    }
+    j.pieces.push_back(ev);
+    j.piece_end.push_back(frames);
     return NPX_OK;
 }
 

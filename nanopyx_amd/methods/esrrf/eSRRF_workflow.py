@@ -4,8 +4,11 @@ Same signature and output: frames are split into timepoints of ``frames_per_time
 timepoint becomes one (H*M, W*M) map of the requested temporal correlation, optionally macro-pixel corrected,
 and the (T, H*M, W*M) float32 result is squeezed. On this path each timepoint is one fused launch
 (eSRRF + AVG/VAR/TAC2, ``npx_esrrf_temporal``) followed by the macro-pixel kernel; the stack is uploaded once and
-the per-frame maps never leave the GPU. With ``use_fht`` each timepoint runs the Fourier-interpolation branch of the
-operator (per-frame maps on the GPU), then ``npx_temporal`` and the macro-pixel kernel.
+the per-frame maps never leave the GPU. A host (NumPy) stack of 64 MiB or more is streamed instead
+(``nanopyx_amd.upload``): pieces go to the device through pinned staging buffers while a serving session's jobs, one
+per timepoint, already run on the frames that have arrived. With ``use_fht`` each timepoint runs the
+Fourier-interpolation branch of the operator (per-frame maps on the GPU), then ``npx_temporal`` and the macro-pixel
+kernel.
 """
 from __future__ import annotations
 
@@ -14,7 +17,7 @@ import numpy as np
 from ...__agent__ import Agent
 from ...core.transform import eSRRF_ST
 from ...core.transform.__interpolation_tools__ import check_image
-from ... import _hip
+from ... import _hip, upload
 
 
 def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float = 1, frames_per_timepoint: int = 0,
@@ -26,6 +29,19 @@ def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float
     import torch
 
     is_tensor = isinstance(image, torch.Tensor)
+    if not is_tensor and not use_fht:
+        x = check_image(image, keep_uint16=True)
+        if x.nbytes >= upload.STREAM_MIN_BYTES:
+            # a large host stack: uploaded in pieces while the timepoints' kernels already run on the arrived frames
+            if _force_run_type not in (None, "HIP_gfx950"):
+                raise ValueError(f"eSRRF: unknown run type {_force_run_type!r}; available: ['HIP_gfx950']")
+            fpt = x.shape[0] if frames_per_timepoint in (0, None) or frames_per_timepoint > x.shape[0] \
+                else frames_per_timepoint
+            out = upload.esrrf_temporal_streamed(np.ascontiguousarray(x), magnification, radius, sensitivity,
+                                                 doIntensityWeighting, temporal_correlation, fpt)
+            if macro_pixel_correction:
+                out = _hip.macro_pixel_(out, int(magnification))
+            return torch.squeeze(out).cpu().numpy().astype(np.float32)
     N = image.shape[0]
     if frames_per_timepoint == 0:
         frames_per_timepoint = N

@@ -220,3 +220,52 @@ def test_host_stack_stepped_after_upload(torch, N, H, W, M, mode):
     assert _report(f"host after upload {mode} vs device source", maps[0], maps[1]) <= 1e-6
     plain = _hip.esrrf_temporal(x, M, 1.5, 1, True, (mode,), workspace=False)[mode].cpu().numpy()
     assert _report(f"host after upload {mode} vs plain", maps[0], plain) <= 1e-6
+
+
+def test_fed_job_errors(torch):
+    from nanopyx_amd import _hip
+    x = torch.zeros((16, 8, 8), device="cuda")
+    sess = _hip.Session(64 << 20, 4, 1 << 20)
+    out = {"AVG": torch.empty((16, 16), device="cuda")}
+    with pytest.raises(ValueError):
+        sess.submit_fed(x, 2, 1.5, 1, True, ("AVG",), out, 0)
+    jid = sess.submit_fed(x, 2, 1.5, 1, True, ("AVG",), out, 8)
+    sess.feed(jid, 8)
+    with pytest.raises(ValueError):
+        sess.feed(jid, 8)          # counts must grow
+    with pytest.raises(ValueError):
+        sess.feed(jid, 17)         # beyond the stack
+    plain = sess.submit(x, 2, 1.5, 1, True, ("AVG",), {"AVG": torch.empty((16, 16), device="cuda")})
+    with pytest.raises(ValueError):
+        sess.feed(plain, 4)        # not a fed job
+    sess.feed(jid, 16)
+    for _ in range(10000):
+        sess.step()
+        if sess.poll(jid) == _hip.Session.DONE and sess.poll(plain) == _hip.Session.DONE:
+            break
+    torch.cuda.synchronize()
+    sess.close()
+
+
+@pytest.mark.parametrize("N,H,W,M,R,S,fpt,mode,dtype,mpc", [
+    (300, 256, 256, 2, 1.5, 2, 100, "AVG", "float32", True),    # 79 MB: three timepoints, 128-frame pieces
+    (260, 384, 384, 3, 2.0, 1, 0, "TAC2", "uint16", False),     # 77 MB uint16, one timepoint
+    (210, 256, 512, 4, 1.0, 3, 70, "VAR", "float32", True),     # 110 MB, pieces straddle timepoints
+])
+def test_workflow_streams_host_stack(torch, N, H, W, M, R, S, fpt, mode, dtype, mpc):
+    """eSRRF() on a host stack of 64 MiB or more streams it (pinned pieces, fed session jobs, one per timepoint): the
+    result equals eSRRF() on the same stack already on the device (<= 1e-6: the frame chunks may differ)."""
+    import nanopyx_amd
+    from nanopyx_amd import synthetic
+    print(f"streamed workflow N{N} {H}x{W} M{M} fpt {fpt} {mode} {dtype} mpc {mpc}", flush=True)
+    sc = synthetic.Scene(N, H, W, max(1, int(0.02 * H * W)), 1.5, 500, 0.3, 100, 0.0, 2.0,
+                         100.0 if dtype == "uint16" else 0.0, dtype, 79)
+    x = synthetic.generate(sc)
+    assert x.nbytes >= 64 << 20
+    got = nanopyx_amd.eSRRF(x, magnification=M, radius=R, sensitivity=S, frames_per_timepoint=fpt,
+                            temporal_correlation=mode, macro_pixel_correction=mpc)
+    ref = nanopyx_amd.eSRRF(torch.from_numpy(x).cuda(), magnification=M, radius=R, sensitivity=S,
+                            frames_per_timepoint=fpt, temporal_correlation=mode,
+                            macro_pixel_correction=mpc).cpu().numpy()
+    assert got.shape == ref.shape and got.dtype == np.float32
+    assert _report(f"streamed workflow {mode} vs device stack", got, ref) <= 1e-6
