@@ -318,8 +318,9 @@ def fht_bench(reps, cpu_frames=1, frames=20):
 def serving_bench(n, rate_hz, seed=5):
     """The served stream: n seeded requests (frames log-uniform 50-1000, H and W from {64..1024}, M from
     {2, 3, 4, 5, 8}, R uniform on [1, 2.5], S from {1, 2, 3}, one of AVG / VAR / TAC2, float32 or uint16) arriving as
-    a Poisson process at rate_hz, stacks in pinned host memory; served by the continuous-batching session and, on
-    the same arrival trace, by one plain fused call per request."""
+    a Poisson process at rate_hz (and at 2x and 3x that rate for the rate curve, latency split by job size), stacks
+    in pinned host memory; served by the continuous-batching session and, on the same arrival trace, by one plain
+    fused call per request."""
     import torch
     from nanopyx_amd import serving
     t0 = time.perf_counter()
@@ -328,10 +329,25 @@ def serving_bench(n, rate_hz, seed=5):
     t_arr = serving.arrivals(n, rate_hz, seed)
     warm = serving.make_requests(2, seed=seed + 1, host=True, max_frames=16)
     serving.serve(warm, 1000.0, seed)  # first-use costs (code objects, pool) outside the measured stream
-    served, _ = serving.serve(reqs, rate_hz, seed, t_arr=t_arr)
+    served, lat_s = serving.serve(reqs, rate_hz, seed, t_arr=t_arr)
     log(f"served: {served}")
-    plain, _ = serving.serve_plain(reqs, rate_hz, seed, t_arr=t_arr)
+    plain, lat_p = serving.serve_plain(reqs, rate_hz, seed, t_arr=t_arr)
     log(f"plain: {plain}")
+    # the same requests at 1x, 2x and 3x the rate, latency split by job size (terciles of output pixel-frames)
+    curve = [{"rate_hz": rate_hz, "served": served, "plain_per_request": plain,
+              "served_by_size": serving.by_size(reqs, lat_s), "plain_by_size": serving.by_size(reqs, lat_p)}]
+    for k in (2, 3):
+        ta = serving.arrivals(n, k * rate_hz, seed)
+        sv, ls = serving.serve(reqs, k * rate_hz, seed, t_arr=ta)
+        pl, lp = serving.serve_plain(reqs, k * rate_hz, seed, t_arr=ta)
+        curve.append({"rate_hz": k * rate_hz, "served": sv, "plain_per_request": pl,
+                      "served_by_size": serving.by_size(reqs, ls), "plain_by_size": serving.by_size(reqs, lp)})
+        log(f"rate {k * rate_hz}: served {sv} plain {pl}")
+    for c in curve:
+        c["p99_speedup"] = round(c["plain_per_request"]["p99_ms"] / c["served"]["p99_ms"], 2)
+        c["p50_speedup"] = round(c["plain_per_request"]["p50_ms"] / c["served"]["p50_ms"], 2)
+        c["throughput_speedup"] = round(c["served"]["throughput_mpx_frames_s"] /
+                                        c["plain_per_request"]["throughput_mpx_frames_s"], 2)
     # saturated: every request arrives at t = 0 (throughput of the packed launches vs one call per request)
     t0s = np.zeros(n)
     served_sat, _ = serving.serve(reqs, rate_hz, seed, t_arr=t0s)
@@ -344,7 +360,7 @@ def serving_bench(n, rate_hz, seed=5):
     return {"workload": f"{n} requests, Poisson {rate_hz}/s, seed {seed}: {px / 1e9:.1f} G output px-frames, "
                         f"{stack_gb:.1f} GB of stacks in pinned host memory",
             "rate_hz": rate_hz, "served": served, "plain_per_request": plain,
-            "saturated_served": served_sat, "saturated_plain_per_request": plain_sat,
+            "saturated_served": served_sat, "saturated_plain_per_request": plain_sat, "rate_curve": curve,
             "p99_speedup": round(plain["p99_ms"] / served["p99_ms"], 2),
             "throughput_speedup": round(served["throughput_mpx_frames_s"] / plain["throughput_mpx_frames_s"], 2)}
 
@@ -568,6 +584,7 @@ def main():
     ap.add_argument("--no-fht", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the cpu_small and plain_256 legs")
     ap.add_argument("--configs-only", action="store_true", help="run only the cpu_small and plain_256 legs")
+    ap.add_argument("--serving-only", action="store_true", help="run only the served-stream leg and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one plain 1 GiB device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fht-only", action="store_true", help="run only the use_fht leg and print its JSON")
@@ -596,6 +613,9 @@ def main():
         return
     if args.fht_only:
         print(json.dumps(fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving_bench(args.serving_requests, args.serving_rate)), flush=True)
         return
     if args.configs_only:
         print(json.dumps({"cpu_small": config_leg("cpu_small", max(5, args.steps), 10.0),

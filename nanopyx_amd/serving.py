@@ -79,6 +79,23 @@ def _summary(reqs, t_arr, t_done, wall):
             "throughput_mpx_frames_s": round(px / 1e6 / wall, 1), "requests_per_s": round(len(reqs) / wall, 2)}
 
 
+def by_size(reqs, lat, edges=None):
+    """p50 / p99 latency (ms) of the requests split into three classes by output pixel-frames: below the first
+    edge, between, above the second (edges default to the terciles of this request set)."""
+    px = np.asarray([r.out_px_frames for r in reqs], np.float64)
+    lat = np.asarray(lat, np.float64)
+    if edges is None:
+        edges = np.percentile(px, [100 / 3, 200 / 3])
+    out = {}
+    for name, sel in (("small", px <= edges[0]), ("medium", (px > edges[0]) & (px <= edges[1])),
+                      ("large", px > edges[1])):
+        if sel.any():
+            out[name] = {"requests": int(sel.sum()), "max_mpx_frames": round(float(px[sel].max()) / 1e6, 1),
+                         "p50_ms": round(float(np.percentile(lat[sel], 50)) * 1e3, 2),
+                         "p99_ms": round(float(np.percentile(lat[sel], 99)) * 1e3, 2)}
+    return out
+
+
 def serve(reqs, rate_hz, seed=5, pool_bytes=64 << 30, max_active=32, px_frames_per_step=1 << 29, t_arr=None):
     """Runs the request stream through one session; returns (summary, per-request latencies in s)."""
     import torch

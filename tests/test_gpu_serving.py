@@ -269,3 +269,39 @@ def test_workflow_streams_host_stack(torch, N, H, W, M, R, S, fpt, mode, dtype, 
                             macro_pixel_correction=mpc).cpu().numpy()
     assert got.shape == ref.shape and got.dtype == np.float32
     assert _report(f"streamed workflow {mode} vs device stack", got, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("N,H,W,M,R,S,mode", [
+    (2, 48, 1024, 5, 1.7, 2, "TAC2"),
+    (2, 48, 1024, 3, 2.3, 1, "AVG"),
+    (2, 24, 1024, 8, 1.2, 3, "VAR"),
+])
+def test_wide_shapes_vs_oracle(torch, oracle_built, N, H, W, M, R, S, mode):
+    """The served stream's widest inputs (1024 columns) at magnifications that are not powers of two: the reference
+    rounds each pixel centre to float32 ((x M + 0.5) / M per position) where the kernels use the exact phase offset,
+    a difference that grows with the coordinate. Per-frame maps, the plain fused call and the served job against the
+    oracle's restatement of the reference chain."""
+    from nanopyx_amd import _hip, synthetic
+    print(f"wide N{N} {H}x{W} M{M} R{R} S{S} {mode}", flush=True)
+    sc = synthetic.Scene(N, H, W, max(1, int(0.02 * H * W)), 1.5, 500, 0.3, 100, 0.0, 2.0, 0.0, "float32", 80 + M)
+    xh = synthetic.generate(sc)
+    x = torch.from_numpy(xh).cuda()
+    ref_maps = oracle_built.esrrf(xh, M, R, S)
+    frames = _hip.esrrf_frames(x, M, R, S, True).cpu().numpy()
+    assert _report(f"wide M{M} frames vs reference restatement", frames, ref_maps) <= TOL
+    # the right-most quarter of the columns, where the coordinates are largest
+    q = frames.shape[2] * 3 // 4
+    assert _report(f"wide M{M} frames, last quarter of columns", frames[..., q:], ref_maps[..., q:]) <= TOL
+    ref = oracle_built.temporal(ref_maps, mode)
+    plain = _hip.esrrf_temporal(x, M, R, S, True, (mode,))[mode].cpu().numpy()
+    assert _report(f"wide M{M} {mode} plain vs reference restatement", plain, ref) <= TOL
+    sess = _hip.Session(1 << 30, 4, 1 << 22)
+    out = {mode: torch.empty((H * M, W * M), device="cuda")}
+    jid = sess.submit(x, M, R, S, True, (mode,), out)
+    for _ in range(100000):
+        sess.step()
+        if sess.poll(jid) == _hip.Session.DONE:
+            break
+    torch.cuda.synchronize()
+    sess.close()
+    assert _report(f"wide M{M} {mode} served vs reference restatement", out[mode].cpu().numpy(), ref) <= TOL
