@@ -222,8 +222,17 @@ __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, i
     static_assert(K <= KL, "ring wider than the lattice");
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0.f;
+    // Wave priority falls as the wave moves through the window columns (3 for the first third, then 2, 1, and 0 for
+    // the last column). The SIMD issues from the highest-priority ready wave first and otherwise from the oldest, so
+    // without this the oldest of its waves runs ahead and the youngest finishes the ring sums alone at a fraction of
+    // the issue rate; with it the waves that are behind are preferred and the waves of a SIMD finish close together.
+    constexpr int P2 = NW / 3, P1 = 2 * NW / 3, P0 = NW - 1;
+    __builtin_amdgcn_s_setprio(3);
 #pragma unroll 1
     for (int i = 0; i < NW; ++i) {
+        if (i == P2) __builtin_amdgcn_s_setprio(2);
+        else if (i == P1) __builtin_amdgcn_s_setprio(1);
+        else if (i == P0) __builtin_amdgcn_s_setprio(0);
         const int c = ex + i + DK;
         const float2* col = s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
         const float dxn = rl[RDX + i];  // -s dx
