@@ -589,6 +589,10 @@ def main():
                     help="one plain 1 GiB device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fht-only", action="store_true", help="run only the use_fht leg and print its JSON")
     ap.add_argument("--esrrf3d-only", action="store_true", help="run only the eSRRF 3D leg and print its JSON")
+    ap.add_argument("--sweep-only", action="store_true", help="run only the parameter-sweep leg and print its JSON")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the timed fused launches of the workload (no other legs, no CPU baseline): for "
+                         "profiling the fused kernel")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=50.0)
     args = ap.parse_args()
@@ -624,6 +628,12 @@ def main():
     if args.esrrf3d_only:
         print(json.dumps(esrrf3d_bench(max(3, args.steps), cpu_frames=0 if args.no_cpu else 1)), flush=True)
         return
+    if args.sweep_only:
+        print(json.dumps(sweep_bench(max(3, args.steps // 2))), flush=True)
+        return
+    if args.kernel_only:
+        for k in ("no_configs", "no_sweep", "no_srrf", "no_esrrf3d", "no_fht", "no_serving", "no_cpu"):
+            setattr(args, k, True)
 
     import torch
     from nanopyx_amd import _hip, synthetic
@@ -738,12 +748,13 @@ def main():
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
-    log("end to end (host stack in, maps out) ...")
-    result["end_to_end_with_pcie"] = end_to_end(x_host, e)
-    result["end_to_end_streamed"] = end_to_end_streamed(x_host, e)
-    result["workflow_numpy_stack"] = workflow_numpy(x_host, e)
-    log(f"end to end: {result['end_to_end_with_pcie']} {result['end_to_end_streamed']} "
-        f"{result['workflow_numpy_stack']}")
+    if not args.kernel_only:
+        log("end to end (host stack in, maps out) ...")
+        result["end_to_end_with_pcie"] = end_to_end(x_host, e)
+        result["end_to_end_streamed"] = end_to_end_streamed(x_host, e)
+        result["workflow_numpy_stack"] = workflow_numpy(x_host, e)
+        log(f"end to end: {result['end_to_end_with_pcie']} {result['end_to_end_streamed']} "
+            f"{result['workflow_numpy_stack']}")
     if not args.no_cpu:
         log("timing the reference CPU path ...")
         result["cpu_baseline"], ref_maps = cpu_baseline(x_host, e, args.cpu_seconds)

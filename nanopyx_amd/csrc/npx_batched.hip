@@ -1,11 +1,12 @@
 // Batched launches of the fused eSRRF block (npx_fused.h): several jobs in one launch.
 //
-// Used by the parameter sweep (one job per magnification; its radii share the gradient lattice and its
-// sensitivities share the ring sums) and by the serving loop (frame slices of several requests packed into
-// one launch). Jobs travel by value in the kernel arguments; block b finds its job by a scan of the block
-// offsets (at most kMaxJobs, wave-uniform) and decomposes its local index into region, phase group and
-// frame chunk. The lattice is built for the largest ring half-width KL of the launch; each wave runs the
-// ring sum of its own radius.
+// Used by the parameter sweep (one job per magnification and radius, or several radii of a small magnification
+// packed into the wave slots of one block, where they share the gradient lattice; the sensitivities of a job share
+// its ring sums) and by the serving loop (frame slices of several requests packed into one launch). Jobs travel by
+// value in the kernel arguments; block b finds its job by a scan of the block offsets (at most kMaxJobs,
+// wave-uniform) and decomposes its local index into region, phase group and frame chunk. Each job carries its own
+// lattice half-width KL: a job whose waves all have ring half-width KL runs the body built for that K, a sweep job
+// whose radii differ (vark) runs the body whose waves each take their own K <= KL.
 #include <hip/hip_runtime.h>
 
 #include "npx_fused.h"
@@ -13,14 +14,23 @@
 
 namespace npx {
 
-template <int KL, int MODE>
-__global__ __launch_bounds__(kBatchThreads) void esrrf_batched(BatchArgs b) {
+// Lattice half-widths the bodies for mixed ring half-widths are built for: the smallest one >= every K of the job.
+static int batched_kl(int kmax) { return kmax <= 2 ? 2 : (kmax <= 4 ? 4 : 6); }
+
+using ckBatch = const __attribute__((address_space(4))) BatchArgs;
+
+template <int MODE>
+__global__ __launch_bounds__(kBatchThreads) void esrrf_batched(BatchArgs args) {
     extern __shared__ __align__(16) unsigned char smem[];
+    (void)args;
+    // the job table is read in place in the kernel-argument segment (scalar loads); a body that takes the job by
+    // reference from the by-value parameter makes the compiler copy the 4 KiB block to scratch memory
+    ckBatch& b = *(ckBatch*)__builtin_amdgcn_kernarg_segment_ptr();
     const int blk = blockIdx.x;
     int j = 0;
     while (j + 1 < b.njobs && blk >= b.start[j + 1]) ++j;
     j = __builtin_amdgcn_readfirstlane(j);
-    const FusedArgs& a = b.job[j];
+    const FusedArgs& a = *(const FusedArgs*)(&b.job[j]);
     int local = blk - b.start[j];
     const int nx = b.nregx[j], ny = b.nregy[j];
     const int bx = local % nx;
@@ -29,49 +39,59 @@ __global__ __launch_bounds__(kBatchThreads) void esrrf_batched(BatchArgs b) {
     local /= ny;
     const int group = local % a.ngroups;
     const int chunk = local / a.ngroups;
-    fused_block<KL, kQTemporal, MODE, true>(a, bx, by, group, chunk, smem);
+    constexpr int Q = kQTemporal;
+    if constexpr (MODE == kModeSweep) {
+        if (a.vark) {
+            switch (a.KL) {
+                case 2: fused_block<2, Q, MODE, true>(a, bx, by, group, chunk, smem); return;
+                case 4: fused_block<4, Q, MODE, true>(a, bx, by, group, chunk, smem); return;
+                default: fused_block<6, Q, MODE, true>(a, bx, by, group, chunk, smem); return;
+            }
+        }
+    }
+    switch (a.KL) {
+        case 0: fused_block<0, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        case 1: fused_block<1, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        case 2: fused_block<2, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        case 3: fused_block<3, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        case 4: fused_block<4, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        case 5: fused_block<5, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+        default: fused_block<6, Q, MODE, false>(a, bx, by, group, chunk, smem); return;
+    }
 }
-
-template <int KL>
-static hipError_t launch_batched_k(const BatchArgs& b, int mode, dim3 grid, size_t lds, hipStream_t st) {
-    const dim3 block(kBatchThreads);
-    if (mode == kModeTemporal)
-        hipLaunchKernelGGL((esrrf_batched<KL, kModeTemporal>), grid, block, lds, st, b);
-    else if (mode == kModeTemporalVar)
-        hipLaunchKernelGGL((esrrf_batched<KL, kModeTemporalVar>), grid, block, lds, st, b);
-    else if (mode == kModeSweep)
-        hipLaunchKernelGGL((esrrf_batched<KL, kModeSweep>), grid, block, lds, st, b);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
-}
-
-// Lattice half-widths the batched kernels are built for: the launch uses the smallest one >= every job's K.
-static int batched_kl(int kmax) { return kmax <= 2 ? 2 : (kmax <= 4 ? 4 : 6); }
 
 hipError_t launch_batched(const BatchArgs& b, int mode, hipStream_t st) {
     if (b.njobs < 1 || b.njobs > kMaxJobs) return hipErrorInvalidValue;
-    int kmax = 0;
+    if (mode != kModeTemporal && mode != kModeTemporalVar && mode != kModeSweep) return hipErrorInvalidValue;
+    const int nw = kBatchThreads / 64;
+    size_t lds = 0;
     for (int j = 0; j < b.njobs; ++j) {
         const FusedArgs& a = b.job[j];
-        if (a.nwaves * 64 != kBatchThreads || a.ngroups * a.nwaves < a.nphase * a.nrad) return hipErrorInvalidValue;
+        if (a.nwaves != nw || a.ngroups * a.nwaves < a.nphase * a.nrad) return hipErrorInvalidValue;
         if (a.nrad < 1 || a.nrad > kMaxRad || a.ns < 1 || a.ns > kMaxSens) return hipErrorInvalidValue;
-        for (int r = 0; r < a.nrad; ++r) kmax = a.K[r] > kmax ? a.K[r] : kmax;
+        if (a.KL < 0 || a.KL > kMaxK) return hipErrorInvalidValue;
+        if (a.vark) {
+            int kmax = 0;
+            for (int r = 0; r < a.nrad; ++r) kmax = a.K[r] > kmax ? a.K[r] : kmax;
+            if (mode != kModeSweep || a.KL != batched_kl(kmax)) return hipErrorInvalidValue;
+        } else {
+            for (int r = 0; r < a.nrad; ++r)
+                if (a.K[r] != a.KL) return hipErrorInvalidValue;
+        }
+        const size_t lj = fused_lds_bytes(a.KL, kQTemporal, nw, mode);
+        lds = lj > lds ? lj : lds;
     }
-    if (kmax > kMaxK) return hipErrorInvalidValue;
-    const int KL = batched_kl(kmax);
-    for (int j = 0; j < b.njobs; ++j)
-        if (b.job[j].KL != KL) return hipErrorInvalidValue;
     const int nblocks = b.start[b.njobs];
     if (nblocks <= 0) return hipSuccess;
-    const size_t lds = fused_lds_bytes(KL, kQTemporal, kBatchThreads / 64, mode);
     if (lds > static_cast<size_t>(kLdsLimit)) return hipErrorInvalidConfiguration;
-    const dim3 grid(nblocks);
-    switch (KL) {
-        case 2: return launch_batched_k<2>(b, mode, grid, lds, st);
-        case 4: return launch_batched_k<4>(b, mode, grid, lds, st);
-        default: return launch_batched_k<6>(b, mode, grid, lds, st);
-    }
+    const dim3 grid(nblocks), block(kBatchThreads);
+    if (mode == kModeTemporal)
+        hipLaunchKernelGGL((esrrf_batched<kModeTemporal>), grid, block, lds, st, b);
+    else if (mode == kModeTemporalVar)
+        hipLaunchKernelGGL((esrrf_batched<kModeTemporalVar>), grid, block, lds, st, b);
+    else
+        hipLaunchKernelGGL((esrrf_batched<kModeSweep>), grid, block, lds, st, b);
+    return hipGetLastError();
 }
 
 int batched_lattice_k(int kmax) { return batched_kl(kmax); }

@@ -313,18 +313,27 @@ bool capturing(hipStream_t st) {
 struct SweepJob {
     int mi, r0, nr, s0, nsn;
     int M, nregx, nregy, ngroups, kmax;
+    double cost;  // estimated time of one of its blocks (ring samples of its waves + the per-frame block work)
     size_t P;
     size_t ws_off;
 };
 
 struct SweepLayout {
-    std::vector<SweepJob> jobs;  // heaviest first
+    std::vector<SweepJob> jobs;  // longest blocks first
     int nchunk = 1, fpc = 1;
     size_t ws_bytes = 0;
 };
 
+// Per-frame work of one block besides its ring sums (tile, gradients, lattice, finalise), in ring-sample units of
+// one wave slot: each slot's ring costs (2K + 1)^2 units, the rest of the block about 750.
+constexpr double kBlockFrameCost = 750.0;
+
 // Job list, frame chunking and workspace of a sweep; deterministic in its arguments (the workspace query and
 // the sweep itself agree). chunks = false: one chunk, no workspace.
+//
+// A magnification whose phases fill whole blocks (M * M a multiple of the 16 wave slots) gets one job per radius,
+// so every block runs one ring half-width; smaller ones pack up to kMaxRad radii into the slots of one block (they
+// share its lattice). Jobs run longest blocks first, so the short blocks fill the end of the launch.
 int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, const double* radii, int n_sens,
                  bool chunks, SweepLayout* L) {
     if (n_mag < 1 || n_rad < 1 || n_sens < 1 || !mags || !radii) return fail(NPX_ERR_ARG, "empty sweep");
@@ -334,15 +343,23 @@ int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, con
     for (int mi = 0; mi < n_mag; ++mi) {
         const int M = mags[mi];
         if (M < 1 || M > 64) return fail(NPX_ERR_ARG, "magnification must be in [1, 64]");
-        for (int r0 = 0; r0 < n_rad; r0 += npx::kMaxRad) {
-            const int nr = std::min(npx::kMaxRad, n_rad - r0);
+        std::vector<int> K(n_rad);
+        for (int r = 0; r < n_rad; ++r) {
+            std::lock_guard<std::mutex> lk(g_mu);
+            Plan* p = get_plan(H, W, M, radii[r]);
+            if (!p) return NPX_ERR_UNSUPPORTED;
+            K[r] = p->host.K;
+        }
+        const int per_job = (M * M) % nw == 0 ? 1 : npx::kMaxRad;
+        for (int r0 = 0; r0 < n_rad; r0 += per_job) {
+            const int nr = std::min(per_job, n_rad - r0);
             int kmax = 0;
+            double ring = 0.0;
             for (int r = r0; r < r0 + nr; ++r) {
-                std::lock_guard<std::mutex> lk(g_mu);
-                Plan* p = get_plan(H, W, M, radii[r]);
-                if (!p) return NPX_ERR_UNSUPPORTED;
-                kmax = std::max(kmax, p->host.K);
+                kmax = std::max(kmax, K[r]);
+                ring += static_cast<double>(M) * M * (2 * K[r] + 1) * (2 * K[r] + 1);
             }
+            const int ngroups = (M * M * nr + nw - 1) / nw;
             for (int s0 = 0; s0 < n_sens; s0 += npx::kMaxSens) {
                 SweepJob j{};
                 j.mi = mi;
@@ -353,15 +370,16 @@ int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, con
                 j.M = M;
                 j.nregx = (W + 63) / 64;
                 j.nregy = (H + npx::kQTemporal - 1) / npx::kQTemporal;
-                j.ngroups = (M * M * nr + nw - 1) / nw;
+                j.ngroups = ngroups;
                 j.kmax = kmax;
+                j.cost = kBlockFrameCost + ring / ngroups;
                 j.P = static_cast<size_t>(H) * M * static_cast<size_t>(W) * M;
                 L->jobs.push_back(j);
             }
         }
     }
     std::stable_sort(L->jobs.begin(), L->jobs.end(), [](const SweepJob& a, const SweepJob& b) {
-        return static_cast<long long>(a.M) * a.M * a.nr > static_cast<long long>(b.M) * b.M * b.nr;
+        return a.cost > b.cost;
     });
     long long spatial = 0;
     for (const auto& j : L->jobs) spatial += static_cast<long long>(j.nregx) * j.nregy * j.ngroups;
@@ -635,9 +653,6 @@ int npx_esrrf_sweep(const void* d_in, int dtype, int N, int H, int W, int n_mag,
     std::vector<std::vector<float*>> merge_outs;
     for (size_t j0 = 0; j0 < L.jobs.size(); j0 += npx::kMaxJobs) {
         const size_t nj = std::min<size_t>(npx::kMaxJobs, L.jobs.size() - j0);
-        int kmax = 0;
-        for (size_t j = j0; j < j0 + nj; ++j) kmax = std::max(kmax, L.jobs[j].kmax);
-        const int KL = npx::batched_lattice_k(kmax);
         npx::BatchArgs b{};
         b.njobs = static_cast<int>(nj);
         int blocks = 0;
@@ -654,7 +669,10 @@ int npx_esrrf_sweep(const void* d_in, int dtype, int N, int H, int W, int n_mag,
                 add_radius(a, p->host, t);
             }
             for (int s = 0; s < sj.nsn; ++s) add_sensitivity(a, sens[sj.s0 + s]);
-            a.KL = KL;
+            // one half-width: the body built for it; radii of different half-widths share a wider lattice
+            a.vark = 0;
+            for (int r = 1; r < sj.nr; ++r) a.vark |= a.K[r] != a.K[0];
+            a.KL = a.vark ? npx::batched_lattice_k(sj.kmax) : a.K[0];
             a.fpc = L.fpc;
             a.nchunk = L.nchunk;
             a.direct = L.nchunk == 1;

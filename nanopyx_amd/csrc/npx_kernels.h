@@ -70,6 +70,8 @@ struct FusedArgs {
     int n0;                  // temporal: frames already folded into `state` (resumed when > 0)
     int iw;
     int KL;                  // lattice half-width (max of K[])
+    int vark;                // batched sweep jobs: 1 when the radii's half-widths K[] differ (each <= KL, KL from
+                             // batched_lattice_k); 0: every K[] equals KL
     int nrad;
     int ns;
     // taps of the gradient lattice (depend on H, W, M) and of the intensity Catmull-Rom (H, W, M)
@@ -122,7 +124,7 @@ size_t fused_lds_bytes(int KL, int Q, int nwaves, int mode);
 
 hipError_t launch_fused(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st);
 hipError_t launch_batched(const BatchArgs& b, int mode, hipStream_t st);
-int batched_lattice_k(int kmax);  // lattice half-width of a batched launch whose widest ring has half-width kmax
+int batched_lattice_k(int kmax);  // lattice half-width of a batched sweep job of mixed half-widths, widest kmax
 hipError_t launch_means_merge(const float* part, int nchunk, int fpc, int N, size_t P, int nout, float* const* outs,
                               hipStream_t st);
 hipError_t launch_merge(const float* part, int nchunk, int fpc, int N, size_t P, float* avg, float* var, float* tac2,

@@ -306,13 +306,8 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
     std::vector<Merge> merges;
     for (size_t j0 = 0; j0 < run.size(); j0 += kMaxJobs) {
         const size_t nj = std::min<size_t>(kMaxJobs, run.size() - j0);
-        int kmax = 0;
         bool var = false;
-        for (size_t k = j0; k < j0 + nj; ++k) {
-            kmax = std::max(kmax, run[k]->plan->K);
-            var = var || (run[k]->mask & NPX_VAR);
-        }
-        const int KL = batched_lattice_k(kmax);
+        for (size_t k = j0; k < j0 + nj; ++k) var = var || (run[k]->mask & NPX_VAR);
         BatchArgs b{};
         b.njobs = static_cast<int>(nj);
         int blocks = 0;
@@ -327,8 +322,7 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
             sh.nregy = (j.H + kQTemporal - 1) / kQTemporal;
             FusedArgs a = host::job_args(j.d_in, j.dtype, j.next_frame + nf, j.H, j.W, j.M, h, &j.t, sh, j.iw);
             host::add_radius(a, h, &j.t);
-            host::add_sensitivity(a, j.S);
-            a.KL = KL;
+            host::add_sensitivity(a, j.S);  // one radius: the lattice half-width is its K (add_radius)
             a.f0 = j.next_frame;
             a.avg = (j.mask & NPX_AVG) ? j.avg : nullptr;
             a.var = (j.mask & NPX_VAR) ? j.var : nullptr;
