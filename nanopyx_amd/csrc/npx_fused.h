@@ -651,27 +651,55 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 acc[q] *= den_inv;
             }
             if constexpr (SWEEP) {
-                // every sensitivity re-applies power, weight and its running mean to the same ring sums
-                float inten[Q];
+                // every sensitivity re-applies power, weight and its running mean to the same ring sums; the
+                // sensitivities and their power modes are read together (one scalar load of the adjacent fields).
+                // Pixels outside the image or the margin enter as 0 (their ring sum and weight set to 0 once per
+                // frame), and v^2 is formed once for the integer powers: v^3 = v^2 v and v^4 = v^2 v^2, the products
+                // the per-sensitivity power forms.
+                float sv[NSM];
+                int sp[NSM];
+#pragma unroll
+                for (int s = 0; s < NSM; ++s) {
+                    sv[s] = a.sens[s];
+                    sp[s] = a.sens_pow[s];
+                }
+                const int ns = a.ns;
+                float inten[Q], sq[Q];
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
                     float sacc = 0.f;
 #pragma unroll
                     for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
-                    inten[q] = a.iw ? sacc : 1.f;
+                    const bool ok = pixel_ok(q);
+                    inten[q] = ok ? (a.iw ? sacc : 1.f) : 0.f;
+                    acc[q] = ok ? acc[q] : 0.f;
+                    sq[q] = acc[q] * acc[q];
                 }
 #pragma unroll
                 for (int s = 0; s < NSM; ++s) {
-                    if (s < a.ns) {
+                    if (s < ns) {
                         float val[Q];
+                        const int pm = sp[s];
+                        if (pm == 2) {
 #pragma unroll
-                        for (int q = 0; q < Q; ++q) val[q] = acc[q];
-                        pow_all<Q>(val, a.sens_pow[s], a.sens[s]);
+                            for (int q = 0; q < Q; ++q) val[q] = sq[q];
+                        } else if (pm == 3) {
+#pragma unroll
+                            for (int q = 0; q < Q; ++q) val[q] = sq[q] * acc[q];
+                        } else if (pm == 4) {
+#pragma unroll
+                            for (int q = 0; q < Q; ++q) val[q] = sq[q] * sq[q];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < Q; ++q) val[q] = acc[q];
+                            pow_all<Q>(val, pm, sv[s]);
+                        }
 #pragma unroll
                         for (int q = 0; q < Q; ++q) {
-                            float v = a.iw ? val[q] * inten[q] : val[q];
-                            v = pixel_ok(q) ? v : 0.f;
+                            // the weight product and the deviation from the mean contract into one FMA: the mean
+                            // takes the unrounded weighted value (within half an ulp of the stored per-frame value)
+                            const float v = val[q] * inten[q];
                             mu[s * Q + q] += (v - mu[s * Q + q]) * inv_np1;
                         }
                     }
