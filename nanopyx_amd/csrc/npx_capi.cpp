@@ -189,6 +189,18 @@ LaunchCfg launch_cfg(int H, int W, int M, double R) {
     return it != g_tuned.end() ? it->second : g_default_cfg;
 }
 
+// Compute units of the current device (256 on MI355X; also when no device answers, so host-side layout queries
+// stay deterministic).
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return n;
+}
+
 LaunchCfg default_cfg() {
     std::lock_guard<std::mutex> lk(g_cfg_mu);
     return g_default_cfg;
@@ -383,9 +395,13 @@ int sweep_layout(int N, int H, int W, int n_mag, const int* mags, int n_rad, con
     });
     long long spatial = 0;
     for (const auto& j : L->jobs) spatial += static_cast<long long>(j.nregx) * j.nregy * j.ngroups;
+    // frame chunks: the fewest that give about four blocks per compute unit (with the jobs ordered longest first,
+    // every CU then runs about as many blocks of each kind; more chunks only add per-block work), each chunk >= 32
+    // frames
     long long want = 1;
     if (chunks) {
-        want = (default_cfg().target_blocks + spatial - 1) / spatial;
+        const long long blocks = 4LL * device_cus();
+        want = (blocks + spatial - 1) / spatial;
         want = std::max<long long>(1, std::min<long long>(want, N / 32));
     }
     L->fpc = static_cast<int>((N + want - 1) / want);
