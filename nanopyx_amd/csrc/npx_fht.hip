@@ -226,6 +226,7 @@ constexpr int kRowsPerBlock = 8;
 constexpr int kRowLdsPoints = 3072;  // double2 of FFT data per block (48 KiB); tables and X come on top
 constexpr int kRowLdsLimit = 64 * 1024;
 constexpr int kRowThreads = 256;
+constexpr int kRowLdsLimit256 = 160 * 1024;  // LDS one workgroup may use (fht_rows256)
 
 __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -447,6 +448,181 @@ __global__ __launch_bounds__(kRowThreads) void fht_rows(FhtTile a, int rb) {
     __syncthreads();
     if (tid == 0) {
         for (int k = 1; k < kRowThreads / 64; ++k) {
+            lo = fminf(lo, s_lo[k]);
+            hi = fmaxf(hi, s_hi[k]);
+        }
+        int* m = reinterpret_cast<int*>(a.mm) + 4 * fb;
+        atomicMin(m + 2, f2o(lo));
+        atomicMax(m + 3, f2o(hi));
+    }
+}
+
+// Row inverse for Ww = 256 (tiles of 256 input pixels and whole frames of 129-256): the same values as fht_rows, each
+// length-256 complex FFT as a four-step FFT in registers (256 = 16 x 16) instead of radix-4 stages through LDS.
+// For n = 16 n1 + n2 and m = m1 + 16 m2, e^{2 pi i n m / 256} = e^{2 pi i n1 m1 / 16} e^{2 pi i n2 m1 / 256}
+// e^{2 pi i n2 m2 / 16}: lane n2 of a 16-lane group takes the 16 points n1 of its column, a 16-point inverse DFT in
+// registers, the twiddle e^{2 pi i n2 m1 / 256}; one transpose through LDS inside the wave (no block barrier), and
+// lane m1 takes a 16-point inverse DFT over n2, giving outputs m = m1 + 16 m2, stored straight from registers. A wave
+// covers 4 rows (one per 16-lane group) and their ceil(M/2) phase pairs one after another; the column-pass values a
+// lane needs (X[q] for its 16 points, the mirror points q = 256 - k as conjugates) stay in registers for the row.
+// The phase twiddle of point q = 16 n +- j is split as e^{i t 16 n r} (wave-uniform, from LDS) times
+// e^{+-i t j r} (per lane), t = 2 pi / (256 M).
+constexpr int kR256Threads = 256;
+constexpr int kR256Rows = 16;            // rows per block: 4 waves x 4 groups of 16 lanes
+constexpr int kR256Ex = 16 * 17;         // exchange entries of one FFT: 16 rows of 16, row stride 17 (bank spread)
+
+__host__ __device__ inline size_t r256_lds_bytes(int M) {
+    const int P = (M + 1) / 2;
+    return sizeof(double2) * (static_cast<size_t>(kR256Rows) * kR256Ex + 256 + static_cast<size_t>(P) * 50);
+}
+
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cmulc(double2 a, double2 b) {  // a conj(b)
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// inverse DFT of 4 points in place: y[k] = sum_n x[n] i^{n k}
+__device__ inline void idft4(double2& x0, double2& x1, double2& x2, double2& x3) {
+    const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2), b0 = cadd(x1, x3);
+    const double2 b1 = make_double2(-(x1.y - x3.y), x1.x - x3.x);  // i (x1 - x3)
+    x0 = cadd(a0, b0);
+    x1 = cadd(a1, b1);
+    x2 = csub(a0, b0);
+    x3 = csub(a1, b1);
+}
+
+// inverse DFT of 16 points in place, n = na + 4 nb and k = k0 + 4 k1; output k is left at index 4 (k % 4) + k / 4
+__device__ inline void idft16(double2 (&a)[16]) {
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+#pragma unroll
+    for (int na = 0; na < 4; ++na) idft4(a[na], a[na + 4], a[na + 8], a[na + 12]);
+    // twiddles e^{2 pi i na k0 / 16} of element na + 4 k0
+    a[5] = cmul(a[5], make_double2(c1, s1));
+    a[9] = cmul(a[9], make_double2(h, h));
+    a[13] = cmul(a[13], make_double2(s1, c1));
+    a[6] = cmul(a[6], make_double2(h, h));
+    a[10] = make_double2(-a[10].y, a[10].x);
+    a[14] = cmul(a[14], make_double2(-h, h));
+    a[7] = cmul(a[7], make_double2(s1, c1));
+    a[11] = cmul(a[11], make_double2(-h, h));
+    a[15] = cmul(a[15], make_double2(-c1, -s1));
+#pragma unroll
+    for (int k0 = 0; k0 < 4; ++k0) idft4(a[4 * k0], a[4 * k0 + 1], a[4 * k0 + 2], a[4 * k0 + 3]);
+}
+
+__global__ __launch_bounds__(kR256Threads) void fht_rows256(FhtTile a) {
+    extern __shared__ double2 lds[];
+    const int HM = a.Hw * a.M, M = a.M, N = 256 * M, P = (M + 1) / 2;
+    constexpr int K = 128, hc = K + 1;
+    double2* ex = lds;                          // [wave][group][kR256Ex]
+    double2* tw = ex + kR256Rows * kR256Ex;     // e^{2 pi i t / 256}, t < 256
+    double2* dpe = tw + 256;                    // [P][9] e^{i t 16 n (2p)}, n <= 8
+    double2* dpo = dpe + 9 * P;                 // [P][9] e^{i t 16 n (2p + 1)}
+    double2* cpe = dpo + 9 * P;                 // [P][16] e^{i t j (2p)}
+    double2* cpo = cpe + 16 * P;                // [P][16] e^{i t j (2p + 1)}
+    const int tid = threadIdx.x;
+    for (int t = tid; t < 256 + 50 * P; t += kR256Threads) {
+        long long k;  // exponent of e^{2 pi i k / N} (or / 256 for the first table), reduced exactly
+        int den = N;
+        if (t < 256) {
+            k = t;
+            den = 256;
+        } else if (t < 256 + 18 * P) {
+            const int u = t - 256, odd = u >= 9 * P, v = u - odd * 9 * P, p = v / 9, n = v - p * 9;
+            k = 16LL * n * (2 * p + odd);
+        } else {
+            const int u = t - 256 - 18 * P, odd = u >= 16 * P, v = u - odd * 16 * P, p = v / 16, j = v - p * 16;
+            k = static_cast<long long>(j) * (2 * p + odd);
+        }
+        double sn, cs;
+        sincospi(2.0 * static_cast<double>(k % den) / den, &sn, &cs);
+        tw[t] = make_double2(cs, sn);
+    }
+    const int ngrp = (HM + kR256Rows - 1) / kR256Rows;
+    const int fb = blockIdx.x / ngrp;
+    const int wave = tid >> 6, lane = tid & 63, g = lane >> 4, j = lane & 15;
+    const int y = (blockIdx.x % ngrp) * kR256Rows + wave * 4 + g;
+    const bool row_ok = y < HM;
+    // this lane's column-pass values: point 16 i + j is X[16 i + j] for i < 8, the mirror X[256 - 16 i - j] for i >= 8
+    double2 x[16];
+    const double2* src = a.cols + static_cast<size_t>(fb) * hc * HM + (row_ok ? y : 0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int q = i < 8 ? 16 * i + j : 16 * (16 - i) - j;
+        x[i] = row_ok ? src[static_cast<size_t>(q) * HM] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();  // tables written
+    const double norm = 1.0 / (static_cast<double>(HM) * N);
+    const int yo = y - a.yroi;
+    const int oh = a.th * M, ow = a.tw * M, gh = a.h * M, gw = a.w * M;
+    const bool row_kept = row_ok && yo >= 0 && yo < oh && on_lattice(a.r0 * M + yo, a.lat, gh);
+    float* dst = a.realf + (static_cast<size_t>(fb) * HM + (row_ok ? y : 0)) * N;
+    double2* xb = ex + (wave * 4 + g) * kR256Ex;
+    float lo = INFINITY, hi = -INFINITY;
+    for (int p = 0; p < P; ++p) {
+        const bool two = 2 * p + 1 < M;
+        const double2 ce = cpe[16 * p + j], co = cpo[16 * p + j];
+        double2 v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int n = i < 8 ? i : 16 - i;
+            // phase twiddles of point q: e^{i t q 2p} and e^{i t q (2p + 1)}, q = 16 n + j (i < 8) or 16 n - j
+            const double2 te = i < 8 ? cmul(dpe[9 * p + n], ce) : cmulc(dpe[9 * p + n], ce);
+            const double2 to = i < 8 ? cmul(dpo[9 * p + n], co) : cmulc(dpo[9 * p + n], co);
+            const double2 y1 = cmul(x[i], te);
+            const double2 y2 = two ? cmul(x[i], to) : make_double2(0.0, 0.0);
+            double2 z = i < 8 ? make_double2(y1.x - y2.y, y1.y + y2.x)     // y1 + i y2
+                              : make_double2(y1.x + y2.y, -y1.y + y2.x);   // conj(y1) + i conj(y2)
+            if (i == 0 && j == 0) z = make_double2(x[0].x, two ? x[0].x : 0.0);  // DC: real part only
+            if (i == 8 && j == 0) z = make_double2(2.0 * y1.x, 2.0 * y2.x);      // k = K
+            v[i] = z;
+        }
+        idft16(v);
+        // A[m1] (at 4 (m1 % 4) + m1 / 4) times e^{2 pi i j m1 / 256}, into row m1 of the group's exchange block
+#pragma unroll
+        for (int m1 = 0; m1 < 16; ++m1) {
+            const double2 w = v[4 * (m1 & 3) + (m1 >> 2)];
+            xb[17 * m1 + j] = m1 ? cmul(w, tw[j * m1]) : w;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = xb[17 * j + n2];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // these reads are done before the next pass writes
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        idft16(v);
+        if (!row_ok) continue;
+        // output m = j + 16 m2 (at 4 (m2 % 4) + m2 / 4): phases 2p (real part) and 2p + 1 (imaginary part)
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) {
+            const double2 z = v[4 * (m2 & 3) + (m2 >> 2)];
+            const int n = M * (j + 16 * m2) + 2 * p;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                if (hh == 1 && !two) break;
+                const float val = static_cast<float>((hh ? z.y : z.x) * norm);
+                lo = fminf(lo, val);
+                hi = fmaxf(hi, val);
+                const int xo = n + hh - a.xroi;
+                if (row_kept && xo >= 0 && xo < ow && on_lattice(a.c0 * M + xo, a.lat, gw)) dst[n + hh] = val;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    __shared__ float s_lo[kR256Threads / 64], s_hi[kR256Threads / 64];
+    if (lane == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < kR256Threads / 64; ++k) {
             lo = fminf(lo, s_lo[k]);
             hi = fmaxf(hi, s_hi[k]);
         }
@@ -877,11 +1053,16 @@ int npx_fht_interpolate_lattice(const float* d_in, int N, int h, int w, int M, i
                                   reinterpret_cast<hipfftDoubleComplex*>(a.cols), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
                     return host::fail(NPX_ERR_HIP, "hipfftExecZ2Z (columns)");
                 const int rb = rows_pass(a.Ww, M);
-                if (rb > 0) {
+                if (rb > 0 || (a.Ww == 256 && M >= 2 && r256_lds_bytes(M) <= size_t(kRowLdsLimit256))) {
                     a.realf = reinterpret_cast<float*>(a.real);
-                    const int ngrp = (a.Hw * M + kRowsPerBlock - 1) / kRowsPerBlock;
-                    hipLaunchKernelGGL(fht_rows, dim3(ngrp * a.B), dim3(kRowThreads), row_shape(a.Ww, M, rb).bytes, st,
-                                       a, rb);
+                    if (a.Ww == 256 && r256_lds_bytes(M) <= size_t(kRowLdsLimit256)) {
+                        const int ngrp = (a.Hw * M + kR256Rows - 1) / kR256Rows;
+                        hipLaunchKernelGGL(fht_rows256, dim3(ngrp * a.B), dim3(kR256Threads), r256_lds_bytes(M), st, a);
+                    } else {
+                        const int ngrp = (a.Hw * M + kRowsPerBlock - 1) / kRowsPerBlock;
+                        hipLaunchKernelGGL(fht_rows, dim3(ngrp * a.B), dim3(kRowThreads), row_shape(a.Ww, M, rb).bytes,
+                                           st, a, rb);
+                    }
                     if (const int ny = kept_count(a.r0 * M, a.th * M, h * M, a.lat); ny > 0)
                     hipLaunchKernelGGL(fht_finish, dim3(ny * a.B), dim3(256), 0, st, a);
                     hipError_t e = hipGetLastError();
