@@ -559,6 +559,7 @@ __global__ __launch_bounds__(kR256Threads) void fht_rows256(FhtTile a) {
     const bool row_kept = row_ok && yo >= 0 && yo < oh && on_lattice(a.r0 * M + yo, a.lat, gh);
     float* dst = a.realf + (static_cast<size_t>(fb) * HM + (row_ok ? y : 0)) * N;
     double2* xb = ex + (wave * 4 + g) * kR256Ex;
+    const bool lat_div = a.lat >= 2 && M % a.lat == 0;
     float lo = INFINITY, hi = -INFINITY;
     for (int p = 0; p < P; ++p) {
         const bool two = 2 * p + 1 < M;
@@ -595,6 +596,14 @@ __global__ __launch_bounds__(kR256Threads) void fht_rows256(FhtTile a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         idft16(v);
         if (!row_ok) continue;
+        // columns kept (lattice output): when lat divides M, column M m + 2p + h of the grid is a multiple of lat for
+        // every m or for none, so the test is per (p, h); otherwise per column
+        const int g0 = a.c0 * M - a.xroi;  // global column of grid position 0
+        bool keep_ph[2] = {true, true};
+        if (lat_div) {
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) keep_ph[hh] = (((g0 + 2 * p + hh) % a.lat) + a.lat) % a.lat == 0;
+        }
         // output m = j + 16 m2 (at 4 (m2 % 4) + m2 / 4): phases 2p (real part) and 2p + 1 (imaginary part)
 #pragma unroll
         for (int m2 = 0; m2 < 16; ++m2) {
@@ -607,7 +616,10 @@ __global__ __launch_bounds__(kR256Threads) void fht_rows256(FhtTile a) {
                 lo = fminf(lo, val);
                 hi = fmaxf(hi, val);
                 const int xo = n + hh - a.xroi;
-                if (row_kept && xo >= 0 && xo < ow && on_lattice(a.c0 * M + xo, a.lat, gw)) dst[n + hh] = val;
+                if (!row_kept || xo < 0 || xo >= ow) continue;
+                const bool col = a.lat < 2 || (lat_div ? keep_ph[hh] || g0 + n + hh == gw - 1
+                                                       : on_lattice(g0 + n + hh, a.lat, gw));
+                if (col) dst[n + hh] = val;
             }
         }
     }
