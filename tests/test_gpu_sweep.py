@@ -63,6 +63,23 @@ def test_sweep_equals_single_launches(torch):
                                one.cpu().numpy()) <= 1e-6
 
 
+def test_sweep_every_half_width_body(torch):
+    """M4 and M8 (phases filling whole blocks: one job per radius) at half-widths K 0..6, 16 jobs in two launches:
+    each job runs the batched body built for its K; every map against the single-job kernel for its setting (the
+    sweep's mean takes the weighted value unrounded, so within 1e-6 rather than bit for bit)."""
+    from nanopyx_amd import _hip
+    x = _stack((12, 24, 40), seed=9)
+    mags, radii, sens = [4, 8], [0.4, 0.6, 1.0, 1.5, 2.0, 2.5, 3.0, 3.4], [1.0, 2.0]
+    print("half-widths", [_hip.plan_info(24, 40, 4, R)["K"] for R in radii], flush=True)
+    out = _hip.esrrf_sweep(x, mags, radii, sens, workspace=False)
+    for M in mags:
+        for R in radii:
+            for S in sens:
+                one = _hip.esrrf_temporal(x, M, R, S, True, ("AVG",), workspace=False)["AVG"]
+                assert _report(f"sweep vs single M{M} R{R} S{S}", out.maps[(M, R, S)].cpu().numpy(),
+                               one.cpu().numpy()) <= 1e-6
+
+
 def test_sweep_chunked_equals_unchunked(torch):
     from nanopyx_amd import _hip
     x = _stack((200, 32, 64), seed=6)
