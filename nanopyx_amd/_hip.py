@@ -4,6 +4,7 @@ There is no CPU path behind these functions: if the library or a ROCm GPU is mis
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -204,12 +205,61 @@ def _dtype_code(t) -> int:
     return DTYPE_U16 if t.dtype == torch.uint16 else DTYPE_F32
 
 
+K_FUSED_MAX = 6  # widest ring half-width int(2 R) the fused kernels are built for (R < 3.5)
+_COMPOSED_CHUNK_BYTES = 2 << 30  # device bytes of one frame chunk of the composed chain
+
+
+def ring_half_width(radius) -> int:
+    """Half-width int(2 R) of the RGC window in lattice points, formed in float32 as the reference forms it
+    (_c_sr_radial_gradient_convergence.c:27-28)."""
+    return int(np.float32(2.0) * np.float32(radius))
+
+
+def _wide(radius) -> bool:
+    return ring_half_width(radius) > K_FUSED_MAX
+
+
+def _composed_frames(x, M, radius, sensitivity, iw, stream):
+    """Per-frame eSRRF maps through the reference's chain of separate steps (_le_esrrf.pyx:190-194), for radii whose
+    ring window the fused kernels are not built for: Catmull-Rom xM of the frames (npx_shift_magnify), Roberts cross
+    (npx_roberts), Catmull-Rom x2M of both gradients, then the RGC on those arrays weighted by the magnified frames
+    (npx_rgc, which runs its per-pixel kernel for these radii). Frame chunks bound the x2M gradient arrays. The steps
+    allocate their outputs, so the chain is not capturable into a graph."""
+    torch = _torch()
+    if stream is not None and int(stream.cuda_stream) in _CAPTURING:
+        raise NpxUnsupported(f"radius {radius}: ring windows wider than {2 * K_FUSED_MAX + 1} lattice points run as "
+                             "separate allocating steps, which a graph capture cannot hold")
+    N, H, W = x.shape
+    HM, WM = H * M, W * M
+    per_frame = 4 * HM * WM * 10 + 8 * H * W  # magnified frame, two x2M gradients, map; Roberts pair
+    chunk = max(1, min(N, _COMPOSED_CHUNK_BYTES // per_frame))
+    if stream is not None:
+        stream.wait_stream(torch.cuda.current_stream())
+        x.record_stream(stream)
+    ctx = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+    with ctx:
+        out = torch.empty((N, HM, WM), dtype=torch.float32, device=x.device)
+        for f0 in range(0, N, chunk):
+            part = x[f0:f0 + chunk]
+            mag = shift_magnify(part, 0, 0, M, M)
+            gc, gr = roberts(part)
+            gci = shift_magnify(gc, 0, 0, 2 * M, 2 * M)
+            gri = shift_magnify(gr, 0, 0, 2 * M, 2 * M)
+            del gc, gr
+            out[f0:f0 + chunk] = rgc(gci, gri, mag, M, radius, sensitivity, iw, False)
+            del gci, gri, mag
+    return out
+
+
 def esrrf_frames(stack, magnification, radius, sensitivity, iw, stream=None):
-    """Per-frame eSRRF maps as a device tensor (N, H*M, W*M)."""
+    """Per-frame eSRRF maps as a device tensor (N, H*M, W*M). Radii beyond the fused kernels (int(2 R) > 6) run the
+    reference's chain of separate steps on the GPU."""
     torch = _torch()
     x = _dev_stack(stack)
     N, H, W = x.shape
     M = int(magnification)
+    if _wide(radius):
+        return _composed_frames(x, M, radius, sensitivity, iw, stream)
     out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
     _check(lib().npx_esrrf_frames(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
                                   int(bool(iw)), out.data_ptr(), _on(torch, stream, x, out)))
@@ -221,11 +271,19 @@ def temporal_workspace_bytes(N, H, W, M, R) -> int:
 
 
 def esrrf_temporal(stack, magnification, radius, sensitivity, iw, modes=("AVG",), stream=None, workspace=True):
-    """Fused eSRRF + temporal correlation; returns {mode: device tensor (H*M, W*M)}."""
+    """Fused eSRRF + temporal correlation; returns {mode: device tensor (H*M, W*M)}. Radii beyond the fused kernels
+    (int(2 R) > 6) take the per-frame maps of the reference's chain of separate steps, then the temporal correlation of
+    that stack (npx_temporal), as the reference does."""
     torch = _torch()
     x = _dev_stack(stack)
     N, H, W = x.shape
     M = int(magnification)
+    if _wide(radius):
+        for m in modes:
+            if m not in MODE_BITS:
+                raise ValueError("Type of correlation must be AVG, VAR or TAC2")
+        maps = _composed_frames(x, M, radius, sensitivity, iw, stream)
+        return temporal(maps, modes, stream=stream)
     mask = 0
     outs = {}
     for m in modes:
@@ -267,8 +325,14 @@ class SweepOutputs:
         self.maps = {}
         for (m, r, s) in self.key:
             self.maps[(m, r, s)] = torch.empty((H * m, W * m), dtype=torch.float32, device=x.device)
-        self.ptrs = (_c_void_p * len(self.key))(*[self.maps[k].data_ptr() for k in self.key])
-        nbytes = sweep_workspace_bytes(N, H, W, magnifications, radii, sensitivities) if workspace else 0
+        # radii the batched kernel covers go through npx_esrrf_sweep (its pointer array and workspace); wider ones
+        # through the composed chain, one setting at a time
+        self.narrow = [float(r) for r in radii if not _wide(r)]
+        self.wide = [float(r) for r in radii if _wide(r)]
+        nkey = [(int(m), r, float(s)) for m in magnifications for r in self.narrow for s in sensitivities]
+        self.ptrs = (_c_void_p * len(nkey))(*[self.maps[k].data_ptr() for k in nkey])
+        nbytes = sweep_workspace_bytes(N, H, W, magnifications, self.narrow, sensitivities) if (
+            workspace and self.narrow) else 0
         self.nbytes = nbytes
         self.work = torch.empty(nbytes, dtype=torch.uint8, device=x.device) if nbytes else None
 
@@ -277,26 +341,38 @@ def esrrf_sweep(stack, magnifications, radii, sensitivities, iw=True, stream=Non
     """eSRRF temporal AVG for every (magnification, radius, sensitivity) in one batched launch.
 
     Returns a SweepOutputs whose ``maps[(M, R, S)]`` are device tensors. Passing the SweepOutputs of an earlier call
-    with the same stack shape and lists reuses its buffers."""
+    with the same stack shape and lists reuses its buffers. Radii beyond the fused kernels (int(2 R) > 6) are left
+    out of the launch and computed per setting through the reference's chain of separate steps."""
     torch = _torch()
     x = _dev_stack(stack)
     N, H, W = x.shape
-    mags, rads, sens = _sweep_lists(magnifications, radii, sensitivities)
+    if not magnifications or not radii or not sensitivities:
+        raise ValueError("a sweep needs at least one magnification, radius and sensitivity")
+    lists = ([int(m) for m in magnifications], [float(r) for r in radii], [float(s) for s in sensitivities])
+    narrow = [r for r in lists[1] if not _wide(r)]
     if outputs is not None:
-        lists = ([int(m) for m in magnifications], [float(r) for r in radii], [float(s) for s in sensitivities])
         if outputs.shape != (N, H, W) or outputs.lists != lists:
             raise ValueError(f"outputs were made for a {outputs.shape} stack and settings {outputs.lists}, not "
                              f"{(N, H, W)} and {lists}")
-        if len(outputs.ptrs) != len(mags) * len(rads) * len(sens):
+        if len(outputs.ptrs) != len(lists[0]) * len(narrow) * len(lists[2]):
             raise ValueError("outputs hold a different number of maps than the settings need")
-        if outputs.work is not None and outputs.nbytes < sweep_workspace_bytes(N, H, W, magnifications, radii,
-                                                                                sensitivities):
+        if outputs.work is not None and narrow and outputs.nbytes < sweep_workspace_bytes(N, H, W, magnifications,
+                                                                                           narrow, sensitivities):
             raise ValueError("outputs' workspace is smaller than this sweep needs")
     out = outputs if outputs is not None else SweepOutputs(torch, x, magnifications, radii, sensitivities, workspace)
-    _check(lib().npx_esrrf_sweep(x.data_ptr(), _dtype_code(x), N, H, W, len(mags), mags, len(rads), rads, len(sens),
-                                 sens, int(bool(iw)), out.ptrs,
-                                 out.work.data_ptr() if out.work is not None else None, out.nbytes,
-                                 _on(torch, stream, x, out.work, *out.maps.values())))
+    if narrow:
+        mags, rads, sens = _sweep_lists(magnifications, narrow, sensitivities)
+        _check(lib().npx_esrrf_sweep(x.data_ptr(), _dtype_code(x), N, H, W, len(mags), mags, len(rads), rads,
+                                     len(sens), sens, int(bool(iw)), out.ptrs,
+                                     out.work.data_ptr() if out.work is not None else None, out.nbytes,
+                                     _on(torch, stream, x, out.work, *out.maps.values())))
+    for m in lists[0]:
+        for r in out.wide:
+            for s in lists[2]:
+                avg = esrrf_temporal(x, m, r, s, iw, ("AVG",), stream=stream)["AVG"]
+                ctx = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+                with ctx:
+                    out.maps[(m, r, s)].copy_(avg)
     return out
 
 

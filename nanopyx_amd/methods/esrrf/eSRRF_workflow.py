@@ -4,7 +4,8 @@ Same signature and output: frames are split into timepoints of ``frames_per_time
 timepoint becomes one (H*M, W*M) map of the requested temporal correlation, optionally macro-pixel corrected,
 and the (T, H*M, W*M) float32 result is squeezed. On this path each timepoint is one fused launch
 (eSRRF + AVG/VAR/TAC2, ``npx_esrrf_temporal``) followed by the macro-pixel kernel; the stack is uploaded once and
-the per-frame maps never leave the GPU. A host (NumPy) stack of 64 MiB or more is streamed instead
+the per-frame maps never leave the GPU (radii with int(2 R) > 6 run the reference's separate steps on the GPU and
+reduce their per-frame maps). A host (NumPy) stack of 64 MiB or more is streamed instead
 (``nanopyx_amd.upload``): pieces go to the device through pinned staging buffers while a serving session's jobs, one
 per timepoint, already run on the frames that have arrived. With ``use_fht`` each timepoint runs the
 Fourier-interpolation branch of the operator (per-frame maps on the GPU), then ``npx_temporal`` and the macro-pixel
@@ -29,7 +30,7 @@ def eSRRF(image, magnification: int = 5, radius: float = 1.5, sensitivity: float
     import torch
 
     is_tensor = isinstance(image, torch.Tensor)
-    if not is_tensor and not use_fht:
+    if not is_tensor and not use_fht and not _hip._wide(radius):
         x = check_image(image, keep_uint16=True)
         if x.nbytes >= upload.STREAM_MIN_BYTES:
             # a large host stack: uploaded in pieces while the timepoints' kernels already run on the arrived frames

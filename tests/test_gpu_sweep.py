@@ -135,8 +135,8 @@ def test_sweep_errors(torch):
     x = _stack((2, 8, 8))
     with pytest.raises(ValueError):
         _hip.esrrf_sweep(x, [], [1.0], [1.0])
-    with pytest.raises(_hip.NpxError):
-        _hip.esrrf_sweep(x, [2], [3.6], [1.0])  # ring half-width 7 > 6
+    with pytest.raises(ValueError):
+        _hip.esrrf_sweep(x, [2], [1.0], [])
 
 
 def test_sweep_outputs_reuse_checked(torch):
