@@ -157,20 +157,21 @@ inline size_t lds_bytes_kq(int nwaves, int mode) {
 #define NPX_RING_GROUPS 1
 #endif
 
-// Window rows [J0, J1) of the ring sums for column i: constants of those rows in VGPRs, lattice rows streamed.
-template <int K, int KL, int Q, int J0, int J1>
-__device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, float dxn, const float* rl, int i) {
+// Window rows [J0, J1) of the ring sums for column i whose offset dx has sign SG (+1, -1, or 0 for dx = 0): constants
+// of those rows in VGPRs, lattice rows streamed. rs = 1 / |dx| (1 where dx = 0).
+template <int K, int KL, int Q, int J0, int J1, int SG>
+__device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, float rs, const float* rl, int i) {
     using GL = Geometry<KL, Q>;
     constexpr int NW = 2 * K + 1;
     constexpr int LP = GL::LP;
     constexpr int NJ = J1 - J0;
     constexpr int R0 = J0, R1 = 2 * (Q - 1) + J1;  // lattice rows these window rows read: [R0, R1)
     constexpr int RDT = 2 * NW;
-    float dyn[NJ], tdv[NJ], tsv[NJ];
+    float cj[NJ], tdv[NJ], tsv[NJ];
     const float2* dt = reinterpret_cast<const float2*>(rl + RDT) + i * NW + J0;
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj) {
-        dyn[jj] = rl[J0 + jj];
+        cj[jj] = SG == 0 ? rl[J0 + jj] : rl[J0 + jj] * rs;  // -dy / |dx| (-dy where dx = 0)
         const float2 e = dt[jj];
         tdv[jj] = e.x;
         tsv[jj] = e.y;
@@ -180,45 +181,77 @@ __device__ __forceinline__ void ring_rows(float (&acc)[Q], const float2* col, fl
     float2 g2 = col[(R0 + 1 < R1 ? R0 + 1 : R0) * LP];
 #pragma unroll
     for (int r = R0; r < R1; ++r) {
-        const float2 g = g1;
+        const float2 g = g1;  // s g, g = G / |G|
         g1 = g2;
         if (r + 2 < R1) g2 = col[(r + 2) * LP];
-        const float pxn = g.x * dxn;     // -s g.x dx
-        const float pys = -(g.y * dxn);  //  s g.y dx
 #pragma unroll
         for (int jj = 0; jj < NJ; ++jj) {
             const int rq = r - (J0 + jj);
             if (rq < 0 || (rq & 1) || (rq >> 1) >= Q) continue;
             const int q = rq >> 1;
-            const float c = __builtin_amdgcn_fmed3f(fmaf(g.y, dyn[jj], pxn), 0.f, 1.f);  // clamp(-s u, 0, 1)
-            const float vs = fmaf(g.x, dyn[jj], pys);                                    // s v
-            const float x = fmaf(fabsf(vs), tsv[jj], tdv[jj]);                            // t (d - |v|)
+            // z = -s u / |dx| and vs = s v / |dx| (u = g.d, v = g x d): sgn(dx) enters as the sign of one operand
+            float z, vs;
+            if constexpr (SG > 0) {
+                z = fmaf(g.y, cj[jj], -g.x);
+                vs = fmaf(g.x, cj[jj], g.y);
+            } else if constexpr (SG < 0) {
+                z = fmaf(g.y, cj[jj], g.x);
+                vs = fmaf(g.x, cj[jj], -g.y);
+            } else {
+                z = g.y * cj[jj];
+                vs = g.x * cj[jj];
+            }
+            const float c = __builtin_amdgcn_fmed3f(z, 0.f, 1.f);  // clamp(-s u / |dx|, 0, 1)
+            const float x = fmaf(fabsf(vs), tsv[jj], tdv[jj]);      // t (d - |v|)
             acc[q] = fmaf(x, c, acc[q]);
         }
     }
 }
 
+// One window column of the ring sums, its rows taken in NPX_RING_GROUPS groups so that only one group's constants
+// are live at a time (scheduling barriers keep each group's loads inside the group).
+template <int K, int KL, int Q, int SG>
+__device__ __forceinline__ void ring_col(float (&acc)[Q], const float2* col, float rs, const float* rl, int i) {
+    constexpr int NW = 2 * K + 1;
+    constexpr int NG = NPX_RING_GROUPS < NW ? NPX_RING_GROUPS : NW;
+    constexpr int JA = (NW + NG - 1) / NG;  // rows per group (the last group may be shorter)
+    ring_rows<K, KL, Q, 0, (JA < NW ? JA : NW), SG>(acc, col, rs, rl, i);
+    if constexpr (JA < NW) {
+        __builtin_amdgcn_sched_barrier(0);
+        ring_rows<K, KL, Q, JA, (2 * JA < NW ? 2 * JA : NW), SG>(acc, col, rs, rl, i);
+    }
+    if constexpr (2 * JA < NW) {
+        __builtin_amdgcn_sched_barrier(0);
+        ring_rows<K, KL, Q, 2 * JA, (3 * JA < NW ? 3 * JA : NW), SG>(acc, col, rs, rl, i);
+    }
+    if constexpr (3 * JA < NW) {
+        __builtin_amdgcn_sched_barrier(0);
+        ring_rows<K, KL, Q, 3 * JA, NW, SG>(acc, col, rs, rl, i);
+    }
+}
+
 // Ring sums of one wave (radius half-width K) over a lattice built for half-width KL >= K. Sample (j, i) of
-// pixel q reads lattice row 2q + ey + j + (KL - K), column 2*lane + ex + i + (KL - K). With u = g.(dx, dy)
-// and v = g x (dx, dy) (g = G/|G|, d = |(dx, dy)|, t = w/d), the reference's contribution
-// [G.d < 0] * w * (1 - |G x d| / (|G| d)) is [u < 0] * t * (d - |v|). The indicator is taken as
-// clamp(-s u, 0, 1) with s = kRingScale, which the compiler folds into the clamp bit of the FMA forming -s u, so
-// a sample costs four FMAs: it equals the indicator for |u| >= 1/s, and below that the term it scales is itself
-// below t u^2 / d (u^2 + v^2 = d^2, so d - |v| <= u^2 / d), about 1e-10 t at s = 2^16. g.x*dx and g.y*dx are
-// formed once per lattice point of the column and shared by the samples that read it. The ring constants come from the wave's
-// LDS copy, so they sit in VGPRs (on gfx950 a VALU operation with an SGPR operand issues at about half the rate
-// of one whose operands are all VGPRs); the window rows are taken in NPX_RING_GROUPS groups so that only one
-// group's constants are live at a time.
+// pixel q reads lattice row 2q + ey + j + (KL - K), column 2*lane + ex + i + (KL - K). The lattice holds s g with
+// g = G/|G| and s = kRingScale. With u = g.(dx, dy) and v = g x (dx, dy) (d = |(dx, dy)|, t = w/d), the
+// reference's contribution [G.d < 0] * w * (1 - |G x d| / (|G| d)) is [u < 0] * t * (d - |v|). Both u and v are
+// taken divided by |dx| (a positive factor of the column, 1 where dx = 0), which leaves the sign of u and, with the
+// factor folded into the constant of |v|, the term unchanged: -s u / |dx| = s g.y c_j - sgn(dx) s g.x and
+// s v / |dx| = s g.x c_j + sgn(dx) s g.y with c_j = -dy_j / |dx|, each one FMA of the lattice value with a
+// per-sample constant, so a sample costs four FMAs and a lattice point no products of its own. The indicator is
+// taken as clamp(-s u / |dx|, 0, 1), which the compiler folds into the clamp bit of the FMA: it equals [u < 0] for
+// |u| >= |dx| / s, and below that the term it scales is itself below t u^2 / d (u^2 + v^2 = d^2, so
+// d - |v| <= u^2 / d), under 3e-9 t / d at s = 2^16 and |dx| <= 3.5. The ring constants come from the wave's LDS copy,
+// so they sit in VGPRs (on gfx950 a VALU operation with an SGPR operand issues at about half the rate of one whose
+// operands are all VGPRs). The window columns with dx < 0, dx = 0 and dx > 0 are the ranges [0, ineg),
+// [ineg, inz) and [inz, NW) (dx grows with the column), each with its own loop body.
 template <int K, int KL, int Q>
 __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
-                                         const float* rl) {
+                                         const float* rl, int ineg, int inz) {
     using GL = Geometry<KL, Q>;
     constexpr int NW = 2 * K + 1;
     constexpr int LP = GL::LP, LPL = GL::LR * GL::LP;
     constexpr int DK = KL - K;
     constexpr int RDX = NW;
-    constexpr int NG = NPX_RING_GROUPS < NW ? NPX_RING_GROUPS : NW;
-    constexpr int JA = (NW + NG - 1) / NG;  // rows per group (the last group may be shorter)
     static_assert(K <= KL, "ring wider than the lattice");
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0.f;
@@ -227,41 +260,41 @@ __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, i
     // without this the oldest of its waves runs ahead and the youngest finishes the ring sums alone at a fraction of
     // the issue rate; with it the waves that are behind are preferred and the waves of a SIMD finish close together.
     constexpr int P2 = NW / 3, P1 = 2 * NW / 3, P0 = NW - 1;
-    __builtin_amdgcn_s_setprio(3);
-#pragma unroll 1
-    for (int i = 0; i < NW; ++i) {
+    auto prio = [&](int i) {
         if (i == P2) __builtin_amdgcn_s_setprio(2);
         else if (i == P1) __builtin_amdgcn_s_setprio(1);
         else if (i == P0) __builtin_amdgcn_s_setprio(0);
+    };
+    auto column = [&](int i) {
         const int c = ex + i + DK;
-        const float2* col = s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
-        const float dxn = rl[RDX + i];  // -s dx
-        // scheduling barriers keep each group's loads inside the group (no hoisting that would keep every
-        // group's constants live at once)
-        ring_rows<K, KL, Q, 0, (JA < NW ? JA : NW)>(acc, col, dxn, rl, i);
-        if constexpr (JA < NW) {
-            __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, JA, (2 * JA < NW ? 2 * JA : NW)>(acc, col, dxn, rl, i);
-        }
-        if constexpr (2 * JA < NW) {
-            __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, 2 * JA, (3 * JA < NW ? 3 * JA : NW)>(acc, col, dxn, rl, i);
-        }
-        if constexpr (3 * JA < NW) {
-            __builtin_amdgcn_sched_barrier(0);
-            ring_rows<K, KL, Q, 3 * JA, NW>(acc, col, dxn, rl, i);
-        }
+        return s_lat + (c & 1) * LPL + (ey + DK) * LP + lane + (c >> 1);
+    };
+    __builtin_amdgcn_s_setprio(3);
+#pragma unroll 1
+    for (int i = 0; i < ineg; ++i) {
+        prio(i);
+        ring_col<K, KL, Q, -1>(acc, column(i), -rl[RDX + i], rl, i);  // rdx = 1 / dx < 0
+    }
+#pragma unroll 1
+    for (int i = ineg; i < inz; ++i) {
+        prio(i);
+        ring_col<K, KL, Q, 0>(acc, column(i), 1.f, rl, i);
+    }
+#pragma unroll 1
+    for (int i = inz; i < NW; ++i) {
+        prio(i);
+        ring_col<K, KL, Q, 1>(acc, column(i), rl[RDX + i], rl, i);
     }
 }
 
 // Dispatch on the wave's (uniform) half-width for lattices of half-width KL.
 template <int KL, int Q>
 __device__ __forceinline__ void ring_sum_any(int K, float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,
-                                             const float* rg) {
+                                             const float* rg, int ineg, int inz) {
     switch (K) {
 #define NPX_RING_CASE(k)                                                             \
     case k:                                                                          \
-        if constexpr (k <= KL) ring_sum<(k <= KL ? k : KL), KL, Q>(acc, s_lat, ey, ex, lane, rg); \
+        if constexpr (k <= KL) ring_sum<(k <= KL ? k : KL), KL, Q>(acc, s_lat, ey, ex, lane, rg, ineg, inz); \
         break;
         NPX_RING_CASE(0)
         NPX_RING_CASE(1)
@@ -343,7 +376,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     const int margin = active ? a.margin[rad] : 0;
 
     // ---- stage the frame-invariant tables. Per wave (block of RWL floats): the ring constants of its radius and
-    // phase in the form the ring sums use (-s dy[j], -s dx[i], (t d, -t / s)[i][j] with s = kRingScale), then the
+    // phase in the form the ring sums use (-dy[j], 1 / dx[i] (0 for dx = 0), (t d, -t |dx| / s)[i][j] with
+    // s = kRingScale and |dx| taken as 1 for dx = 0), then the
     // normaliser part of its ring block (wsum, then the (NW+1)^2 prefix sums of w).
     for (int k = tid; k < a.nwaves * RWL; k += nthr) {
         const int w = k / RWL, e = k - w * RWL;
@@ -353,9 +387,19 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             const int nw = 2 * a.K[rw] + 1;
             const float* blk = a.ring[rw] + static_cast<size_t>(pw) * a.rs[rw];
             if (e < RCL) {
-                if (e < 2 * nw) v = -kRingScale * blk[e];
-                else if (e < 2 * nw + 2 * nw * nw)  // host pairs (d, t)
-                    v = ((e - 2 * nw) & 1) ? blk[e] * (-1.f / kRingScale) : blk[e + 1] * blk[e];
+                if (e < nw) {
+                    v = -blk[e];  // -dy
+                } else if (e < 2 * nw) {
+                    v = blk[e] != 0.f ? 1.0f / blk[e] : 0.f;  // 1 / dx, 0 where dx = 0
+                } else if (e < 2 * nw + 2 * nw * nw) {      // host pairs (d, t) of column (e - 2 nw) / (2 nw)
+                    const int pe = e - 2 * nw;
+                    if (pe & 1) {
+                        const float dxa = fabsf(blk[nw + pe / (2 * nw)]);
+                        v = blk[e] * (-1.f / kRingScale) * (dxa != 0.f ? dxa : 1.f);  // -t |dx| / s
+                    } else {
+                        v = blk[e + 1] * blk[e];  // t d
+                    }
+                }
             } else if (e - RCL < 1 + (nw + 1) * (nw + 1)) {
                 v = blk[2 * nw + 3 * nw * nw + (e - RCL)];
             }
@@ -477,6 +521,18 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
     __syncthreads();
     const float* ringC = s_ring + wave * RWL;  // ring constants of this wave
     const float* ringL = ringC + RCL;          // wsum, prefix sums
+    // the wave's window columns with dx < 0 are [0, ring_ineg), with dx = 0 [ring_ineg, ring_inz)
+    int ring_ineg = 0, ring_inz = 0;
+    for (int i = 0; i < NWw; ++i) {
+        const float rdx = ringC[NWw + i];
+        ring_ineg += rdx < 0.f ? 1 : 0;
+        ring_inz += rdx <= 0.f ? 1 : 0;
+    }
+    ring_ineg = __builtin_amdgcn_readfirstlane(ring_ineg);
+    ring_inz = __builtin_amdgcn_readfirstlane(ring_inz);
+    // 1 / sum(w) of the wave's ring (frame-invariant): interior pixels multiply by it
+    const float wsum_inv = __int_as_float(
+        __builtin_amdgcn_readfirstlane(__float_as_int(ringL[0] > 0.f ? 1.0f / ringL[0] : 0.f)));
 
     // intensity Catmull-Rom row pass, shared by the waves of one phase row: T[set][q][c] = sum_u w_u tile[b+u][c]
     // (the 4 x 4 taps lie inside the tile for every lane: tap base floor(r - 0.5) - 1 with r in [y0, y0 + Q),
@@ -499,7 +555,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         if (!ARRAYS && T_OWN && a.iw) intensity_rows();
         if constexpr (ARRAYS) {
             // lattice point (a, b) is the caller's gradient at row min(a M, 2 HM - 1), column min(b M, 2 WM - 1) of
-            // its x2M arrays, normalised (_c_sr_radial_gradient_convergence.c:45-52 reads those positions); points
+            // its x2M arrays, normalised and scaled by s = kRingScale (_c_sr_radial_gradient_convergence.c:45-52 reads
+            // those positions); points
             // outside a in [1, 2 HM - 2], b in [1, 2 WM - 2] are outside every window and hold 0
             const size_t P4 = static_cast<size_t>(4) * a.HM * a.WM;
             const int CW = 2 * a.W + 1;  // compact lattice: row min(a, 2H), column min(b, 2W)
@@ -520,7 +577,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                     const float gxv = a.agx[o], gyv = a.agy[o];
                     const float n2 = gxv * gxv + gyv * gyv;
                     if (n2 > 0.f) {
-                        const float inv = rsqrtf(n2);
+                        const float inv = rsqrtf(n2) * kRingScale;  // s / |G|
                         g = make_float2(gxv * inv, gyv * inv);
                     }
                 }
@@ -564,7 +621,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 s_rxy[idx] = make_float2(sx, sy);
             }
             __syncthreads();
-            // (3b) ... then along columns, normalised: lattice point (a, b) holds G/|G| at (a/2, b/2); stored in two
+            // (3b) ... then along columns, normalised: lattice point (a, b) holds s G/|G| (s = kRingScale) at (a/2, b/2); stored in two
             // column-parity planes so the 64 lanes of one read touch 64 consecutive float2. A thread keeps one
             // lattice column (its taps and weights) and takes every rg_n-th lattice row of it.
             {
@@ -583,17 +640,20 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                     for (int v = 0; v < 4; ++v) tcv[v] = min(max(cb + v, 0), TGC - 1);
                     for (int la = rg; la < LR; la += rg_n) {
                         const float2* row = s_rxy + la * TGC;
+                        // the four taps are read before the first is used, so their LDS reads overlap
+                        float2 rv[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) rv[v] = row[tcv[v]];
                         float gx = 0.f, gy = 0.f;
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const float2 rv = row[tcv[v]];
-                            gx += rv.x * wcv[v];
-                            gy += rv.y * wcv[v];
+                            gx += rv[v].x * wcv[v];
+                            gy += rv[v].y * wcv[v];
                         }
                         const float n2 = gx * gx + gy * gy;
                         float2 g = make_float2(0.f, 0.f);
                         if (n2 > 0.f) {
-                            const float inv = rsqrtf(n2);
+                            const float inv = rsqrtf(n2) * kRingScale;  // s / |G|
                             g = make_float2(gx * inv, gy * inv);
                         }
                         s_lat[plane * LPL + la * LP + pos] = g;
@@ -616,16 +676,14 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         if (active) {
             // (4) ring sums
             float acc[Q];
-            if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, ringC);
-            else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, ringC);
+            if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, ringC, ring_ineg, ring_inz);
+            else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, ringC, ring_ineg, ring_inz);
             // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Each step runs over
             // the lane's Q pixels, so the wave-uniform decisions (power, weighting, first frame) are taken once per
             // frame; pixels whose window crosses the border take their normaliser from the prefix-sum table.
             const float wcv[4] = {wcol.x, wcol.y, wcol.z, wcol.w};
             const int nabs = n0 + (f - f_begin);  // values already folded into the moments
             const float inv_np1 = 1.0f / static_cast<float>(nabs + 1);
-            const float wsum = ringL[0];
-            const float wsum_inv = wsum > 0.f ? 1.0f / wsum : 0.f;  // once per frame; pixels multiply by it
             const float* P = ringL + 1;
             const int b0 = 2 * xi + ex;
             const int i0 = max(0, 1 - (b0 - Kw));
@@ -665,12 +723,20 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
                 }
                 const int ns = a.ns;
                 float inten[Q], sq[Q];
+                // the intensity row values of all Q pixels are read before the first is used (their LDS reads overlap)
+                float tv[Q][4];
+                {
+                    const float* rowp = s_irow + rset * Q * TIC + ccol;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+#pragma unroll
+                        for (int v2 = 0; v2 < 4; ++v2) tv[q][v2] = rowp[q * TIC + v2];
+                }
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
-                    const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
                     float sacc = 0.f;
 #pragma unroll
-                    for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
+                    for (int v2 = 0; v2 < 4; ++v2) sacc += tv[q][v2] * wcv[v2];
                     const bool ok = pixel_ok(q);
                     inten[q] = ok ? (a.iw ? sacc : 1.f) : 0.f;
                     acc[q] = ok ? acc[q] : 0.f;
@@ -707,19 +773,32 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
             } else {
                 pow_all<Q>(acc, a.sens_pow[0], a.sens[0]);
                 if (a.iw) {
+                    if constexpr (ARRAYS) {
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) {
-                        float sacc;
-                        if constexpr (ARRAYS) {
+                        for (int q = 0; q < Q; ++q) {
                             const int rM = min(M * (y0 + q) + py, a.HM - 1);
-                            sacc = a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs];
-                        } else {
-                            const float* rowp = s_irow + (rset * Q + q) * TIC + ccol;
-                            sacc = 0.f;
-#pragma unroll
-                            for (int v2 = 0; v2 < 4; ++v2) sacc += rowp[v2] * wcv[v2];
+                            acc[q] = acc[q] * a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs];
                         }
-                        acc[q] = acc[q] * sacc;
+                    } else {
+                        // the row values of up to 8 pixels are read before the first of them is used, so that their
+                        // LDS reads overlap instead of waiting one pixel at a time
+                        constexpr int QG = Q < 8 ? Q : 8;
+                        const float* rowp = s_irow + rset * Q * TIC + ccol;
+#pragma unroll
+                        for (int g = 0; g < Q; g += QG) {
+                            float tv[QG][4];
+#pragma unroll
+                            for (int q = 0; q < QG; ++q)
+#pragma unroll
+                                for (int v2 = 0; v2 < 4; ++v2) tv[q][v2] = rowp[(g + q) * TIC + v2];
+#pragma unroll
+                            for (int q = 0; q < QG; ++q) {
+                                float sacc = 0.f;
+#pragma unroll
+                                for (int v2 = 0; v2 < 4; ++v2) sacc += tv[q][v2] * wcv[v2];
+                                acc[g + q] = acc[g + q] * sacc;
+                            }
+                        }
                     }
                 }
 #pragma unroll

@@ -1,8 +1,8 @@
 """CPU model of the fused gfx950 kernel (nanopyx_amd/csrc/npx_kernels.hip, esrrf_fused), block by block.
 
 It repeats the kernel's index arithmetic statement for statement — tiles, halos, the row then column Catmull-Rom
-passes of the lattice, the two column-parity lattice planes, the per-lane column sweep with its per-lattice-point
-products, and the edge normaliser — on the library's own plan tables, in float32, and
+passes of the lattice, the two column-parity lattice planes, the per-lane column sweep (u and v taken divided by
+|dx|, the column's sign in the operands), and the edge normaliser — on the library's own plan tables, in float32, and
 asserts that every LDS / table / global index the kernel forms stays inside its buffer and that every tap with a
 non-zero weight reads inside its tile (clamped taps must carry weight 0). Tests run it on small stacks against the
 oracle, so an indexing mistake shows up on the CPU before the kernel ever runs on a GPU.
@@ -14,7 +14,7 @@ import numpy as np
 from nanopyx_amd import _hip
 
 f32 = np.float32
-RING_SCALE = 65536.0  # kRingScale of the kernel: s in the ring sums' indicator clamp(-s u, 0, 1)
+RING_SCALE = 65536.0  # kRingScale of the kernel: the lattice holds s G/|G|; indicator clamp(-s u / |dx|, 0, 1)
 
 
 def _fma(a, b, c):
@@ -190,21 +190,32 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
                             # each read stays inside its own column-parity plane
                             assert ((rows - (c & 1) * LPL) < LPL).all()
                             g = s_lat[rows]                       # (NR, 64, 2)
-                            # the kernel stages -s dx, -s dy, t d and -t / s (s = RING_SCALE)
-                            dxn = np.float32(-RING_SCALE * dxt[ph, i])
+                            # the kernel stages -dy, 1 / dx (0 for dx = 0), t d and -t |dx| / s (s = RING_SCALE,
+                            # |dx| taken as 1 for dx = 0); the lattice holds s g, and u, v enter divided by |dx|
+                            dxv = np.float32(dxt[ph, i])
+                            rdx = np.float32(np.float32(1.0) / dxv) if dxv != 0 else np.float32(0)
+                            sg = 1 if rdx > 0 else (-1 if rdx < 0 else 0)
+                            rs = np.float32(abs(rdx)) if sg else np.float32(1)
+                            dxa = np.float32(abs(dxv)) if sg else np.float32(1)
+                            gs = (g * np.float32(RING_SCALE)).astype(np.float32)  # exact: a power of two
                             for r in range(NR):
-                                pxn = (g[r, :, 0] * dxn).astype(np.float32)
-                                pys = (-(g[r, :, 1] * dxn)).astype(np.float32)
                                 for jj in range(NW):
                                     rq = r - jj
                                     if rq < 0 or rq % 2 or rq // 2 >= Q:
                                         continue
                                     q = rq // 2
-                                    dyn = np.float32(-RING_SCALE * dyt[ph, jj])
+                                    dyn = np.float32(-dyt[ph, jj])
+                                    cj = dyn if sg == 0 else np.float32(dyn * rs)
                                     td = np.float32(ttab[ph, i, jj] * dtab[ph, i, jj])
-                                    ts = np.float32(ttab[ph, i, jj] * np.float32(-1.0 / RING_SCALE))
-                                    c = np.clip(_fma(g[r, :, 1], dyn, pxn), 0, 1).astype(np.float32)
-                                    vs = _fma(g[r, :, 0], dyn, pys)
+                                    ts = np.float32(np.float32(ttab[ph, i, jj] * np.float32(-1.0 / RING_SCALE)) * dxa)
+                                    gx, gy = gs[r, :, 0], gs[r, :, 1]
+                                    if sg > 0:
+                                        z, vs = _fma(gy, cj, -gx), _fma(gx, cj, gy)
+                                    elif sg < 0:
+                                        z, vs = _fma(gy, cj, gx), _fma(gx, cj, -gy)
+                                    else:
+                                        z, vs = (gy * cj).astype(np.float32), (gx * cj).astype(np.float32)
+                                    c = np.clip(z, 0, 1).astype(np.float32)
                                     x = _fma(np.abs(vs), ts, td)
                                     acc[q] = _fma(x, c, acc[q])
                         cMs = np.minimum(cM, WM - 1)

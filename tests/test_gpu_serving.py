@@ -9,6 +9,11 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
+# Served jobs run their frames in slices and chunks of their own, so their moments are merged at other frames than a
+# plain call's: the merged float32 moments then differ by rounding alone, bounded as for the temporal kernel's chunked
+# against single-pass comparison (TAC2, a difference of near-equal products, moves most). Where the frame chunks are
+# the same the tests ask for equality bit for bit.
+CHUNK_TOL = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -95,7 +100,7 @@ def test_served_equals_plain_call(torch):
     for i, r in enumerate(reqs):
         plain = _hip.esrrf_temporal(r.stack.cuda(), r.magnification, r.radius, r.sensitivity, True, (r.mode,),
                                     workspace=False)[r.mode].cpu().numpy()
-        assert _report(f"job {i} served vs plain", served[i], plain) <= 1e-6
+        assert _report(f"job {i} served vs plain", served[i], plain) <= CHUNK_TOL
 
 
 def test_served_matches_oracle(torch, oracle_built):
@@ -151,7 +156,7 @@ def test_serve_loop_small(torch):
     for r in reqs:
         plain = __import__("nanopyx_amd")._hip.esrrf_temporal(r.stack.cuda(), r.magnification, r.radius,
                                                                 r.sensitivity, True, (r.mode,), workspace=False)
-        assert rel_err(r.outputs[r.mode].cpu().numpy(), plain[r.mode].cpu().numpy())[0] <= 1e-6
+        assert rel_err(r.outputs[r.mode].cpu().numpy(), plain[r.mode].cpu().numpy())[0] <= CHUNK_TOL
 
 
 @pytest.mark.parametrize("N,H,W,M,R,mode,steps", [
@@ -161,7 +166,7 @@ def test_serve_loop_small(torch):
 ])
 def test_host_stack_in_pieces(torch, N, H, W, M, R, mode, steps):
     """A host stack is uploaded in pieces while its first slices run: the maps equal those of the same job with the
-    stack already on the device, bit for bit, and the plain fused call (<= 1e-6)."""
+    stack already on the device, bit for bit, and the plain fused call (<= CHUNK_TOL)."""
     from nanopyx_amd import _hip, serving, synthetic
     print(f"host pieces N{N} {H}x{W} M{M} R{R} {mode}", flush=True)
     sc = synthetic.Scene(N, H, W, max(1, int(0.02 * H * W)), 1.5, 500, 0.3, 100, 0.0, 2.0, 0.0, "float32", 77)
@@ -174,7 +179,7 @@ def test_host_stack_in_pieces(torch, N, H, W, M, R, mode, steps):
     print("session stats", st, flush=True)
     assert np.array_equal(got[0], got[1]), "host-source maps differ from device-source maps"
     plain = _hip.esrrf_temporal(x, M, R, 1, True, (mode,), workspace=False)[mode].cpu().numpy()
-    assert _report(f"host pieces {mode} vs plain", got[0], plain) <= 1e-6
+    assert _report(f"host pieces {mode} vs plain", got[0], plain) <= CHUNK_TOL
 
 
 @pytest.mark.parametrize("N,H,W,M,mode", [
@@ -185,7 +190,7 @@ def test_host_stack_stepped_after_upload(torch, N, H, W, M, mode):
     """A host stack whose upload has fully landed before the session's next step: the step may take every frame at
     once, and its slice must still run no more frame chunks than the job has partial slots (a longer slice would
     write partial moments past its slots, into the job's own stack). Maps equal the device-source job and
-    the plain fused call (<= 1e-6); bit for bit when both have the same frame chunks."""
+    the plain fused call (<= CHUNK_TOL); bit for bit when both have the same frame chunks."""
     from nanopyx_amd import _hip, synthetic
     print(f"host stack stepped after upload N{N} {H}x{W} M{M} {mode}", flush=True)
     plan = _hip.Session.slice_plan(N, H, W, M, 1.5, False, True)
@@ -217,9 +222,9 @@ def test_host_stack_stepped_after_upload(torch, N, H, W, M, mode):
     dev_plan = _hip.Session.slice_plan(N, H, W, M, 1.5, False, False)
     if dev_plan["cfpc"] == plan["cfpc"]:
         assert np.array_equal(maps[0], maps[1]), "host-source maps differ from device-source maps"
-    assert _report(f"host after upload {mode} vs device source", maps[0], maps[1]) <= 1e-6
+    assert _report(f"host after upload {mode} vs device source", maps[0], maps[1]) <= CHUNK_TOL
     plain = _hip.esrrf_temporal(x, M, 1.5, 1, True, (mode,), workspace=False)[mode].cpu().numpy()
-    assert _report(f"host after upload {mode} vs plain", maps[0], plain) <= 1e-6
+    assert _report(f"host after upload {mode} vs plain", maps[0], plain) <= CHUNK_TOL
 
 
 def test_fed_job_errors(torch):
@@ -254,7 +259,7 @@ def test_fed_job_errors(torch):
 ])
 def test_workflow_streams_host_stack(torch, N, H, W, M, R, S, fpt, mode, dtype, mpc):
     """eSRRF() on a host stack of 64 MiB or more streams it (pinned pieces, fed session jobs, one per timepoint): the
-    result equals eSRRF() on the same stack already on the device (<= 1e-6: the frame chunks may differ)."""
+    result equals eSRRF() on the same stack already on the device (<= CHUNK_TOL: the frame chunks may differ)."""
     import nanopyx_amd
     from nanopyx_amd import synthetic
     print(f"streamed workflow N{N} {H}x{W} M{M} fpt {fpt} {mode} {dtype} mpc {mpc}", flush=True)
@@ -268,7 +273,7 @@ def test_workflow_streams_host_stack(torch, N, H, W, M, R, S, fpt, mode, dtype, 
                             frames_per_timepoint=fpt, temporal_correlation=mode,
                             macro_pixel_correction=mpc).cpu().numpy()
     assert got.shape == ref.shape and got.dtype == np.float32
-    assert _report(f"streamed workflow {mode} vs device stack", got, ref) <= 1e-6
+    assert _report(f"streamed workflow {mode} vs device stack", got, ref) <= CHUNK_TOL
 
 
 @pytest.mark.parametrize("N,H,W,M,R,S,mode", [
