@@ -286,7 +286,7 @@ int npx_macro_pixel(float *d_img, int S, int HM, int WM, int M, float *d_cmap, v
 
 /*
  * Host-only introspection of the geometry tables the kernels use (no GPU needed). info[16] receives
- * K, margin, HM, WM, nphase, Q (rows per lane) of the per-frame kernel, Q of the temporal kernel, floats per
+ * K, margin, HM, WM, nphase, Q (rows per lane) of the per-frame kernel for this shape, Q of the temporal kernel, floats per
  * phase ring block, HG, HI (tile halos), LP, waves per block, phase groups, prefetch slots, 0, 0; finfo[4] receives
  * fwhm, sigma, tSS, tSO. No reference counterpart: it exists so the host can check launch geometry and tables.
  */

@@ -495,7 +495,8 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     if (rc) return rc;
     const LaunchCfg c = launch_cfg(H, W, M, R);
     Shape s = fused_shape(H, W, M, c);
-    s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
+    const int qf = npx::frames_q(s.nwaves, s.ngroups);
+    s.nregy = (H + qf - 1) / qf;
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
     // frames split into chunks (each block streams one chunk) for about target_blocks blocks; per-frame maps need
     // no merge, so the only cost of a chunk is the block's set-up (tables, first tile)
@@ -795,7 +796,8 @@ static int rgc_impl(const float* d_gx, const float* d_gy, const float* d_img, in
         if (rc == NPX_OK) {
             const LaunchCfg c = launch_cfg(H, W, M, R);
             Shape s = fused_shape(H, W, M, c);
-            s.nregy = (H + npx::kQFrames - 1) / npx::kQFrames;
+            const int qf = npx::frames_q(s.nwaves, s.ngroups);
+            s.nregy = (H + qf - 1) / qf;
             npx::FusedArgs a = base_args(d_img, NPX_DTYPE_F32, N, H, W, M, p->host, t, s, S, iw);
             a.agx = d_gx;
             a.agy = d_gy;
@@ -864,7 +866,8 @@ int npx_plan_info(int H, int W, int M, double R, int* info, float* finfo) {
     const Shape s = fused_shape(H, W, M, launch_cfg(H, W, M, R));
     const int K = h.K;
     const int HG = K / 2 + 3, HI = HG + 1;
-    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::kQFrames, npx::kQTemporal, h.ring_stride, HG, HI,
+    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::frames_q(s.nwaves, s.ngroups), npx::kQTemporal,
+                       h.ring_stride, HG, HI,
                        64 + K, s.nwaves, s.ngroups, npx::kPrefetch, 0, 0};
     for (int i = 0; i < 16; ++i) info[i] = v[i];
     finfo[0] = h.fwhm;

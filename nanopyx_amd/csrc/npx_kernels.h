@@ -14,7 +14,7 @@ constexpr int kMaxRad = 4;           // radii of one sweep job (they share the g
 constexpr int kMaxSens = 4;          // sensitivities of one sweep job (they share the ring sums)
 constexpr int kMaxJobs = 8;          // jobs of one batched launch (passed by value as kernel arguments)
 constexpr int kMaxK = 6;             // largest ring half-width int(2 * radius) the kernels are built for
-constexpr int kQFrames = 16;         // input rows per lane, per-frame-map kernel
+constexpr int kQFrames = 16;         // input rows per lane, per-frame-map kernel of one 16-wave block per region
 constexpr int kQTemporal = 8;        // input rows per lane, temporal and sweep kernels
 constexpr int kPrefetch = 4;         // input-tile elements per thread prefetched for the next frame
 constexpr int kBatchThreads = 1024;  // block size of batched launches (16 wavefronts)
@@ -118,6 +118,12 @@ struct BatchArgs {
     FusedArgs job[kMaxJobs];
 };
 static_assert(sizeof(BatchArgs) <= 4096, "batched kernel arguments must fit the 4 KiB kernel-argument limit");
+
+// Input rows per lane of the per-frame-map and arrays kernels: the tall tile (kQFrames) when one block of 16 waves
+// covers every phase of a region (M = 4: one block per CU either way, and a taller tile spreads the lattice build over
+// more rows); otherwise kQTemporal, whose smaller LDS footprint lets more blocks share a CU (M = 2: 4-wave blocks, 4
+// blocks per CU instead of 2).
+inline int frames_q(int nwaves, int ngroups) { return nwaves == 16 && ngroups == 1 ? kQFrames : kQTemporal; }
 
 // Dynamic LDS bytes of the fused kernels for lattice half-width KL, Q rows per lane, nwaves per block.
 size_t fused_lds_bytes(int KL, int Q, int nwaves, int mode);

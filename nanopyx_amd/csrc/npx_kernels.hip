@@ -249,13 +249,17 @@ __global__ __launch_bounds__(256) void macro_pixel_divide(float* __restrict__ im
 
 template <int K>
 static hipError_t launch_fused_k(const FusedArgs& a, dim3 grid, dim3 block, int mode, hipStream_t st) {
-    const int Q = (mode == kModeFrames || mode == kModeArrays) ? kQFrames : kQTemporal;
+    const int Q = (mode == kModeFrames || mode == kModeArrays) ? frames_q(a.nwaves, a.ngroups) : kQTemporal;
     const size_t dyn = fused_lds_bytes(K, Q, a.nwaves, mode);
     if (dyn > static_cast<size_t>(kLdsLimit)) return hipErrorInvalidConfiguration;
-    if (mode == kModeFrames)
+    if (mode == kModeFrames && Q == kQFrames)
         hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeFrames>), grid, block, dyn, st, a);
-    else if (mode == kModeArrays)
+    else if (mode == kModeFrames)
+        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeFrames>), grid, block, dyn, st, a);
+    else if (mode == kModeArrays && Q == kQFrames)
         hipLaunchKernelGGL((esrrf_fused<K, kQFrames, kModeArrays>), grid, block, dyn, st, a);
+    else if (mode == kModeArrays)
+        hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeArrays>), grid, block, dyn, st, a);
     else if (mode == kModeTemporal)
         hipLaunchKernelGGL((esrrf_fused<K, kQTemporal, kModeTemporal>), grid, block, dyn, st, a);
     else if (mode == kModeTemporalVar)
