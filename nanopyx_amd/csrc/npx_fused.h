@@ -241,8 +241,8 @@ __device__ __forceinline__ void ring_col(float (&acc)[Q], const float2* col, flo
 // taken as clamp(-s u / |dx|, 0, 1), which the compiler folds into the clamp bit of the FMA: it equals [u < 0] for
 // |u| >= |dx| / s, and below that the term it scales is itself below t u^2 / d (u^2 + v^2 = d^2, so
 // d - |v| <= u^2 / d), under 3e-9 t / d at s = 2^16 and |dx| <= 3.5. The ring constants come from the wave's LDS copy,
-// so they sit in VGPRs (on gfx950 a VALU operation with an SGPR operand issues at about half the rate of one whose
-// operands are all VGPRs). The window columns with dx < 0, dx = 0 and dx > 0 are the ranges [0, ineg),
+// so they sit in VGPRs (on gfx950 this four-FMA sample issues about a quarter slower with its constants in SGPRs than
+// with every operand a VGPR). The window columns with dx < 0, dx = 0 and dx > 0 are the ranges [0, ineg),
 // [ineg, inz) and [inz, NW) (dx grows with the column), each with its own loop body.
 template <int K, int KL, int Q>
 __device__ __forceinline__ void ring_sum(float (&acc)[Q], const float2* s_lat, int ey, int ex, int lane,

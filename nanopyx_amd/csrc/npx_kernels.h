@@ -18,7 +18,7 @@ constexpr int kQFrames = 16;         // input rows per lane, per-frame-map kerne
 constexpr int kQTemporal = 8;        // input rows per lane, temporal and sweep kernels
 constexpr int kPrefetch = 4;         // input-tile elements per thread prefetched for the next frame
 constexpr int kBatchThreads = 1024;  // block size of batched launches (16 wavefronts)
-constexpr float kRingScale = 65536.f;  // s of the ring sums' clamped indicator clamp(-s u, 0, 1) (a power of two)
+constexpr float kRingScale = 65536.f;  // s: the lattice holds s G/|G|, the indicator is clamp(-s u / |dx|, 0, 1)
 
 // LDS tile geometry of the fused kernel for lattice half-width K and Q rows per lane.
 template <int K, int Q>
