@@ -189,6 +189,112 @@ def srrf_bench(reps, cpu_frames=2, workload="srrf_256"):
     return res
 
 
+def operators_bench(reps, cpu_seconds=6.0, frames=100, rgc_frames=20):
+    """The two operators of the path measured on their own, on the first frames of the plain_256 scene (256 x 256
+    float32, M5 R1.5 S2, intensity weighting):
+    - eSRRF_ST.run, per-frame maps (npx_esrrf_frames) of `frames` frames;
+    - RadialGradientConvergence.run on caller arrays (npx_rgc) of `rgc_frames` frames: the x2M Catmull-Rom of both
+      Roberts gradients and the xM Catmull-Rom of the image, made on the GPU by the library's separate steps
+      (npx_shift_magnify, npx_roberts) as the reference's eSRRF_ST composes them, then the RGC on those arrays.
+    Kernel time from HIP events on the launch stream over `reps` calls; FP32 roofline on F = U (9 S + 40) as for the
+    fused kernel; the CPU baseline is the reference's own C kernels (oracle/_ref) on a bounded frame sample of the
+    same inputs (the chain for eSRRF_ST, _c_calculate_rgc for the RGC), and parity is taken on that sample."""
+    import torch
+    from nanopyx_amd import _hip, synthetic
+    from oracle import ref32
+    sc = synthetic.CONFIGS["plain_256"]
+    e = sc.esrrf
+    M, R, S, iw = e["magnification"], e["radius"], e["sensitivity"], e["doIntensityWeighting"]
+    x_host = synthetic.generate(sc, frames=frames)
+    x = torch.from_numpy(x_host).cuda()
+    N, H, W = x.shape
+    s_bar = ring_samples_mean(H, W, M, R)
+    ref32.build()
+    kind = "port"
+    if ref32.reference_available():
+        ref32.use_reference_kernels(True)
+        kind = "reference"
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            out = fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, out
+
+    def record(ms, n, in_bytes, what):
+        U = n * H * M * W * M
+        flops = U * (9.0 * s_bar + 40.0)
+        achieved = flops / (ms / 1e3) / 1e12
+        alg = in_bytes + U * 4
+        return {"workload": what, "kernel_ms": round(ms, 4), "value": round(U / 1e6 / (ms / 1e3), 1),
+                "unit": "Mpx·frames/s",
+                "roofline": {"bound": BOUND, "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                             "flops_per_launch": flops, "ring_samples_mean": s_bar, "algorithmic_bytes": alg,
+                             "hbm_gbs_algorithmic": round(alg / (ms / 1e3) / 1e9, 1)}}
+
+    def cpu_record(fn, n_total, what):
+        done, kept, t0 = 0, [], time.perf_counter()
+        while done < n_total:
+            kept.append(fn(done, min(n_total, done + 2)))
+            done = min(n_total, done + 2)
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        return {"value": round(done * H * M * W * M / 1e6 / dt, 3), "unit": "Mpx·frames/s",
+                "cores": ref32.num_threads(), "kind": kind,
+                "sample": f"first {done} of {n_total} frames, {what}, in {dt:.1f} s"}, np.concatenate(kept), done
+
+    def parity(got, ref):
+        g, r = got.astype(np.float64), ref.astype(np.float64)
+        err = np.abs(g - r) / max(np.abs(r).max(), 1e-30)
+        return {"max": float(err.max()), "mean": float(err.mean())}
+
+    try:
+        ms, maps = timed(lambda: _hip.esrrf_frames(x, M, R, S, iw))
+        fr = record(ms, N, N * H * W * 4, f"eSRRF_ST.run per-frame maps: {N} frames x {H}x{W} float32 (plain_256), "
+                                          f"M{M} R{R} S{S} intensity-weighted -> {N} x {H * M}x{W * M}")
+        cpu, ref, n = cpu_record(lambda a, b: ref32.esrrf(x_host[a:b], M, R, S, iw), N, "reference CPU chain")
+        fr["cpu_baseline"] = cpu
+        fr["parity"] = dict(frames=n, against="reference CPU chain (oracle/_ref kernels)",
+                            **parity(maps[:n].cpu().numpy(), ref))
+        del maps
+        # RadialGradientConvergence on caller arrays
+        xs = x[:rgc_frames]
+        img_i = _hip.shift_magnify(xs, 0.0, 0.0, M, M)
+        gc, gr = _hip.roberts(xs)
+        gci = _hip.shift_magnify(gc, 0.0, 0.0, 2 * M, 2 * M)
+        gri = _hip.shift_magnify(gr, 0.0, 0.0, 2 * M, 2 * M)
+        del gc, gr
+        nr = img_i.shape[0]
+        ms, out = timed(lambda: _hip.rgc(gci, gri, img_i, M, R, S, iw, False))
+        # algorithmic input bytes: the lattice positions the ring reads (every M-th row and column of the x2M
+        # arrays, 8 B for the gradient pair) and the xM image
+        lat = nr * (2 * H + 1) * (2 * W + 1) * 8 + nr * H * M * W * M * 4
+        rg = record(ms, nr, lat, f"RadialGradientConvergence.run on caller arrays: {nr} frames, gradients "
+                                 f"{2 * H * M}x{2 * W * M} and image {H * M}x{W * M} float32 (from plain_256), "
+                                 f"M{M} R{R} S{S} intensity-weighted")
+        g_h, r_h, i_h = gci.cpu().numpy(), gri.cpu().numpy(), img_i.cpu().numpy()
+        cpu, ref, n = cpu_record(lambda a, b: ref32.rgc(g_h[a:b], r_h[a:b], i_h[a:b], M, R, S, iw, False), nr,
+                                 "reference _c_calculate_rgc")
+        rg["cpu_baseline"] = cpu
+        rg["parity"] = dict(frames=n, against="reference _c_calculate_rgc (oracle/_ref) on the same arrays",
+                            **parity(out[:n].cpu().numpy(), ref))
+        del out, gci, gri, img_i, g_h, r_h, i_h
+    finally:
+        ref32.use_reference_kernels(False)
+    del x
+    torch.cuda.empty_cache()
+    return {"esrrf_frames": fr, "rgc_arrays": rg}
+
+
 def esrrf3d_samples_mean(Mxy, Mz, radius, radius_z):
     """Mean count of window samples that pass the distance conditions, over the Mz x Mxy x Mxy output phases (interior
     voxels)."""
@@ -582,6 +688,9 @@ def main():
     ap.add_argument("--srrf-only", action="store_true", help="run only the SRRF radiality leg and print its JSON")
     ap.add_argument("--no-esrrf3d", action="store_true")
     ap.add_argument("--no-fht", action="store_true")
+    ap.add_argument("--no-operators", action="store_true", help="skip the eSRRF_ST.run and RGC-on-arrays leg")
+    ap.add_argument("--operators-only", action="store_true",
+                    help="run only the eSRRF_ST.run and RGC-on-arrays leg and print its JSON")
     ap.add_argument("--no-configs", action="store_true", help="skip the cpu_small and plain_256 legs")
     ap.add_argument("--configs-only", action="store_true", help="run only the cpu_small and plain_256 legs")
     ap.add_argument("--serving-only", action="store_true", help="run only the served-stream leg and print its JSON")
@@ -615,6 +724,9 @@ def main():
     if args.srrf_only:
         print(json.dumps(srrf_bench(max(5, args.steps), cpu_frames=0 if args.no_cpu else 2)), flush=True)
         return
+    if args.operators_only:
+        print(json.dumps(operators_bench(max(5, args.steps), cpu_seconds=0.0 if args.no_cpu else 6.0)), flush=True)
+        return
     if args.fht_only:
         print(json.dumps(fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)), flush=True)
         return
@@ -632,7 +744,7 @@ def main():
         print(json.dumps(sweep_bench(max(3, args.steps // 2))), flush=True)
         return
     if args.kernel_only:
-        for k in ("no_configs", "no_sweep", "no_srrf", "no_esrrf3d", "no_fht", "no_serving", "no_cpu"):
+        for k in ("no_configs", "no_sweep", "no_srrf", "no_esrrf3d", "no_fht", "no_serving", "no_cpu", "no_operators"):
             setattr(args, k, True)
 
     import torch
@@ -722,6 +834,7 @@ def main():
         "srrf": None,
         "esrrf3d": None,
         "use_fht": None,
+        "operators": None,
     }
     if not args.no_configs:
         log("cpu_small and plain_256 workloads ...")
@@ -745,6 +858,10 @@ def main():
         log("eSRRF use_fht ...")
         result["use_fht"] = fht_bench(max(3, args.steps // 2), cpu_frames=0 if args.no_cpu else 1)
         log(f"use_fht: {result['use_fht']}")
+    if not args.no_operators:
+        log("eSRRF_ST.run and RGC on arrays ...")
+        result["operators"] = operators_bench(max(5, args.steps), cpu_seconds=0.0 if args.no_cpu else 6.0)
+        log(f"operators: {result['operators']}")
     if not args.no_serving:
         log("served stream ...")
         result["serving"] = serving_bench(args.serving_requests, args.serving_rate)
