@@ -128,3 +128,31 @@ def test_tune_table_host_only():
     assert _hip.tune_get(H, W, M, R)[2] is False
     with pytest.raises(ValueError):
         _hip.tune_set(H, W, M, R, 17, 512)
+
+
+@pytest.mark.parametrize("M,R,q_frames", [(4, 2.0, 16), (2, 1.0, 8), (3, 1.5, 8), (5, 1.5, 8), (8, 1.0, 8)])
+def test_frames_rows_per_lane_by_block_shape(M, R, q_frames):
+    """The per-frame and arrays kernels take 16 rows per lane only when one block of 16 waves covers every phase of a
+    region (M = 4 at the default shape); smaller or several blocks per region take 8 (their smaller LDS tile lets more
+    blocks share a CU). The temporal kernel always takes 8."""
+    d = _hip.plan_info(40, 72, M, R)
+    assert d["Q_frames"] == q_frames and d["Q_temporal"] == 8
+    assert (d["nwaves"] == 16 and d["ngroups"] == 1) == (q_frames == 16)
+
+
+def test_temporal_chunks_keep_eight_frames():
+    """The temporal kernel splits the frames into chunks for about target_blocks blocks, each of at least 8 frames:
+    the workspace (5 partial maps per chunk) gives the chunk count without a GPU."""
+    H, W, M, R = 64, 64, 5, 1.5
+    P = H * M * W * M
+    _hip.tune_set(H, W, M, R, 16, 2048)
+    try:
+        # 1 x 8 regions x 2 phase groups = 16 blocks per chunk, 2048 / 16 = 128 chunks wanted, at most N // 8
+        for N, want in ((100, 12), (16, 2), (7, 0), (1000, 125), (2000, 128)):
+            ws = _hip.temporal_workspace_bytes(N, H, W, M, R)
+            nchunk = ws // (5 * P * 4)
+            assert nchunk == want, (N, nchunk)
+            if nchunk:
+                assert -(-N // nchunk) >= 8 or nchunk == 1
+    finally:
+        _hip.tune_set(H, W, M, R, 0, 0)
