@@ -147,8 +147,9 @@ def test_temporal_chunks_keep_eight_frames():
     P = H * M * W * M
     _hip.tune_set(H, W, M, R, 16, 2048)
     try:
-        # 1 x 8 regions x 2 phase groups = 16 blocks per chunk, 2048 / 16 = 128 chunks wanted, at most N // 8
-        for N, want in ((100, 12), (16, 2), (7, 0), (1000, 125), (2000, 128)):
+        # 1 x 8 regions x 2 phase groups = 16 blocks per chunk, 2048 / 16 = 128 chunks wanted, at most N // 8 (then
+        # rounded to whole frames per chunk)
+        for N, want in ((100, 12), (16, 2), (7, 0), (1000, 125), (2000, 125)):  # 2000: 16 frames each
             ws = _hip.temporal_workspace_bytes(N, H, W, M, R)
             nchunk = ws // (5 * P * 4)
             assert nchunk == want, (N, nchunk)
