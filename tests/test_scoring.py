@@ -1,5 +1,6 @@
 """Sweep scoring on the host (nanopyx_amd.core.analysis / core.transform): Fourier interpolation, Pearson, error
-map, FIRE number and QnR, pinned by closed forms, invariants and scalar restatements of the reference's loops."""
+map, FIRE number and QnR, pinned by closed forms, invariants and scalar restatements of the reference's loops, and
+FIRE and decorrelation against the reference's own compiled analyses (tests/golden/scoring_golden.npz)."""
 import math
 
 import numpy as np
@@ -248,3 +249,52 @@ def test_decorr_pieces():
     assert np.allclose(d._get_d_corr_max(rising, 0, 1), [0, 0])
     bump_then_rise = np.array([0, .4, .7, .5, .45, .5, .6, .7, .8, .9, 1.0], np.float32)
     assert np.allclose(d._get_d_corr_max(bump_then_rise, 0.2, 0.7), [0.2 + 0.5 * 2 / 10, 0.7])
+
+
+def _scoring_golden():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scoring_golden.npz")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("n", [64, 96])
+def test_fire_number_matches_reference_vectors(n):
+    """FIRE numbers of two noisy realisations of one emitter field, against the reference's own FIRECalculator
+    (core/analysis/frc.pyx:17-294, compiled from the reference checkout by tests/golden/make_scoring_golden.py)."""
+    g = _scoring_golden()
+    got = FIRECalculator(pixel_size=1, units="pixel").calculate_fire_number(g[f"pair{n}_a"], g[f"pair{n}_b"])
+    ref = float(g[f"fire{n}"])
+    assert abs(got - ref) <= 1e-5 * abs(ref), (got, ref)
+
+
+@pytest.mark.parametrize("name,kw", [("pattern80", {}), ("blur72", {}), ("pair64_a", {}), ("pair96_a", {}),
+                                     ("pair96_b", dict(rmin=0.05, rmax=0.8, n_r=40, n_g=8, pixel_size=2.5,
+                                                       units="nm"))])
+def test_decorr_resolution_matches_reference_vectors(name, kw):
+    """Decorrelation resolutions of a band-limited pattern, a blurred noise field and emitter images (default and
+    non-default analysis parameters), against the reference's own DecorrAnalysis (core/analysis/decorr.pyx:22-413,
+    compiled from the reference checkout by tests/golden/make_scoring_golden.py)."""
+    from nanopyx_amd.core.analysis.decorr import DecorrAnalysis
+    g = _scoring_golden()
+    d = DecorrAnalysis(**kw)
+    d.run_analysis(g[name])
+    ref = float(g[f"decorr_{name}" + ("_params" if kw else "")])
+    assert abs(d.resolution - ref) <= 1e-5 * abs(ref), (d.resolution, ref)
+
+
+def test_scoring_golden_sources_recorded():
+    """The fixture names the reference files it was built from (paths and SHA-256)."""
+    s = str(_scoring_golden()["sources_sha256"])
+    assert "src/nanopyx/core/analysis/frc.pyx" in s and "src/nanopyx/core/analysis/decorr.pyx" in s
+
+
+@pytest.mark.parametrize("a,b,key", [("pair64_a", "pair64_b", "pearson_pair64"),
+                                     ("pattern80", "blur72", "pearson_pattern_blur")])
+def test_pearson_matches_reference_vectors(a, b, key):
+    """Pearson correlation (the error map's RSP) against the reference's own pearson_correlation
+    (core/analysis/pearson_correlation.pyx:36-60, float32 sums): within 1e-5 (the reference accumulates in float32)."""
+    g = _scoring_golden()
+    x, y = g[a], g[b]
+    n = min(x.shape[0], y.shape[0])
+    got = pearson_correlation(x[:n, :n], y[:n, :n])
+    assert abs(got - float(g[key])) <= 1e-5, (got, float(g[key]))
