@@ -1,8 +1,8 @@
-"""Writes tests/golden/scoring_golden.npz: the reference's own FRC/FIRE, decorrelation and Pearson correlation on
-fixed images.
+"""Writes tests/golden/cython_golden.npz: outputs of the reference's own Cython functions that no other fixture pins —
+the sweep scoring's FRC/FIRE, decorrelation and Pearson correlation, and the macro-pixel corrector — on fixed inputs.
 
-The reference's Cython modules core/analysis/frc.pyx, core/analysis/decorr.pyx and core/analysis/pearson_correlation.pyx
-(the error map's RSP), and the modules they import or
+The reference's Cython modules core/analysis/frc.pyx, core/analysis/decorr.pyx, core/analysis/pearson_correlation.pyx
+(the error map's RSP) and core/transform/mpcorrector.pyx, and the modules they import or
 cimport (core/analysis/ccm_helper_functions, core/utils/math, core/utils/array, core/utils/timeit,
 core/generate/mask, core/transform/padding, core/transform/edges, core/transform/normalize), are compiled from where
 they lie under the reference checkout into a temporary directory outside this repository, with the reference's Linux
@@ -12,7 +12,7 @@ __init__ file: Python imports the folders as namespace packages, so none of the 
 script imports the two analyses from there and records inputs and outputs; nothing of the reference is copied into
 the repository. Run it in the build container (it needs the reference checkout, Cython, gcc):
 
-    python tests/golden/make_scoring_golden.py [REFERENCE_ROOT]
+    python tests/golden/make_cython_golden.py [REFERENCE_ROOT]
 """
 from __future__ import annotations
 
@@ -28,7 +28,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODULES = [  # (package folder under nanopyx/core, module, compiled)
     ("analysis", "frc", True), ("analysis", "decorr", True), ("analysis", "ccm_helper_functions", True),
-    ("analysis", "pearson_correlation", True),
+    ("analysis", "pearson_correlation", True), ("transform", "mpcorrector", True),
     ("utils", "math", True), ("utils", "array", True), ("utils", "timeit", False),
     ("generate", "mask", True), ("transform", "padding", False), ("transform", "edges", True),
     ("transform", "normalize", True),
@@ -96,6 +96,7 @@ def main():
         from nanopyx.core.analysis.frc import FIRECalculator
         from nanopyx.core.analysis.decorr import DecorrAnalysis
         from nanopyx.core.analysis.pearson_correlation import pearson_correlation
+        from nanopyx.core.transform.mpcorrector import macro_pixel_corrector
         imgs = images()
         rec = dict(imgs)
         for n in (64, 96):
@@ -112,8 +113,20 @@ def main():
         rec["decorr_pair96_b_params"] = np.float64(d.resolution)
         rec["pearson_pair64"] = np.float64(pearson_correlation(imgs["pair64_a"], imgs["pair64_b"]))
         rec["pearson_pattern_blur"] = np.float64(pearson_correlation(imgs["pattern80"][:72, :72], imgs["blur72"]))
+        # macro-pixel corrector on eSRRF-like maps: a positive phase pattern times a smooth field (3 slices at M5),
+        # one slice at M4 with zero phase entries (the division skips them), and one slice at M2
+        rng = np.random.default_rng(7)
+        yy, xx = np.mgrid[0:40, 0:35]
+        field = 1.0 + 0.5 * np.sin(xx / 6.0) * np.cos(yy / 5.0)
+        ph = 1.0 + 0.3 * rng.random((5, 5))
+        mpc5 = np.stack([field * np.tile(ph, (8, 7)) * (s + 1) + rng.random((40, 35)) * 0.01 for s in range(3)])
+        mpc4 = (rng.random((1, 24, 32)) * np.tile(rng.random((4, 4)) > 0.2, (6, 8))).astype(np.float32)
+        mpc2 = (rng.random((1, 18, 22)) + 0.5).astype(np.float32)
+        for name, img, M in (("mpc5", mpc5, 5), ("mpc4", mpc4, 4), ("mpc2", mpc2, 2)):
+            rec[f"{name}_in"] = img.astype(np.float32)
+            rec[f"{name}_out"] = np.asarray(macro_pixel_corrector(img.astype(np.float32), M), np.float32)
         rec["sources_sha256"] = np.array("\n".join(digests))
-        path = os.path.join(HERE, "scoring_golden.npz")
+        path = os.path.join(HERE, "cython_golden.npz")
         np.savez_compressed(path, **rec)
         print(f"wrote {path}: " + ", ".join(f"{k}={float(v):.6g}" for k, v in rec.items() if np.ndim(v) == 0
                                               and k != "sources_sha256"))

@@ -179,6 +179,17 @@ def test_temporal_and_macro_pixel_kernels(torch, oracle_built):
                    oracle_built.macro_pixel(img, 5)) <= TOL
 
 
+@pytest.mark.parametrize("name,M", [("mpc5", 5), ("mpc4", 4), ("mpc2", 2)])
+def test_macro_pixel_kernel_vs_reference_vectors(torch, name, M):
+    """The GPU macro-pixel corrector on the inputs of the reference's own compiled macro_pixel_corrector
+    (tests/golden/cython_golden.npz): within 1e-6 (its phase means sum in another order)."""
+    import os
+    from nanopyx_amd import macro_pixel_corrector
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cython_golden.npz"))
+    got = np.asarray(macro_pixel_corrector(g[f"{name}_in"], M)).reshape(g[f"{name}_out"].shape)
+    assert _report(f"macro pixel {name} vs reference vectors", got, g[f"{name}_out"]) <= 1e-6
+
+
 def test_workflow_timepoints(torch, oracle_built):
     import nanopyx_amd
     rng = np.random.default_rng(6)

@@ -170,6 +170,17 @@ def test_macro_pixel_closed_form(oracle_built):
     assert np.all(fp64.macro_pixel(img2, M)[0, ::M, ::M] == 0)
 
 
+@pytest.mark.parametrize("name,M", [("mpc5", 5), ("mpc4", 4), ("mpc2", 2)])
+def test_macro_pixel_matches_reference_vectors(oracle_built, name, M):
+    """The restatement of macro_pixel_corrector against the reference's own compiled function
+    (core/transform/mpcorrector.pyx:7-73, built by tests/golden/make_cython_golden.py): bit for bit, including phases
+    whose mean is 0 (left undivided)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cython_golden.npz"))
+    got = oracle_built.macro_pixel(g[f"{name}_in"], M)
+    assert np.array_equal(got.reshape(g[f"{name}_out"].shape), g[f"{name}_out"])
+
+
 def test_temporal_unknown_mode_raises():
     with pytest.raises(ValueError):
         fp64.temporal(np.zeros((2, 2, 2)), "XYZ")

@@ -1,6 +1,6 @@
 """Sweep scoring on the host (nanopyx_amd.core.analysis / core.transform): Fourier interpolation, Pearson, error
 map, FIRE number and QnR, pinned by closed forms, invariants and scalar restatements of the reference's loops, and
-FIRE and decorrelation against the reference's own compiled analyses (tests/golden/scoring_golden.npz)."""
+FIRE, decorrelation and Pearson against the reference's own compiled analyses (tests/golden/cython_golden.npz)."""
 import math
 
 import numpy as np
@@ -253,14 +253,14 @@ def test_decorr_pieces():
 
 def _scoring_golden():
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scoring_golden.npz")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cython_golden.npz")
     return np.load(path)
 
 
 @pytest.mark.parametrize("n", [64, 96])
 def test_fire_number_matches_reference_vectors(n):
     """FIRE numbers of two noisy realisations of one emitter field, against the reference's own FIRECalculator
-    (core/analysis/frc.pyx:17-294, compiled from the reference checkout by tests/golden/make_scoring_golden.py)."""
+    (core/analysis/frc.pyx:17-294, compiled from the reference checkout by tests/golden/make_cython_golden.py)."""
     g = _scoring_golden()
     got = FIRECalculator(pixel_size=1, units="pixel").calculate_fire_number(g[f"pair{n}_a"], g[f"pair{n}_b"])
     ref = float(g[f"fire{n}"])
@@ -273,7 +273,7 @@ def test_fire_number_matches_reference_vectors(n):
 def test_decorr_resolution_matches_reference_vectors(name, kw):
     """Decorrelation resolutions of a band-limited pattern, a blurred noise field and emitter images (default and
     non-default analysis parameters), against the reference's own DecorrAnalysis (core/analysis/decorr.pyx:22-413,
-    compiled from the reference checkout by tests/golden/make_scoring_golden.py)."""
+    compiled from the reference checkout by tests/golden/make_cython_golden.py)."""
     from nanopyx_amd.core.analysis.decorr import DecorrAnalysis
     g = _scoring_golden()
     d = DecorrAnalysis(**kw)
