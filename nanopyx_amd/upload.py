@@ -71,36 +71,46 @@ def esrrf_temporal_streamed(x: np.ndarray, magnification, radius, sensitivity, i
             _state["session"] = _hip.Session(need + (16 << 20), 64)
             _state["pool_bytes"] = need
         sess = _state["session"]
-        jobs = [sess.submit_fed(d[a:b], M, radius, sensitivity, iw, (mode,), {mode: out[t]}, pf)
-                for t, (a, b) in enumerate(spans)]
-        fed = [0] * T
         cs = res["stream"]
-        cs.wait_stream(torch.cuda.current_stream())  # the device stack's memory is free of earlier work
-        for p, a in enumerate(range(0, N, pf)):
-            b = min(N, a + pf)
-            k = p % 2
-            if res["events"][k] is not None:
-                res["events"][k].synchronize()  # the buffer's previous transfer has been read
-            staged = res["bufs"][k][: (b - a) * fb]
-            _host_copy(res["pool"], staged.numpy().view(x.dtype).reshape(b - a, H, W), x[a:b])
-            with torch.cuda.stream(cs):
-                d8[a:b].copy_(staged.view(b - a, fb), non_blocking=True)
-                ev = torch.cuda.Event()
-                ev.record(cs)
-                res["events"][k] = ev
-            for t, (ta, tb) in enumerate(spans):
-                have = min(b, tb) - ta
-                if have > fed[t]:
-                    sess.feed(jobs[t], have, cs)
-                    fed[t] = have
-            sess.step()
-        for _ in range(10_000_000):
-            sess.step()
-            if all(sess.poll(j) == _hip.Session.DONE for j in jobs):
-                break
-        else:
-            raise _hip.NpxError("streamed eSRRF jobs did not finish")
-        for j in jobs:
-            sess.forget(j)
+        try:
+            jobs = [sess.submit_fed(d[a:b], M, radius, sensitivity, iw, (mode,), {mode: out[t]}, pf)
+                    for t, (a, b) in enumerate(spans)]
+            fed = [0] * T
+            cs.wait_stream(torch.cuda.current_stream())  # the device stack's memory is free of earlier work
+            for p, a in enumerate(range(0, N, pf)):
+                b = min(N, a + pf)
+                k = p % 2
+                if res["events"][k] is not None:
+                    res["events"][k].synchronize()  # the buffer's previous transfer has been read
+                staged = res["bufs"][k][: (b - a) * fb]
+                _host_copy(res["pool"], staged.numpy().view(x.dtype).reshape(b - a, H, W), x[a:b])
+                with torch.cuda.stream(cs):
+                    d8[a:b].copy_(staged.view(b - a, fb), non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record(cs)
+                    res["events"][k] = ev
+                for t, (ta, tb) in enumerate(spans):
+                    have = min(b, tb) - ta
+                    if have > fed[t]:
+                        sess.feed(jobs[t], have, cs)
+                        fed[t] = have
+                sess.step()
+            for _ in range(10_000_000):
+                sess.step()
+                if all(sess.poll(j) == _hip.Session.DONE for j in jobs):
+                    break
+            else:
+                raise _hip.NpxError("streamed eSRRF jobs did not finish")
+            for j in jobs:
+                sess.forget(j)
+        except BaseException:
+            # a failed call leaves no job in the shared session: the session is destroyed (which waits for the
+            # device, so no slice still reads this call's stack) and the next call makes a new one
+            _state.pop("pool_bytes", None)
+            try:
+                _state.pop("session").close()
+            finally:
+                cs.synchronize()
+            raise
         torch.cuda.current_stream().wait_stream(cs)
     return out

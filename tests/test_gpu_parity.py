@@ -155,6 +155,21 @@ def test_large_u16_vs_reference_subset(torch, oracle_built):
                        oracle_built.temporal(maps, m)) <= TOL
 
 
+@pytest.mark.parametrize("M,R", [(3, 1.5), (4, 2.0)])
+def test_single_frame_temporal(torch, oracle_built, M, R):
+    """One frame: AVG is the frame's map, VAR is 0 and TAC2 is NaN everywhere, as the reference's NumPy reduction
+    gives (sr_temporal_correlations.py:64-70: the variance of one value; :205-221: the mean of an empty product)."""
+    from nanopyx_amd import _hip
+    rng = np.random.default_rng(11)
+    xh = (rng.random((1, 21, 37)) * 200 + 50).astype(np.float32)
+    ref_map = oracle_built.esrrf(xh, M, R, 2.0)
+    outs = _hip.esrrf_temporal(torch.from_numpy(xh).cuda(), M, R, 2.0, True, ("AVG", "VAR", "TAC2"))
+    avg, var, tac2 = (outs[m].cpu().numpy() for m in ("AVG", "VAR", "TAC2"))
+    assert _report("single frame AVG vs reference map", avg, ref_map[0]) <= TOL
+    assert (var == 0).all() and (oracle_built.temporal(ref_map, "VAR") == 0).all()
+    assert np.isnan(tac2).all() and np.isnan(oracle_built.temporal(ref_map, "TAC2")).all()
+
+
 def test_uint16_equals_float32(torch):
     from nanopyx_amd import _hip
     rng = np.random.default_rng(4)
