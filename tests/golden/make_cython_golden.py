@@ -9,8 +9,8 @@ they lie under the reference checkout into a temporary directory outside this re
 wheel flags (-O3 -fopenmp). The directory holds only those modules, in their package folders and without any
 __init__ file: Python imports the folders as namespace packages, so none of the reference's package initialisation
 (its installed-metadata lookup, the Liquid Engine) runs and no stand-in for any reference module is written. The
-script imports the two analyses from there and records inputs and outputs; nothing of the reference is copied into
-the repository. Run it in the build container (it needs the reference checkout, Cython, gcc):
+script imports the analyses and the corrector from there and records inputs and outputs; nothing of the reference
+is copied into the repository. Run it in the build container (it needs the reference checkout, Cython, gcc):
 
     python tests/golden/make_cython_golden.py [REFERENCE_ROOT]
 """
@@ -47,7 +47,9 @@ def build(ref_root: str, tmp: str) -> list[str]:
             src = os.path.join(core, folder, mod + ext)
             if os.path.exists(src):
                 os.symlink(src, os.path.join(dst, mod + ext))
-                digests.append(f"{hashlib.sha256(open(src, 'rb').read()).hexdigest()}  src/nanopyx/core/{folder}/{mod}{ext}")
+                with open(src, "rb") as fh:
+                    digest = hashlib.sha256(fh.read()).hexdigest()
+                digests.append(f"{digest}  src/nanopyx/core/{folder}/{mod}{ext}")
         if compiled:
             exts.append(f"nanopyx/core/{folder}/{mod}.pyx")
     setup = os.path.join(tmp, "setup_golden.py")
