@@ -216,14 +216,15 @@ Shape fused_shape(int H, int W, int M, const LaunchCfg& c) {
     return s;
 }
 
-// Frame chunking of the temporal kernel: about target_blocks blocks, while each chunk keeps >= 8 frames. A chunk adds
+// Frame chunking of the temporal kernel: about target_blocks blocks, while each chunk keeps >= 6 frames. A chunk adds
 // 5 partial moments per output pixel (20 B written and read once by the merge) against the ring work of its frames
 // (~700 flops per output pixel and frame), so the merge stays a small fraction of the work; images large enough to
-// fill the chip take one chunk. (100 x 64^2 at M5: 3 -> 8 chunks, 0.35 -> 0.16 ms.)
+// fill the chip take one chunk. (100 x 64^2 at M5, 16 blocks per chunk: chunks of >= 32 frames 0.37 ms, >= 8 frames
+// 12 chunks 0.139 ms, >= 6 frames 15 chunks, 240 blocks for the 256 CUs, 0.119 ms.)
 void temporal_chunks(int N, const Shape& s, int target_blocks, int* nchunk, int* fpc) {
     const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
     long long want = (target_blocks + spatial - 1) / spatial;
-    want = std::min<long long>(want, std::max(1, N / 8));
+    want = std::min<long long>(want, std::max(1, N / 6));
     want = std::max<long long>(want, 1);
     int f = static_cast<int>((N + want - 1) / want);
     *fpc = std::max(f, 1);
