@@ -321,6 +321,18 @@ int npx_autotune(const void *d_in, int dtype, int N, int H, int W, int M, double
                  void *stream, double *best_ms);
 int npx_tune_set(int H, int W, int M, double R, int max_waves_per_block, int target_blocks);
 int npx_tune_get(int H, int W, int M, double R, int *cfg);
+/*
+ * The same for the per-frame-map kernels (npx_esrrf_frames, and npx_rgc / npx_rgc_lattice on an H x W image magnified
+ * M times), whose best shape can differ from the temporal kernel's: npx_autotune_frames times every candidate with
+ * npx_esrrf_frames on the caller's stack into d_out ((N, H*M, W*M) float32; blocking) and keeps the fastest for
+ * (H, W, M, R); npx_tune_frames_set / npx_tune_frames_get store and read such a shape (get returns 1 when the
+ * per-frame kernels have their own, 0 when they use the temporal kernel's shape, which they do until tuned). The
+ * maps do not depend on the shape. No reference counterpart (as npx_autotune).
+ */
+int npx_autotune_frames(const void *d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                        float *d_out, void *stream, double *best_ms);
+int npx_tune_frames_set(int H, int W, int M, double R, int max_waves_per_block, int target_blocks);
+int npx_tune_frames_get(int H, int W, int M, double R, int *cfg);
 
 /* Turns per-call HIP-event timing of the eSRRF kernels on (1) or off (0). No reference counterpart: it lets a
  * caller (the benchmark) read the fused kernel's own duration on the stream it was launched on. */

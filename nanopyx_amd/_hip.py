@@ -110,6 +110,10 @@ def lib():
                                    ctypes.POINTER(_c_double)]
         L.npx_tune_set.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _c_int]
         L.npx_tune_get.argtypes = [_c_int, _c_int, _c_int, _c_double, _ip]
+        L.npx_autotune_frames.argtypes = [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_double,
+                                          _c_int, _c_void_p, _c_void_p, ctypes.POINTER(_c_double)]
+        L.npx_tune_frames_set.argtypes = L.npx_tune_set.argtypes
+        L.npx_tune_frames_get.argtypes = L.npx_tune_get.argtypes
         L.npx_radiality.argtypes = [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int,
                                     _c_int, _c_int, _c_void_p, _c_void_p]
         L.npx_radiality_info.argtypes = [_c_int, _c_int, _c_int, _c_double, _c_int, _ip, _fp]
@@ -802,6 +806,37 @@ def autotune(stack, magnification, radius, sensitivity, iw, modes=("AVG",), stre
                               ctypes.byref(best)))
     mw, tb, _ = tune_get(H, W, M, radius)
     return mw, tb, best.value
+
+
+def autotune_frames(stack, magnification, radius, sensitivity, iw, stream=None):
+    """Times the candidate launch shapes of the per-frame-map kernel on this stack (npx_autotune_frames) and keeps
+    the fastest for its (H, W, M, R). Returns (max_waves_per_block, target_blocks, ms per call)."""
+    torch = _torch()
+    x = _dev_stack(stack)
+    N, H, W = x.shape
+    M = int(magnification)
+    out = torch.empty((N, H * M, W * M), dtype=torch.float32, device=x.device)
+    best = ctypes.c_double(0)
+    _check(lib().npx_autotune_frames(x.data_ptr(), _dtype_code(x), N, H, W, M, float(radius), float(sensitivity),
+                                     int(bool(iw)), out.data_ptr(), _on(torch, stream, x, out), ctypes.byref(best)))
+    mw, tb, _ = tune_frames_get(H, W, M, radius)
+    return mw, tb, best.value
+
+
+def tune_frames_set(H, W, M, R, max_waves_per_block: int, target_blocks: int) -> None:
+    """Stores a launch shape of the per-frame-map kernels for (H, W, M, R) (0, 0 forgets it)."""
+    _check(lib().npx_tune_frames_set(int(H), int(W), int(M), float(R), int(max_waves_per_block),
+                                     int(target_blocks)))
+
+
+def tune_frames_get(H, W, M, R):
+    """(max_waves_per_block, target_blocks, own?) that per-frame-map calls with these parameters use; own is False
+    when they use the temporal kernel's shape."""
+    cfg = (ctypes.c_int * 2)()
+    rc = lib().npx_tune_frames_get(int(H), int(W), int(M), float(R), cfg)
+    if rc < 0:
+        _check(rc)
+    return cfg[0], cfg[1], bool(rc)
 
 
 def tune_set(H, W, M, R, max_waves_per_block: int, target_blocks: int) -> None:

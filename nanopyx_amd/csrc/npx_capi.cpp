@@ -182,11 +182,23 @@ struct LaunchCfg {
 std::mutex g_cfg_mu;
 LaunchCfg g_default_cfg{16, 2048};
 std::map<Key, LaunchCfg> g_tuned;
+std::map<Key, LaunchCfg> g_tuned_frames;  // shapes of the per-frame-map kernels, where tuned apart from the temporal
 
 LaunchCfg launch_cfg(int H, int W, int M, double R) {
     std::lock_guard<std::mutex> lk(g_cfg_mu);
     auto it = g_tuned.find(Key{H, W, M, bits(R)});
     return it != g_tuned.end() ? it->second : g_default_cfg;
+}
+
+// Launch shape of the per-frame-map kernels (npx_esrrf_frames, npx_rgc's fused path): their own tuned shape, else the
+// temporal kernel's (tuned or default).
+LaunchCfg launch_cfg_frames(int H, int W, int M, double R) {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    const Key k{H, W, M, bits(R)};
+    auto it = g_tuned_frames.find(k);
+    if (it != g_tuned_frames.end()) return it->second;
+    auto jt = g_tuned.find(k);
+    return jt != g_tuned.end() ? jt->second : g_default_cfg;
 }
 
 // Compute units of the current device (256 on MI355X; also when no device answers, so host-side layout queries
@@ -229,6 +241,24 @@ void temporal_chunks(int N, const Shape& s, int target_blocks, int* nchunk, int*
     int f = static_cast<int>((N + want - 1) / want);
     *fpc = std::max(f, 1);
     *nchunk = (N + *fpc - 1) / *fpc;
+}
+
+// Frame chunking of the per-frame-map kernel: about target_blocks blocks (per-frame maps need no merge, so the only
+// cost of a chunk is the block's set-up: tables, first tile).
+void frames_chunks(int N, const Shape& s, int target_blocks, int* nchunk, int* fpc) {
+    const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
+    long long want = std::max<long long>(1, (target_blocks + spatial - 1) / spatial);
+    want = std::min<long long>({want, N, 65535 / s.ngroups});
+    *fpc = static_cast<int>((N + want - 1) / want);
+    *nchunk = (N + *fpc - 1) / *fpc;
+}
+
+// Shape of the per-frame-map kernel: fused_shape with its rows per lane (frames_q).
+Shape frames_shape(int H, int W, int M, const LaunchCfg& c) {
+    Shape s = fused_shape(H, W, M, c);
+    const int qf = npx::frames_q(s.nwaves, s.ngroups);
+    s.nregy = (H + qf - 1) / qf;
+    return s;
 }
 
 int pow_mode_of(double S) {
@@ -496,18 +526,11 @@ int npx_esrrf_frames(const void* d_in, int dtype, int N, int H, int W, int M, do
     Plan* p = nullptr;
     rc = plan_and_tables(H, W, M, R, &p, &t, static_cast<hipStream_t>(stream));
     if (rc) return rc;
-    const LaunchCfg c = launch_cfg(H, W, M, R);
-    Shape s = fused_shape(H, W, M, c);
-    const int qf = npx::frames_q(s.nwaves, s.ngroups);
-    s.nregy = (H + qf - 1) / qf;
+    const LaunchCfg c = launch_cfg_frames(H, W, M, R);
+    const Shape s = frames_shape(H, W, M, c);
     npx::FusedArgs a = base_args(d_in, dtype, N, H, W, M, p->host, t, s, S, iw);
-    // frames split into chunks (each block streams one chunk) for about target_blocks blocks; per-frame maps need
-    // no merge, so the only cost of a chunk is the block's set-up (tables, first tile)
-    const long long spatial = static_cast<long long>(s.nregx) * s.nregy * s.ngroups;
-    long long want = std::max<long long>(1, (c.target_blocks + spatial - 1) / spatial);
-    want = std::min<long long>({want, N, 65535 / s.ngroups});
-    a.fpc = static_cast<int>((N + want - 1) / want);
-    a.nchunk = (N + a.fpc - 1) / a.fpc;
+    // frames split into chunks, each block streaming one chunk
+    frames_chunks(N, s, c.target_blocks, &a.nchunk, &a.fpc);
     a.out = d_out;
     dim3 grid(s.nregx, s.nregy, s.ngroups * a.nchunk);
     dim3 block(64 * s.nwaves);
@@ -797,10 +820,8 @@ static int rgc_impl(const float* d_gx, const float* d_gy, const float* d_img, in
         const std::string prev = g_err;
         const int rc = plan_and_tables(H, W, M, R, &p, &t, st);
         if (rc == NPX_OK) {
-            const LaunchCfg c = launch_cfg(H, W, M, R);
-            Shape s = fused_shape(H, W, M, c);
-            const int qf = npx::frames_q(s.nwaves, s.ngroups);
-            s.nregy = (H + qf - 1) / qf;
+            const LaunchCfg c = launch_cfg_frames(H, W, M, R);
+            const Shape s = frames_shape(H, W, M, c);
             npx::FusedArgs a = base_args(d_img, NPX_DTYPE_F32, N, H, W, M, p->host, t, s, S, iw);
             a.agx = d_gx;
             a.agy = d_gy;
@@ -867,9 +888,10 @@ int npx_plan_info(int H, int W, int M, double R, int* info, float* finfo) {
     if (!p) return NPX_ERR_UNSUPPORTED;
     const npx::PlanHost& h = p->host;
     const Shape s = fused_shape(H, W, M, launch_cfg(H, W, M, R));
+    const Shape sf = fused_shape(H, W, M, launch_cfg_frames(H, W, M, R));
     const int K = h.K;
     const int HG = K / 2 + 3, HI = HG + 1;
-    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::frames_q(s.nwaves, s.ngroups), npx::kQTemporal,
+    const int v[16] = {K, h.margin, h.HM, h.WM, h.nphase, npx::frames_q(sf.nwaves, sf.ngroups), npx::kQTemporal,
                        h.ring_stride, HG, HI,
                        64 + K, s.nwaves, s.ngroups, npx::kPrefetch, 0, 0};
     for (int i = 0; i < 16; ++i) info[i] = v[i];
@@ -929,6 +951,29 @@ int npx_tune_get(int H, int W, int M, double R, int* cfg) {
     cfg[0] = c.max_waves;
     cfg[1] = c.target_blocks;
     return it != g_tuned.end() ? 1 : 0;
+}
+
+int npx_tune_frames_set(int H, int W, int M, double R, int max_waves_per_block, int target_blocks) {
+    if (max_waves_per_block < 0 || max_waves_per_block > 16 || target_blocks < 0)
+        return fail(NPX_ERR_ARG, "max_waves_per_block must be in [1, 16], target_blocks >= 1 (0, 0 = forget)");
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    const Key k{H, W, M, bits(R)};
+    if (max_waves_per_block == 0 || target_blocks == 0) g_tuned_frames.erase(k);
+    else g_tuned_frames[k] = LaunchCfg{max_waves_per_block, target_blocks};
+    return NPX_OK;
+}
+
+int npx_tune_frames_get(int H, int W, int M, double R, int* cfg) {
+    if (!cfg) return fail(NPX_ERR_ARG, "NULL output");
+    bool own = false;
+    {
+        std::lock_guard<std::mutex> lk(g_cfg_mu);
+        own = g_tuned_frames.count(Key{H, W, M, bits(R)}) != 0;
+    }
+    const LaunchCfg c = launch_cfg_frames(H, W, M, R);
+    cfg[0] = c.max_waves;
+    cfg[1] = c.target_blocks;
+    return own ? 1 : 0;
 }
 
 // Candidate launch shapes of the autotuner.
@@ -1012,6 +1057,70 @@ int npx_autotune(const void* d_in, int dtype, int N, int H, int W, int M, double
         return rc != NPX_OK ? rc : fail(NPX_ERR_ARG, "no launch shape fits the workspace");
     }
     g_tuned[key] = best;
+    if (best_ms) *best_ms = best_t;
+    return NPX_OK;
+}
+
+int npx_autotune_frames(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
+                        float* d_out, void* stream, double* best_ms) {
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune_frames cannot run under stream capture");
+    int rc = check_common(d_in, dtype, N, H, W, M);
+    if (rc) return rc;
+    if (!d_out) return fail(NPX_ERR_ARG, "output pointer is NULL");
+    const Key key{H, W, M, bits(R)};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate(autotune)");
+    }
+    LaunchCfg best{0, 0};
+    float best_t = 0.f;
+    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks) already timed
+    for (int mw : kTuneWaves) {
+        for (int tb : kTuneBlocks) {
+            const LaunchCfg c{mw, tb};
+            const Shape s = frames_shape(H, W, M, c);
+            int nchunk = 1, fpc = N;
+            frames_chunks(N, s, tb, &nchunk, &fpc);
+            const std::pair<int, int> id{s.nwaves, nchunk};
+            if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
+            seen.push_back(id);
+            {
+                std::lock_guard<std::mutex> lk(g_cfg_mu);
+                g_tuned_frames[key] = c;
+            }
+            float t = 0.f;
+            for (int rep = 0; rep < 3 && rc == NPX_OK; ++rep) {  // one warm-up, then the best of two
+                e = hipEventRecord(e0, st);
+                if (e != hipSuccess) { rc = hip_fail(e, "hipEventRecord(autotune)"); break; }
+                rc = npx_esrrf_frames(d_in, dtype, N, H, W, M, R, S, iw, d_out, stream);
+                if (rc) break;
+                e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) { rc = hip_fail(e, "autotune timing"); break; }
+                if (rep == 1 || (rep == 2 && ms < t)) t = ms;
+            }
+            if (rc) break;
+            if (best.max_waves == 0 || t < best_t) {
+                best = c;
+                best_t = t;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    if (rc != NPX_OK || best.max_waves == 0) {
+        g_tuned_frames.erase(key);
+        return rc != NPX_OK ? rc : fail(NPX_ERR_ARG, "no launch shape was timed");
+    }
+    g_tuned_frames[key] = best;
     if (best_ms) *best_ms = best_t;
     return NPX_OK;
 }

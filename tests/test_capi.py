@@ -140,6 +140,26 @@ def test_frames_rows_per_lane_by_block_shape(M, R, q_frames):
     assert (d["nwaves"] == 16 and d["ngroups"] == 1) == (q_frames == 16)
 
 
+def test_frames_launch_shape_apart_from_temporal():
+    """The per-frame kernels take their own stored launch shape where there is one, else the temporal kernel's; the
+    plan's per-frame rows per lane follow it."""
+    H, W, M, R = 40, 72, 4, 2.0
+    try:
+        assert _hip.tune_frames_get(H, W, M, R)[2] is False
+        _hip.tune_set(H, W, M, R, 8, 512)
+        assert _hip.tune_frames_get(H, W, M, R) == (8, 512, False)  # the temporal kernel's shape
+        assert _hip.plan_info(H, W, M, R)["Q_frames"] == 8          # 2 blocks of 8 waves per region
+        _hip.tune_frames_set(H, W, M, R, 16, 1024)
+        assert _hip.tune_frames_get(H, W, M, R) == (16, 1024, True)
+        assert _hip.tune_get(H, W, M, R)[:2] == (8, 512)
+        assert _hip.plan_info(H, W, M, R)["Q_frames"] == 16         # one block of 16 waves covers the 16 phases
+        with pytest.raises(ValueError):
+            _hip.tune_frames_set(H, W, M, R, 17, 512)
+    finally:
+        _hip.tune_set(H, W, M, R, 0, 0)
+        _hip.tune_frames_set(H, W, M, R, 0, 0)
+
+
 def test_temporal_chunks_keep_six_frames():
     """The temporal kernel splits the frames into chunks for about target_blocks blocks, each of at least 6 frames:
     the workspace (5 partial maps per chunk) gives the chunk count without a GPU."""

@@ -274,6 +274,36 @@ def test_autotune_shape_and_store(torch, tmp_path, monkeypatch):
     _hip.tune_set(40, 72, M, R, 0, 0)
 
 
+def test_autotune_frames_shape_and_store(torch, tmp_path, monkeypatch):
+    """The per-frame kernels' autotuner picks a shape of their own: per-frame maps and the RGC on arrays of that size
+    are bit-identical under it (no cross-frame reduction), the temporal kernel's shape stays untouched, and the
+    store hands the entry back after a reset."""
+    from nanopyx_amd import _hip, autotune
+    monkeypatch.setenv("NANOPYX_AMD_TUNE_FILE", str(tmp_path / "tune.json"))
+    rng = np.random.default_rng(10)
+    H, W, M, R, S = 24, 40, 8, 1.0, 2
+    x = torch.from_numpy((rng.random((30, H, W)) * 100).astype(np.float32)).cuda()
+    _hip.tune_set(H, W, M, R, 0, 0)
+    _hip.tune_frames_set(H, W, M, R, 0, 0)
+    ref = _hip.esrrf_frames(x, M, R, S, True).cpu().numpy()
+    gx = torch.from_numpy(rng.normal(0, 10, (4, 2 * H * M, 2 * W * M)).astype(np.float32)).cuda()
+    gy = torch.from_numpy(rng.normal(0, 10, (4, 2 * H * M, 2 * W * M)).astype(np.float32)).cuda()
+    img = torch.from_numpy((rng.random((4, H * M, W * M)) * 100).astype(np.float32)).cuda()
+    ref_rgc = _hip.rgc(gx, gy, img, M, R, S, True, False).cpu().numpy()
+    try:
+        entry = autotune.tune_frames(x, M, R, S, True)
+        print("autotune frames", entry, flush=True)
+        assert entry["max_waves"] in (4, 8, 16) and entry["ms"] > 0
+        assert _hip.tune_frames_get(H, W, M, R)[2] and not _hip.tune_get(H, W, M, R)[2]
+        assert np.array_equal(_hip.esrrf_frames(x, M, R, S, True).cpu().numpy(), ref)
+        assert np.array_equal(_hip.rgc(gx, gy, img, M, R, S, True, False).cpu().numpy(), ref_rgc)
+        _hip.tune_frames_set(H, W, M, R, 0, 0)
+        assert autotune.load() == 1
+        assert _hip.tune_frames_get(H, W, M, R) == (entry["max_waves"], entry["target_blocks"], True)
+    finally:
+        _hip.tune_frames_set(H, W, M, R, 0, 0)
+
+
 def _degenerate_images(H=24, W=70):
     """Images whose gradients vanish or are constant: flat (G = 0 everywhere), ramps along x, y and the diagonal
     (constant G, so ring samples perpendicular to G have G . d = 0 exactly), and a step edge."""
