@@ -307,8 +307,9 @@ int npx_set_launch(int max_waves_per_block, int target_blocks);
 /*
  * Autotuner of the launch shape, per (H, W, M, R). npx_autotune times every candidate shape — waves (output phases)
  * per block from {16, 8, 4} and a target block count from {256 .. 8192}, which sets the frames per chunk — with
- * npx_esrrf_temporal on the caller's stack and outputs (HIP events on `stream`; blocking), keeps the fastest for
- * later calls with the same (H, W, M, R) and returns its time per call in *best_ms. Candidates whose chunking needs
+ * npx_esrrf_temporal on the caller's stack and outputs (HIP events on `stream`; blocking; after >= 20 ms of warm-up
+ * calls, each candidate's best of three interleaved batches of >= ~2 ms), keeps the fastest for later calls with the
+ * same (H, W, M, R) and returns its time per call in *best_ms. Candidates whose chunking needs
  * more than work_bytes are skipped; npx_autotune_workspace gives the workspace that admits all of them.
  * npx_tune_set stores a shape directly (0, 0 forgets it); npx_tune_get writes the shape calls with these parameters
  * use into cfg[2] and returns 1 if it was tuned, 0 for the defaults. No reference counterpart: it takes the place

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -251,6 +252,62 @@ void frames_chunks(int N, const Shape& s, int target_blocks, int* nchunk, int* f
     want = std::min<long long>({want, N, 65535 / s.ngroups});
     *fpc = static_cast<int>((N + want - 1) / want);
     *nchunk = (N + *fpc - 1) / *fpc;
+}
+
+// Times candidate launch shapes of one entry point on the caller's data and returns the fastest in *best (its time
+// per call in *best_ms). use(c) makes shape c the one the next call takes; call() makes one call on `st`. The GPU is
+// first warmed up with the first candidate (calls for >= 20 ms, so that its clocks have settled), then every
+// candidate is timed in three rounds, each a batch of calls of about 2 ms or more, keeping its best batch: with the
+// rounds interleaved, clock drift during tuning does not favour the candidates timed last.
+int time_candidates(hipStream_t st, const std::vector<LaunchCfg>& cands,
+                    const std::function<void(const LaunchCfg&)>& use, const std::function<int()>& call,
+                    LaunchCfg* best, float* best_ms) {
+    if (cands.empty()) return fail(NPX_ERR_ARG, "no launch shape to time");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate(autotune)");
+    }
+    auto timed = [&](int n, float* ms) -> int {
+        hipError_t err = hipEventRecord(e0, st);
+        if (err != hipSuccess) return hip_fail(err, "hipEventRecord(autotune)");
+        for (int k = 0; k < n; ++k) {
+            const int r = call();
+            if (r) return r;
+        }
+        err = hipEventRecord(e1, st);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
+        return err == hipSuccess ? NPX_OK : hip_fail(err, "autotune timing");
+    };
+    int rc = NPX_OK;
+    float t = 0.f, warm = 0.f;
+    use(cands[0]);
+    for (int k = 0; k < 200 && rc == NPX_OK && warm < 20.f; ++k) {
+        rc = timed(1, &t);
+        warm += t;
+    }
+    const int batch = std::max(1, std::min(50, static_cast<int>(2.0f / std::max(t, 0.01f)) + 1));
+    std::vector<float> per_call(cands.size(), 0.f);
+    for (int round = 0; round < 3 && rc == NPX_OK; ++round) {
+        for (size_t i = 0; i < cands.size() && rc == NPX_OK; ++i) {
+            use(cands[i]);
+            rc = timed(batch, &t);
+            const float pc = t / static_cast<float>(batch);
+            if (round == 0 || pc < per_call[i]) per_call[i] = pc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != NPX_OK) return rc;
+    size_t bi = 0;
+    for (size_t i = 1; i < cands.size(); ++i)
+        if (per_call[i] < per_call[bi]) bi = i;
+    *best = cands[bi];
+    *best_ms = per_call[bi];
+    return NPX_OK;
 }
 
 // Shape of the per-frame-map kernel: fused_shape with its rows per lane (frames_q).
@@ -1002,14 +1059,8 @@ int npx_autotune(const void* d_in, int dtype, int N, int H, int W, int M, double
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune cannot run under stream capture");
     const Key key{H, W, M, bits(R)};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e != hipSuccess) return hip_fail(e, "hipEventCreate(autotune)");
-    LaunchCfg best{0, 0};
-    float best_t = 0.f;
-    int rc = NPX_OK;
-    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks) already timed
+    std::vector<LaunchCfg> cands;
+    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks): candidates that launch alike are timed once
     for (int mw : kTuneWaves) {
         for (int tb : kTuneBlocks) {
             const LaunchCfg c{mw, tb};
@@ -1023,38 +1074,27 @@ int npx_autotune(const void* d_in, int dtype, int N, int H, int W, int M, double
             const std::pair<int, int> id{s.nwaves, nchunk};
             if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
             seen.push_back(id);
-            {
-                std::lock_guard<std::mutex> lk(g_cfg_mu);
-                g_tuned[key] = c;
-            }
-            float t = 0.f;
-            for (int rep = 0; rep < 3 && rc == NPX_OK; ++rep) {  // one warm-up, then the best of two
-                e = hipEventRecord(e0, st);
-                if (e != hipSuccess) { rc = hip_fail(e, "hipEventRecord(autotune)"); break; }
-                rc = npx_esrrf_temporal(d_in, dtype, N, H, W, M, R, S, iw, mode_mask, d_avg, d_var, d_tac2, d_work,
-                                        work_bytes, stream);
-                if (rc) break;
-                e = hipEventRecord(e1, st);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                float ms = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) { rc = hip_fail(e, "autotune timing"); break; }
-                if (rep == 1 || (rep == 2 && ms < t)) t = ms;
-            }
-            if (rc) break;
-            if (best.max_waves == 0 || t < best_t) {
-                best = c;
-                best_t = t;
-            }
+            cands.push_back(c);
         }
-        if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (cands.empty()) return fail(NPX_ERR_ARG, "no launch shape fits the workspace");
+    LaunchCfg best{0, 0};
+    float best_t = 0.f;
+    const int rc = time_candidates(
+        st, cands,
+        [&](const LaunchCfg& c) {
+            std::lock_guard<std::mutex> lk(g_cfg_mu);
+            g_tuned[key] = c;
+        },
+        [&]() {
+            return npx_esrrf_temporal(d_in, dtype, N, H, W, M, R, S, iw, mode_mask, d_avg, d_var, d_tac2, d_work,
+                                      work_bytes, stream);
+        },
+        &best, &best_t);
     std::lock_guard<std::mutex> lk(g_cfg_mu);
-    if (rc != NPX_OK || best.max_waves == 0) {
+    if (rc != NPX_OK) {
         g_tuned.erase(key);
-        return rc != NPX_OK ? rc : fail(NPX_ERR_ARG, "no launch shape fits the workspace");
+        return rc;
     }
     g_tuned[key] = best;
     if (best_ms) *best_ms = best_t;
@@ -1065,20 +1105,12 @@ int npx_autotune_frames(const void* d_in, int dtype, int N, int H, int W, int M,
                         float* d_out, void* stream, double* best_ms) {
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune_frames cannot run under stream capture");
-    int rc = check_common(d_in, dtype, N, H, W, M);
-    if (rc) return rc;
+    const int rc0 = check_common(d_in, dtype, N, H, W, M);
+    if (rc0) return rc0;
     if (!d_out) return fail(NPX_ERR_ARG, "output pointer is NULL");
     const Key key{H, W, M, bits(R)};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e != hipSuccess) {
-        if (e0) (void)hipEventDestroy(e0);
-        return hip_fail(e, "hipEventCreate(autotune)");
-    }
-    LaunchCfg best{0, 0};
-    float best_t = 0.f;
-    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks) already timed
+    std::vector<LaunchCfg> cands;
+    std::vector<std::pair<int, int>> seen;  // (waves per block, chunks)
     for (int mw : kTuneWaves) {
         for (int tb : kTuneBlocks) {
             const LaunchCfg c{mw, tb};
@@ -1088,37 +1120,22 @@ int npx_autotune_frames(const void* d_in, int dtype, int N, int H, int W, int M,
             const std::pair<int, int> id{s.nwaves, nchunk};
             if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
             seen.push_back(id);
-            {
-                std::lock_guard<std::mutex> lk(g_cfg_mu);
-                g_tuned_frames[key] = c;
-            }
-            float t = 0.f;
-            for (int rep = 0; rep < 3 && rc == NPX_OK; ++rep) {  // one warm-up, then the best of two
-                e = hipEventRecord(e0, st);
-                if (e != hipSuccess) { rc = hip_fail(e, "hipEventRecord(autotune)"); break; }
-                rc = npx_esrrf_frames(d_in, dtype, N, H, W, M, R, S, iw, d_out, stream);
-                if (rc) break;
-                e = hipEventRecord(e1, st);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                float ms = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) { rc = hip_fail(e, "autotune timing"); break; }
-                if (rep == 1 || (rep == 2 && ms < t)) t = ms;
-            }
-            if (rc) break;
-            if (best.max_waves == 0 || t < best_t) {
-                best = c;
-                best_t = t;
-            }
+            cands.push_back(c);
         }
-        if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    LaunchCfg best{0, 0};
+    float best_t = 0.f;
+    const int rc = time_candidates(
+        st, cands,
+        [&](const LaunchCfg& c) {
+            std::lock_guard<std::mutex> lk(g_cfg_mu);
+            g_tuned_frames[key] = c;
+        },
+        [&]() { return npx_esrrf_frames(d_in, dtype, N, H, W, M, R, S, iw, d_out, stream); }, &best, &best_t);
     std::lock_guard<std::mutex> lk(g_cfg_mu);
-    if (rc != NPX_OK || best.max_waves == 0) {
+    if (rc != NPX_OK) {
         g_tuned_frames.erase(key);
-        return rc != NPX_OK ? rc : fail(NPX_ERR_ARG, "no launch shape was timed");
+        return rc;
     }
     g_tuned_frames[key] = best;
     if (best_ms) *best_ms = best_t;
