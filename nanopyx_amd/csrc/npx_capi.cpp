@@ -1103,11 +1103,11 @@ int npx_autotune(const void* d_in, int dtype, int N, int H, int W, int M, double
 
 int npx_autotune_frames(const void* d_in, int dtype, int N, int H, int W, int M, double R, double S, int iw,
                         float* d_out, void* stream, double* best_ms) {
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune_frames cannot run under stream capture");
     const int rc0 = check_common(d_in, dtype, N, H, W, M);
     if (rc0) return rc0;
     if (!d_out) return fail(NPX_ERR_ARG, "output pointer is NULL");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (capturing(st)) return fail(NPX_ERR_ARG, "npx_autotune_frames cannot run under stream capture");
     const Key key{H, W, M, bits(R)};
     std::vector<LaunchCfg> cands;
     std::vector<std::pair<int, int>> seen;  // (waves per block, chunks)

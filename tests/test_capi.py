@@ -42,6 +42,13 @@ def test_version_and_error_paths():
     assert rc == -3 and b"M >= 2" in lib.npx_last_error()
     rc = lib.npx_fht_interpolate_lattice(p, 1, 8, 8, 2, 1, 0, 16, -1, p, p, ctypes.c_size_t(1 << 20), None)
     assert rc == -1 and b"stride" in lib.npx_last_error()
+    # the per-frame tuner checks its stack and output before timing anything
+    rc = lib.npx_autotune_frames(None, 0, 4, 8, 8, 2, 1.5, 1.0, 1, p, None, None)
+    assert rc == -1 and b"NULL" in lib.npx_last_error()
+    rc = lib.npx_autotune_frames(p, 0, 4, 8, 8, 2, 1.5, 1.0, 1, None, None, None)
+    assert rc == -1 and b"output" in lib.npx_last_error()
+    rc = lib.npx_autotune_frames(p, 0, 4, 8, 8, 0, 1.5, 1.0, 1, p, None, None)
+    assert rc == -1 and b"magnification" in lib.npx_last_error()
 
 
 @pytest.mark.parametrize("R,M", [(1.0, 2), (1.5, 5), (2.0, 4), (2.5, 8), (1.2345, 3)])
