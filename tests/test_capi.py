@@ -167,6 +167,31 @@ def test_frames_launch_shape_apart_from_temporal():
         _hip.tune_frames_set(H, W, M, R, 0, 0)
 
 
+def test_packaged_store_entries_load():
+    """The packaged launch-shape store (bench.py loads it) is well-formed: every key is device|H|W|M|R with an optional
+    |frames for the per-frame kernels, every shape one the autotuner can produce, and each entry is accepted by the
+    library's table for its kind (read back with the matching getter)."""
+    import json
+    from nanopyx_amd.tuning import PACKAGED
+    with open(PACKAGED) as f:
+        data = json.load(f)
+    assert data
+    for key, v in data.items():
+        parts = key.split("|")
+        assert len(parts) in (5, 6) and parts[0] == "gfx950/256", key
+        frames = len(parts) == 6
+        assert not frames or parts[5] == "frames", key
+        H, W, M, R = int(parts[1]), int(parts[2]), int(parts[3]), float(parts[4])
+        mw, tb = int(v["max_waves"]), int(v["target_blocks"])
+        assert mw in (4, 8, 16) and tb in (256, 512, 1024, 2048, 4096, 8192), key
+        setter, getter = (_hip.tune_frames_set, _hip.tune_frames_get) if frames else (_hip.tune_set, _hip.tune_get)
+        try:
+            setter(H, W, M, R, mw, tb)
+            assert getter(H, W, M, R) == (mw, tb, True), key
+        finally:
+            setter(H, W, M, R, 0, 0)
+
+
 def test_temporal_chunks_keep_six_frames():
     """The temporal kernel splits the frames into chunks for about target_blocks blocks, each of at least 6 frames:
     the workspace (5 partial maps per chunk) gives the chunk count without a GPU."""
