@@ -368,11 +368,12 @@ npx::FusedArgs job_args(const void* d_in, int dtype, int N, int H, int W, int M,
 }
 
 void add_radius(npx::FusedArgs& a, const npx::PlanHost& h, const npx::DevTables* t) {
+    (void)t;  // the kernels form the ring geometry of each block themselves (from tSS, tSO and the pixel centres)
     const int r = a.nrad++;
-    a.ring[r] = t->ring;
     a.K[r] = h.K;
-    a.rs[r] = h.ring_stride;
     a.margin[r] = h.margin;
+    a.tss[r] = h.tSS;
+    a.tso[r] = h.tSO;
     a.KL = std::max(a.KL, h.K);
 }
 

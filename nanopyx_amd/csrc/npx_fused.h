@@ -377,34 +377,64 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 
     // ---- stage the frame-invariant tables. Per wave (block of RWL floats): the ring constants of its radius and
     // phase in the form the ring sums use (-dy[j], 1 / dx[i] (0 for dx = 0), (t d, -t |dx| / s)[i][j] with
-    // s = kRingScale and |dx| taken as 1 for dx = 0), then the
-    // normaliser part of its ring block (wsum, then the (NW+1)^2 prefix sums of w).
-    for (int k = tid; k < a.nwaves * RWL; k += nthr) {
-        const int w = k / RWL, e = k - w * RWL;
+    // s = kRingScale and |dx| taken as 1 for dx = 0), then the normaliser part (wsum, then the (NW+1)^2 prefix sums
+    // of w). The geometry is the reference's for this block's pixels: it takes each output pixel's centre as
+    // float32((x M + 0.5) / M) (_c_sr_radial_gradient_convergence.c:21-22), and the window offsets, distances and
+    // weights from that rounded centre (:31-44, :4-6, :63). Every pixel of a phase in the block has the same rounded
+    // fraction (columns x0 .. x0 + 63 lie in one binade for x0 >= 64, rows y0 .. y0 + Q - 1 for y0 >= Q), so the
+    // offsets are those of the block's last column and row (in the first block the other pixels' centres, below 64,
+    // differ by less than 2^-18). One work item per sample writes its two ring constants and its weight into a grid
+    // (the lattice-row area, unused before the first frame); after a barrier the row prefix sums of each grid go to
+    // the lattice storage (double), after another the column sums of those give the prefix sums.
+    constexpr int NWL = 2 * KL + 1, WG = NWL * NWL;
+    float* const s_wgrid = s_rx;                                      // [wave][WG] weights w[j][i], j outer
+    double* const s_wrow = reinterpret_cast<double*>(s_lat);          // [wave][WG] sum over i' < i + 1 of w[j][i']
+    static_assert(16 * WG <= L::NXY, "weight grids must fit the lattice-row area");
+    static_assert(2 * 16 * WG <= 4 * LR * LP, "row prefix sums must fit the lattice storage");
+    const int xr = min(x0 + 63, a.W - 1), yr = min(y0 + Q - 1, a.H - 1);
+    // offset o (0 .. 2K) of the window of half-width K around pixel r, output phase p, along rows or columns
+    auto offset = [&](int r, int p, int K, int o) {
+        const float c = static_cast<float>((static_cast<double>(M * r + p) + 0.5) / M);
+        const float v = static_cast<float>(static_cast<int>(2.0f * c) + (o - K)) / 2.0f;
+        return v - c;
+    };
+    for (int k = tid; k < a.nwaves * RCL; k += nthr) {  // dy, 1 / dx and the padding (pairs below)
+        const int w = k / RCL, e = k - w * RCL;
         const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
+        const int Kr = rw < a.nrad ? a.K[rw] : 0, nw = rw < a.nrad ? 2 * Kr + 1 : 0;
+        if (e >= 2 * nw && e < 2 * nw + 2 * nw * nw) continue;
         float v = 0.f;
-        if (rw < a.nrad) {
-            const int nw = 2 * a.K[rw] + 1;
-            const float* blk = a.ring[rw] + static_cast<size_t>(pw) * a.rs[rw];
-            if (e < RCL) {
-                if (e < nw) {
-                    v = -blk[e];  // -dy
-                } else if (e < 2 * nw) {
-                    v = blk[e] != 0.f ? 1.0f / blk[e] : 0.f;  // 1 / dx, 0 where dx = 0
-                } else if (e < 2 * nw + 2 * nw * nw) {      // host pairs (d, t) of column (e - 2 nw) / (2 nw)
-                    const int pe = e - 2 * nw;
-                    if (pe & 1) {
-                        const float dxa = fabsf(blk[nw + pe / (2 * nw)]);
-                        v = blk[e] * (-1.f / kRingScale) * (dxa != 0.f ? dxa : 1.f);  // -t |dx| / s
-                    } else {
-                        v = blk[e + 1] * blk[e];  // t d
-                    }
-                }
-            } else if (e - RCL < 1 + (nw + 1) * (nw + 1)) {
-                v = blk[2 * nw + 3 * nw * nw + (e - RCL)];
-            }
+        if (e < nw) {
+            v = -offset(yr, pw / M, Kr, e);  // -dy
+        } else if (e < 2 * nw) {
+            const float dx = offset(xr, pw % M, Kr, e - nw);
+            v = dx != 0.f ? 1.0f / dx : 0.f;  // 1 / dx, 0 where dx = 0
         }
-        s_ring[k] = v;
+        s_ring[w * RWL + e] = v;
+    }
+    for (int k = tid; k < a.nwaves * WG; k += nthr) {  // sample (i, j): pair i nw + j
+        const int w = k / WG, q = k - w * WG;
+        const int sw = group * a.nwaves + w, rw = sw / a.nphase, pw = sw - rw * a.nphase;
+        if (rw >= a.nrad) continue;
+        const int Kr = a.K[rw], nw = 2 * Kr + 1;
+        if (q >= nw * nw) continue;
+        const int i = q / nw, j = q - i * nw;
+        const float dx = offset(xr, pw % M, Kr, i), dy = offset(yr, pw / M, Kr, j);
+        const float dd = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+        const float d = static_cast<float>(sqrt(static_cast<double>(dd)));
+        float wt = 0.f, t = 0.f;
+        if (d != 0.f && d <= a.tso[rw]) {  // :44
+            const double dn = d;
+            const double y = dn * exp((-dn * dn) / static_cast<double>(a.tss[rw]));
+            const double y2 = y * y;
+            wt = static_cast<float>(y2 * y2);  // the fourth power (:5)
+            t = static_cast<float>(static_cast<double>(wt) / dn);
+        }
+        const float dxa = fabsf(dx);
+        float* const pr = s_ring + w * RWL + 2 * nw + 2 * q;
+        pr[0] = t * d;                                                 // t d
+        pr[1] = t * (-1.f / kRingScale) * (dxa != 0.f ? dxa : 1.f);  // -t |dx| / s
+        s_wgrid[w * WG + j * nw + i] = wt;
     }
     for (int la = tid; la < LR; la += nthr) {
         const int aa = a_lo + la;
@@ -430,6 +460,42 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         s_lcw[lb] = w;
         s_lcb[lb] = b;
     }
+    __syncthreads();  // the weight grids are complete
+    for (int k = tid; k < a.nwaves * NWL; k += nthr) {  // row j of each grid: prefix sums along i
+        const int w = k / NWL, j = k - w * NWL;
+        const int rw = (group * a.nwaves + w) / a.nphase;
+        const int nw = rw < a.nrad ? 2 * a.K[rw] + 1 : 0;
+        if (j >= nw) continue;
+        double acc = 0.0;
+        for (int i = 0; i < nw; ++i) {
+            acc += s_wgrid[w * WG + j * nw + i];
+            s_wrow[w * WG + j * nw + i] = acc;
+        }
+    }
+    __syncthreads();  // the row prefix sums are complete
+    // wsum (float, j outer and i inner, :30-35,64) and the prefix sums P[jj][ii] = sum over j < jj, i < ii of w
+    for (int k = tid; k < a.nwaves * G::RN; k += nthr) {
+        const int w = k / G::RN, e = k - w * G::RN;
+        const int rw = (group * a.nwaves + w) / a.nphase;
+        const int nw = rw < a.nrad ? 2 * a.K[rw] + 1 : 0;
+        float* const out = s_ring + w * RWL + RCL;
+        if (e == 0) {
+            float v = 0.f;
+            for (int q = 0; q < nw * nw; ++q) v += s_wgrid[w * WG + q];
+            out[0] = v;
+        } else if (e - 1 < nw + 1) {  // column ii: P[jj][ii] for every jj
+            const int ii = e - 1;
+            double acc = 0.0;
+            out[1 + ii] = 0.f;
+            for (int jj = 1; jj <= nw; ++jj) {
+                if (ii > 0) acc += s_wrow[w * WG + (jj - 1) * nw + ii - 1];
+                out[1 + jj * (nw + 1) + ii] = static_cast<float>(acc);
+            }
+        } else if (e - 1 >= (nw + 1) * (nw + 1)) {
+            out[e] = 0.f;  // padding
+        }
+    }
+
     // intensity Catmull-Rom row taps of every (phase row set, q); the wave's set is the index of its phase row
     int rows[kRowSets];
     const int nrows = group_rows(a, group, rows);

@@ -83,11 +83,12 @@ struct FusedArgs {
     const int* crb;
     const float4* ccw;
     const int* ccb;
-    // per radius: ring blocks [phase][rs], half-width, block stride, margin
-    const float* ring[kMaxRad];
+    // per radius: half-width, margin, and the reference's tSS = 2 sigma^2 and tSO = 2 sigma + 1 (float32) from which
+    // the kernels form each block's ring geometry
     int K[kMaxRad];
-    int rs[kMaxRad];
     int margin[kMaxRad];
+    float tss[kMaxRad];
+    float tso[kMaxRad];
     // per sensitivity: value and power mode (0: none, 1: general, 2/3/4: integer)
     float sens[kMaxSens];
     int sens_pow[kMaxSens];

@@ -2,7 +2,8 @@
 
 It repeats the kernel's index arithmetic statement for statement — tiles, halos, the row then column Catmull-Rom
 passes of the lattice, the two column-parity lattice planes, the per-lane column sweep (u and v taken divided by
-|dx|, the column's sign in the operands), and the edge normaliser — on the library's own plan tables, in float32, and
+|dx|, the column's sign in the operands), and the edge normaliser — on the library's own tap tables and the ring
+geometry each block forms from its pixel centres (block_ring), in float32, and
 asserts that every LDS / table / global index the kernel forms stays inside its buffer and that every tap with a
 non-zero weight reads inside its tile (clamped taps must carry weight 0). Tests run it on small stacks against the
 oracle, so an indexing mistake shows up on the CPU before the kernel ever runs on a GPU.
@@ -29,6 +30,39 @@ def model_frames(stack, M, R, S, iw=True, Q=None):
 
 def _pow_mode(Sf):
     return 0 if not Sf > 1 else {2.0: 2, 3.0: 3, 4.0: 4}.get(float(Sf), 1)
+
+
+def block_ring(M, K, xr, yr, tSS, tSO):
+    """The ring constants one block stages for half-width K, as the kernel forms them: each output pixel centre
+    float32((r M + p + 0.5) / M) of the block's last column xr and last row yr
+    (_c_sr_radial_gradient_convergence.c:21-22), window offsets float32((int(2 c) + o - K) / 2) - c (:31-37),
+    d = sqrt(float32 dx^2 + dy^2) in double rounded to float32, w = (d exp(-d^2 / tSS))^4 in double (:4-6, :44),
+    t = w / d; wsum is the float32 sum of w (j outer, i inner) and the prefix sums the double row sums, then column
+    sums, of w. Returns dy[P, NW], dx[P, NW], d and t [P, NW(i), NW(j)], wsum[P], prefix[P, NW + 1, NW + 1]."""
+    NW = 2 * K + 1
+    o = np.arange(NW) - K
+
+    def offsets(r):
+        c = np.array([(M * r + p + 0.5) / M for p in range(M)], np.float64).astype(f32)
+        v = ((np.floor(f32(2) * c)[:, None] + o[None, :]).astype(f32) / f32(2)).astype(f32)
+        return (v - c[:, None]).astype(f32)                        # [phase index, o]
+
+    offx, offy = offsets(xr), offsets(yr)
+    ph = np.arange(M * M)
+    dx, dy = offx[ph % M], offy[ph // M]                           # [P, NW]
+    dd = ((dx * dx)[:, :, None] + (dy * dy)[:, None, :]).astype(f32)
+    d = np.sqrt(dd.astype(np.float64)).astype(f32)
+    ok = (d != 0) & (d <= f32(tSO))
+    dn = np.where(ok, d, 1).astype(np.float64)
+    y = dn * np.exp((-dn * dn) / np.float64(f32(tSS)))
+    w = np.where(ok, ((y * y) * (y * y)).astype(f32), f32(0))
+    t = np.where(ok, (w.astype(np.float64) / dn).astype(f32), f32(0))
+    wj = np.transpose(w, (0, 2, 1))                                # [P, j, i]
+    wsum = np.cumsum(wj.reshape(len(ph), -1), axis=1, dtype=f32)[:, -1]
+    prefix = np.zeros((len(ph), NW + 1, NW + 1), f32)
+    rows = np.cumsum(wj.astype(np.float64), axis=2)                # sum over i' <= i of w[j][i']
+    prefix[:, 1:, 1:] = np.cumsum(rows, axis=1).astype(f32)
+    return dict(dy=dy, dx=dx, d=d, t=t, wsum=wsum, prefix=prefix)
 
 
 def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
@@ -60,6 +94,7 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
     outs = {(float(R), float(S)): np.zeros((N, HM, WM), np.float32) for R in radii for S in sens}
     nregx, nregy = (W + 63) // 64, (H + Q - 1) // Q
     lanes = np.arange(64)
+    ring_cache = {}
     for f in range(N):
         img = stack[f].astype(np.float32)
         for by in range(nregy):
@@ -168,14 +203,17 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
                             s_lat[idx] = (gxv * inv, gyv * inv)
                 # (4)+(5) waves: one per (radius, phase)
                 for ri, R in enumerate(radii):
-                    tr_ = tabs[ri]
                     K = Ks[ri]
                     DK = KL - K
                     NW = 2 * K + 1
                     NR = 2 * Q + 2 * K - 1
                     margin = 0 if (arrays is not None and arrays[3]) else infos[ri]["margin"]
-                    dyt, dxt, dtab, ttab = tr_["dy"], tr_["dx"], tr_["d"], tr_["t"]
-                    wsum = tr_["wsum"]
+                    key = (ri, min(x0 + 63, W - 1), min(y0 + Q - 1, H - 1))
+                    if key not in ring_cache:
+                        ring_cache[key] = block_ring(M, K, key[1], key[2], infos[ri]["tSS"], infos[ri]["tSO"])
+                    br = ring_cache[key]
+                    dyt, dxt, dtab, ttab = br["dy"], br["dx"], br["d"], br["t"]
+                    wsum = br["wsum"]
                     for ph in range(nphase):
                         py, px = divmod(ph, M)
                         ey, ex = (2 * py + 1) // M, (2 * px + 1) // M
@@ -230,7 +268,7 @@ def model_multi(stack, M, radii, sens, iw=True, Q=None, KL=None, arrays=None):
                             b0 = 2 * xi + ex
                             den = np.full(64, wsum[ph], np.float32)
                             # border pixels: normaliser from the 2D prefix sums of w over the valid window part
-                            P = tr_["prefix"][ph]
+                            P = br["prefix"][ph]
                             j0 = max(0, 1 - (a0 - K))
                             j1 = max(0, min(NW, 2 * HM - 2 - (a0 - K) + 1))
                             i0 = np.maximum(0, 1 - (b0 - K))

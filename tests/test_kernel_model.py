@@ -97,3 +97,46 @@ def test_model_arrays_mode(oracle_built, N, H, W, M, R, S, iw, pad):
     out = model_multi(np.zeros((N, H, W), np.float32), M, [R], [S], iw=iw, arrays=(gx, gy, img, pad))
     ref = oracle_built.rgc(gx, gy, img, M, R, S, iw, pad)
     assert rel_err(out[(float(R), float(S))], ref)[0] <= 1e-5
+
+
+@pytest.mark.parametrize("M,R", [(2, 1.0), (4, 2.0), (8, 0.6), (3, 1.5), (5, 0.5), (7, 2.5)])
+def test_block_ring_against_plan_tables(M, R):
+    """The geometry a block forms from its rounded pixel centres against the library's host tables of the exact
+    geometry: identical at power-of-two magnifications (every centre is exact in float32), at any magnification for
+    the first block (centres below 64 carry rounding errors under 2^-18); far along a row at other magnifications the
+    offsets move by the rounding of the centre, which the reference's maps carry too."""
+    from nanopyx_amd import _hip
+    from kernel_model import block_ring
+    H = W = 1200
+    t = _hip.plan_tables(H, W, M, R)
+    d = t["info"]
+    pow2 = M & (M - 1) == 0
+    for xr, yr in ((63, 7), (1023, 15), (1100, 600)):
+        br = block_ring(M, d["K"], xr, yr, d["tSS"], d["tSO"])
+        exact = pow2 or (xr, yr) == (63, 7)
+        tol = 2.0 ** -18 if exact else 1e-3
+        for k in ("dy", "dx", "d", "t"):
+            assert np.allclose(br[k], t[k], rtol=0, atol=tol * max(1.0, float(np.abs(t[k]).max()))), (k, xr, yr)
+        # w ~ d^4 near the centre: a position error e moves w by about 4 e / d relative (d >= 0.1 here)
+        assert np.allclose(br["wsum"], t["wsum"], rtol=tol * 100, atol=0)
+        assert np.allclose(br["prefix"], t["prefix"], rtol=0, atol=tol * 100 * float(t["prefix"].max()))
+        if pow2:
+            for k in ("dy", "dx", "d", "t", "wsum", "prefix"):
+                assert np.array_equal(br[k], t[k]), (k, xr, yr)
+        elif xr > 64:
+            assert not np.array_equal(br["dx"], t["dx"])
+
+
+@pytest.mark.parametrize("M,R,W", [(7, 0.54, 300), (5, 0.5, 330), (3, 1.0, 520)])
+def test_model_follows_reference_pixel_centres(oracle_built, M, R, W):
+    """Rows wider than 256 at magnifications that are not powers of two: the reference rounds each pixel centre to
+    float32, which moves its maps 5e-5 - 1.1e-4 from the exact geometry here; the blocks' rounded-centre geometry
+    follows the reference (measured 5.6e-6 - 1.5e-5)."""
+    from oracle import fp64
+    rng = np.random.default_rng(3)
+    x = (rng.random((1, 3, W)) * 100 + 1).astype(np.float32)
+    out = model_frames(x, M, R, 2.0)
+    ref = oracle_built.esrrf(x, M, R, 2.0)
+    d = rel_err(fp64.esrrf_frames(x, M, R, 2.0, True), ref)[0]
+    e = rel_err(out, ref)[0]
+    assert e <= 2.5e-5 and e < d / 3, (e, d)
