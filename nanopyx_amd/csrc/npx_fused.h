@@ -616,40 +616,60 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         }
     };
 
+    // arrays mode: lattice point (a, b) is the caller's gradient at row min(a M, 2 HM - 1), column min(b M, 2 WM - 1)
+    // of its x2M arrays (or row min(a, 2H), column min(b, 2W) of a compact lattice), normalised and scaled by
+    // s = kRingScale (_c_sr_radial_gradient_convergence.c:45-52 reads those positions); points outside
+    // a in [1, 2 HM - 2], b in [1, 2 WM - 2] are outside every window and hold 0. The first kLatPf points of each
+    // thread are gathered a frame ahead into registers, so the gather of frame f + 1 is in flight during frame f's
+    // ring sums (the Q = 16 bodies and the widest rings have no registers to spare and gather in place).
+    constexpr int kLatPf = ARRAYS && Q <= 8 && KL <= 4 ? 4 : 0;
+    float2 lpf[kLatPf > 0 ? kLatPf : 1];
+    auto lat_load = [&](int f, int idx) {
+        const int plane = idx / LPL;
+        const int rem = idx - plane * LPL;
+        const int la = rem / LP;
+        const int pos = rem - la * LP;
+        const int aa = a_lo + la, bb = b_lo + 2 * pos + plane;
+        float2 v = make_float2(0.f, 0.f);
+        if (aa >= 1 && aa <= 2 * a.HM - 2 && bb >= 1 && bb <= 2 * a.WM - 2) {
+            const size_t o = a.alat ? (static_cast<size_t>(f) * (2 * a.H + 1) + min(aa, 2 * a.H)) * (2 * a.W + 1) +
+                                          min(bb, 2 * a.W)
+                                    : (static_cast<size_t>(f) * (2 * a.HM) + min(aa * M, 2 * a.HM - 1)) * (2 * a.WM) +
+                                          min(bb * M, 2 * a.WM - 1);
+            v = make_float2(a.agx[o], a.agy[o]);
+        }
+        return v;
+    };
+    // the prefetched and the in-place points must round alike whatever the block size: the sum of squares is
+    // written with explicit roundings so the compiler cannot contract the two copies differently
+    auto lat_norm = [](float2 v) {
+        const float n2 = __fmaf_rn(v.x, v.x, __fmul_rn(v.y, v.y));
+        if (!(n2 > 0.f)) return make_float2(0.f, 0.f);
+        const float inv = rsqrtf(n2) * kRingScale;  // s / |G|
+        return make_float2(v.x * inv, v.y * inv);
+    };
+    auto lat_fetch = [&](int f) {
+#pragma unroll
+        for (int k = 0; k < kLatPf; ++k) {
+            const int idx = tid + k * nthr;
+            lpf[k] = idx < 2 * LPL ? lat_load(f, idx) : make_float2(0.f, 0.f);
+        }
+    };
+    if (ARRAYS && f_begin < f_end) lat_fetch(f_begin);
+
     for (int f = f_begin; f < f_end; ++f) {
         // intensity rows with their own storage are computed here, beside Roberts (both only read the tile)
         if (!ARRAYS && T_OWN && a.iw) intensity_rows();
         if constexpr (ARRAYS) {
-            // lattice point (a, b) is the caller's gradient at row min(a M, 2 HM - 1), column min(b M, 2 WM - 1) of
-            // its x2M arrays, normalised and scaled by s = kRingScale (_c_sr_radial_gradient_convergence.c:45-52 reads
-            // those positions); points
-            // outside a in [1, 2 HM - 2], b in [1, 2 WM - 2] are outside every window and hold 0
-            const size_t P4 = static_cast<size_t>(4) * a.HM * a.WM;
-            const int CW = 2 * a.W + 1;  // compact lattice: row min(a, 2H), column min(b, 2W)
-            const size_t CP = static_cast<size_t>(2 * a.H + 1) * CW;
-            for (int idx = tid; idx < 2 * LPL; idx += nthr) {
-                const int plane = idx / LPL;
-                const int rem = idx - plane * LPL;
-                const int la = rem / LP;
-                const int pos = rem - la * LP;
-                const int aa = a_lo + la, bb = b_lo + 2 * pos + plane;
-                float2 g = make_float2(0.f, 0.f);
-                if (aa >= 1 && aa <= 2 * a.HM - 2 && bb >= 1 && bb <= 2 * a.WM - 2) {
-                    const size_t o = a.alat ? static_cast<size_t>(f) * CP + static_cast<size_t>(min(aa, 2 * a.H)) * CW +
-                                                  min(bb, 2 * a.W)
-                                            : static_cast<size_t>(f) * P4 +
-                                                  static_cast<size_t>(min(aa * M, 2 * a.HM - 1)) * (2 * a.WM) +
-                                                  min(bb * M, 2 * a.WM - 1);
-                    const float gxv = a.agx[o], gyv = a.agy[o];
-                    const float n2 = gxv * gxv + gyv * gyv;
-                    if (n2 > 0.f) {
-                        const float inv = rsqrtf(n2) * kRingScale;  // s / |G|
-                        g = make_float2(gxv * inv, gyv * inv);
-                    }
-                }
-                s_lat[idx] = g;
+            // the first kLatPf lattice points of each thread arrived in registers during the previous frame
+#pragma unroll
+            for (int k = 0; k < kLatPf; ++k) {
+                const int idx = tid + k * nthr;
+                if (idx < 2 * LPL) s_lat[idx] = lat_norm(lpf[k]);
             }
+            for (int idx = tid + kLatPf * nthr; idx < 2 * LPL; idx += nthr) s_lat[idx] = lat_norm(lat_load(f, idx));
             __syncthreads();
+            if (f + 1 < f_end) lat_fetch(f + 1);  // in flight during this frame's ring sums
         } else {
             // (2) Roberts cross (_c_gradients.c:53-72); gradients outside the image are 0 (CR skips them)
             for (int idx = tid; idx < TGR * TGC; idx += nthr) {
@@ -742,6 +762,17 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
         if (active) {
             // (4) ring sums
             float acc[Q];
+            // arrays mode: the caller's intensity weights of the lane's pixels, loaded before the ring sums so that
+            // their latency hides behind them (the Q = 16 bodies have no registers to spare and load them after)
+            constexpr bool IW_EARLY = ARRAYS && Q <= 8;
+            float aiw[IW_EARLY ? Q : 1];
+            if constexpr (IW_EARLY) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int rM = min(M * (y0 + q) + py, a.HM - 1);
+                    aiw[q] = a.iw ? a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs] : 0.f;
+                }
+            }
             if constexpr (VARK) ring_sum_any<KL, Q>(Kw, acc, s_lat, ey, ex, lane, ringC, ring_ineg, ring_inz);
             else ring_sum<KL, KL, Q>(acc, s_lat, ey, ex, lane, ringC, ring_ineg, ring_inz);
             // (5) normalise, sensitivity, intensity weight; then store or fold into the moments. Each step runs over
@@ -843,7 +874,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, int bx, int by, 
 #pragma unroll
                         for (int q = 0; q < Q; ++q) {
                             const int rM = min(M * (y0 + q) + py, a.HM - 1);
-                            acc[q] = acc[q] * a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs];
+                            acc[q] = acc[q] * (IW_EARLY ? aiw[q]
+                                                        : a.aimg[(static_cast<size_t>(f) * a.HM + rM) * a.WM + cMs]);
                         }
                     } else {
                         // the row values of up to 8 pixels are read before the first of them is used, so that their
