@@ -294,7 +294,9 @@ int npx_plan_info(int H, int W, int M, double R, int *info, float *finfo);
 
 /* Copies the plan tables to host arrays sized from npx_plan_info (ring nphase*ring_block floats,
  * wsum nphase, lattice taps 2H+1 / 2W+1 entries, intensity taps HM / WM entries; *_w arrays hold 4 floats
- * per entry). No reference counterpart (introspection, as npx_plan_info). */
+ * per entry). The ring blocks and wsum are the exact per-phase geometry; the kernels form each block's ring
+ * geometry from its float32 pixel centres, equal to these at power-of-two magnifications. No reference
+ * counterpart (introspection, as npx_plan_info). */
 int npx_plan_tables(int H, int W, int M, double R, float *ring, float *wsum, int *lrb, float *lrw, int *lcb,
                     float *lcw, int *crb, float *crw, int *ccb, float *ccw);
 

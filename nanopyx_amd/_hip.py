@@ -759,7 +759,8 @@ def plan_info(H, W, M, R):
 
 
 def plan_tables(H, W, M, R):
-    """Host copies of the kernel tables (ring, wsum, lattice and intensity taps)."""
+    """Host copies of the plan tables: the exact per-phase ring geometry (ring, wsum; the kernels form each block's
+    from its float32 pixel centres, equal at power-of-two magnifications) and the lattice and intensity taps."""
     d = plan_info(H, W, M, R)
     K, P, NW = d["K"], d["nphase"], 2 * d["K"] + 1
     ring = np.zeros((P, d["ring_stride"]), np.float32)

@@ -72,8 +72,9 @@ Plan* get_plan(int H, int W, int M, double R) {
 
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-// One contiguous blob of the plan's tables (ring blocks, wsum, the four tap tables as float4 weights + int bases),
-// every segment 256-byte aligned; tables_in(base) gives the device view of the blob copied to `base`.
+// One contiguous blob of the plan's device tables (the four tap tables as float4 weights + int bases), every segment
+// 256-byte aligned; tables_in(base) gives the device view of the blob copied to `base`. The ring geometry is not in
+// it: the kernels form each block's from its pixel centres.
 void pack_tables(const PlanHost& h, TableBlob* out) {
     auto taps = [](const std::vector<CrTap>& v, std::vector<float>& w, std::vector<int>& b) {
         w.resize(v.size() * 4);
@@ -89,33 +90,29 @@ void pack_tables(const PlanHost& h, TableBlob* out) {
     taps(h.lat_col, lcw, lcb);
     taps(h.cr_row, crw, crb);
     taps(h.cr_col, ccw, ccb);
-    const void* src[10] = {h.ring.data(), h.wsum.data(), lrw.data(), lrb.data(), lcw.data(),
-                           lcb.data(),    crw.data(),    crb.data(), ccw.data(), ccb.data()};
-    const size_t bytes[10] = {h.ring.size() * 4, h.wsum.size() * 4, lrw.size() * 4, lrb.size() * 4,
-                              lcw.size() * 4,    lcb.size() * 4,    crw.size() * 4, crb.size() * 4,
-                              ccw.size() * 4,    ccb.size() * 4};
+    const void* src[8] = {lrw.data(), lrb.data(), lcw.data(), lcb.data(), crw.data(), crb.data(), ccw.data(), ccb.data()};
+    const size_t bytes[8] = {lrw.size() * 4, lrb.size() * 4, lcw.size() * 4, lcb.size() * 4,
+                             crw.size() * 4, crb.size() * 4, ccw.size() * 4, ccb.size() * 4};
     size_t total = 0;
-    for (int i = 0; i < 10; ++i) {
+    for (int i = 0; i < 8; ++i) {
         out->off[i] = total;
         total += align256(bytes[i]);
     }
     out->bytes.assign(total, 0);
-    for (int i = 0; i < 10; ++i) std::memcpy(out->bytes.data() + out->off[i], src[i], bytes[i]);
+    for (int i = 0; i < 8; ++i) std::memcpy(out->bytes.data() + out->off[i], src[i], bytes[i]);
 }
 
 DevTables TableBlob::tables_in(void* mem) const {
     auto* base = static_cast<uint8_t*>(mem);
     DevTables t;
-    t.ring = reinterpret_cast<const float*>(base + off[0]);
-    t.wsum = reinterpret_cast<const float*>(base + off[1]);
-    t.lrw = reinterpret_cast<const float4*>(base + off[2]);
-    t.lrb = reinterpret_cast<const int*>(base + off[3]);
-    t.lcw = reinterpret_cast<const float4*>(base + off[4]);
-    t.lcb = reinterpret_cast<const int*>(base + off[5]);
-    t.crw = reinterpret_cast<const float4*>(base + off[6]);
-    t.crb = reinterpret_cast<const int*>(base + off[7]);
-    t.ccw = reinterpret_cast<const float4*>(base + off[8]);
-    t.ccb = reinterpret_cast<const int*>(base + off[9]);
+    t.lrw = reinterpret_cast<const float4*>(base + off[0]);
+    t.lrb = reinterpret_cast<const int*>(base + off[1]);
+    t.lcw = reinterpret_cast<const float4*>(base + off[2]);
+    t.lcb = reinterpret_cast<const int*>(base + off[3]);
+    t.crw = reinterpret_cast<const float4*>(base + off[4]);
+    t.crb = reinterpret_cast<const int*>(base + off[5]);
+    t.ccw = reinterpret_cast<const float4*>(base + off[6]);
+    t.ccb = reinterpret_cast<const int*>(base + off[7]);
     return t;
 }
 
@@ -367,8 +364,8 @@ npx::FusedArgs job_args(const void* d_in, int dtype, int N, int H, int W, int M,
     return a;
 }
 
-void add_radius(npx::FusedArgs& a, const npx::PlanHost& h, const npx::DevTables* t) {
-    (void)t;  // the kernels form the ring geometry of each block themselves (from tSS, tSO and the pixel centres)
+// The kernels form each block's ring geometry themselves, from tSS, tSO and the block's pixel centres.
+void add_radius(npx::FusedArgs& a, const npx::PlanHost& h) {
     const int r = a.nrad++;
     a.K[r] = h.K;
     a.margin[r] = h.margin;
@@ -387,7 +384,7 @@ npx::FusedArgs base_args(const void* d_in, int dtype, int N, int H, int W, int M
                          const npx::DevTables* t, const Shape& s, double S, int iw) {
     npx::FusedArgs a = job_args(d_in, dtype, N, H, W, M, h, t, s, iw);
     a.KL = h.K;
-    add_radius(a, h, t);
+    add_radius(a, h);
     add_sensitivity(a, S);
     return a;
 }
@@ -767,7 +764,7 @@ int npx_esrrf_sweep(const void* d_in, int dtype, int N, int H, int W, int n_mag,
                 rc = plan_and_tables(H, W, sj.M, radii[sj.r0 + r], &p, &t, st);
                 if (rc) return rc;
                 if (r == 0) a = job_args(d_in, dtype, N, H, W, sj.M, p->host, t, sh, iw);
-                add_radius(a, p->host, t);
+                add_radius(a, p->host);
             }
             for (int s = 0; s < sj.nsn; ++s) add_sensitivity(a, sens[sj.s0 + s]);
             // one half-width: the body built for it; radii of different half-widths share a wider lattice

@@ -26,7 +26,7 @@ int pow_mode_of(double S);
 // The plan's device tables as one contiguous blob (for callers that place them in memory of their own).
 struct TableBlob {
     std::vector<uint8_t> bytes;
-    size_t off[10];
+    size_t off[8];
     DevTables tables_in(void* mem) const;  // device view of the blob copied to `mem`
 };
 void pack_tables(const PlanHost& h, TableBlob* out);
@@ -40,7 +40,7 @@ struct Shape {
 // are added with add_radius / add_sensitivity.
 FusedArgs job_args(const void* d_in, int dtype, int N, int H, int W, int M, const PlanHost& h, const DevTables* t,
                    const Shape& s, int iw);
-void add_radius(FusedArgs& a, const PlanHost& h, const DevTables* t);
+void add_radius(FusedArgs& a, const PlanHost& h);
 void add_sensitivity(FusedArgs& a, double S);
 
 bool capturing(hipStream_t st);

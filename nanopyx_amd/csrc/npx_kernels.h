@@ -41,8 +41,6 @@ constexpr int kLdsLimit = 160 * 1024;  // LDS bytes one workgroup may use on gfx
 
 // Device copies of the plan tables (npx_plan.h).
 struct DevTables {
-    const float* ring;   // [phase] blocks: dy[j], dx[i], (d, t)[i][j], w[i][j], wsum, prefix sums of w
-    const float* wsum;   // [phase]
     const float4* lrw;   // lattice row taps [2H+1]
     const int* lrb;
     const float4* lcw;   // lattice column taps [2W+1]

@@ -321,7 +321,7 @@ int step(Session& s, hipStream_t st, int* nlaunch) {
             sh.nregx = (j.W + 63) / 64;
             sh.nregy = (j.H + kQTemporal - 1) / kQTemporal;
             FusedArgs a = host::job_args(j.d_in, j.dtype, j.next_frame + nf, j.H, j.W, j.M, h, &j.t, sh, j.iw);
-            host::add_radius(a, h, &j.t);
+            host::add_radius(a, h);
             host::add_sensitivity(a, j.S);  // one radius: the lattice half-width is its K (add_radius)
             a.f0 = j.next_frame;
             a.avg = (j.mask & NPX_AVG) ? j.avg : nullptr;
