@@ -175,6 +175,41 @@ def test_wide_rows_pixel_centre_rounding(torch, oracle_built, M, R):
     assert e_ref <= 3e-5 and e_ref < d / 3
 
 
+def test_wide_rows_pixel_centre_rounding_every_path(torch, oracle_built):
+    """The temporal kernel, the batched sweep (two half-widths on one lattice) and the RGC on caller arrays share the
+    block body that forms the ring geometry from the reference's float32 pixel centres: on 1024-wide rows at M3 each
+    follows the reference restatement where the exact geometry is 7e-5 - 1.8e-4 away."""
+    from nanopyx_amd import _hip
+    from oracle import fp64
+    rng = np.random.default_rng(21)
+    M, S = 3, 2.0
+    x = (rng.random((3, 8, 1024)) * 100 + 1).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    radii = [0.5, 1.0]
+    sw = _hip.esrrf_sweep(xd, [M], radii, [S], iw=True)
+    for R in radii:
+        ref = oracle_built.temporal(oracle_built.esrrf(x, M, R, S), "AVG")
+        d = rel_err(fp64.temporal(fp64.esrrf_frames(x, M, R, S, True), "AVG"), ref)[0]
+        print(f"R{R}: exact geometry vs reference {d:.3e}", flush=True)
+        avg = _hip.esrrf_temporal(xd, M, R, S, True, ("AVG",))["AVG"].cpu().numpy()
+        e_t = _report(f"1024-wide temporal AVG M{M} R{R} vs reference restatement", avg, ref)
+        e_s = _report(f"1024-wide sweep AVG M{M} R{R} vs reference restatement",
+                      sw.maps[(M, float(R), float(S))].cpu().numpy(), ref)
+        assert e_t <= 3e-5 and e_t < d / 3 and e_s <= 3e-5 and e_s < d / 3
+    HM, WM = 8 * M, 1024 * M
+    gx = rng.normal(0, 10, (2, 2 * HM, 2 * WM)).astype(np.float32)
+    gy = rng.normal(0, 10, (2, 2 * HM, 2 * WM)).astype(np.float32)
+    img = (rng.random((2, HM, WM)) * 100 + 1).astype(np.float32)
+    for R in radii:
+        ref = oracle_built.rgc(gx, gy, img, M, R, S, True, False)
+        d = rel_err(fp64.rgc_from_arrays(gx, gy, img, M, R, S, True), ref)[0]
+        got = _hip.rgc(torch.from_numpy(gx).cuda(), torch.from_numpy(gy).cuda(), torch.from_numpy(img).cuda(),
+                       M, R, S, True, False).cpu().numpy()
+        e = _report(f"1024-wide RGC on arrays M{M} R{R} vs reference restatement", got, ref)
+        print(f"R{R}: exact geometry vs reference {d:.3e}", flush=True)
+        assert e <= 3e-5 and e < d / 3
+
+
 @pytest.mark.parametrize("M,R", [(3, 1.5), (4, 2.0)])
 def test_single_frame_temporal(torch, oracle_built, M, R):
     """One frame: AVG is the frame's map, VAR is 0 and TAC2 is NaN everywhere, as the reference's NumPy reduction
