@@ -210,6 +210,29 @@ def test_wide_rows_pixel_centre_rounding_every_path(torch, oracle_built):
         assert e <= 3e-5 and e < d / 3
 
 
+@pytest.mark.parametrize("M,R", [(5, 0.5), (3, 1.0), (7, 0.54)])
+def test_tall_columns_pixel_centre_rounding(torch, oracle_built, M, R):
+    """The row direction of the reference's float32 pixel centres: on 1024-tall, 8-wide images the blocks below the
+    first take the rounded fraction of their last row (per-frame maps: 16 or 8 rows per block; temporal: 8). Per-frame
+    maps and AVG follow the reference restatement where the exact geometry is 6e-5 - 1.9e-4 away."""
+    from nanopyx_amd import _hip
+    from oracle import fp64
+    rng = np.random.default_rng(31)
+    x = (rng.random((2, 1024, 8)) * 100 + 1).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    ref = oracle_built.esrrf(x, M, R, 2.0)
+    exact = fp64.esrrf_frames(x, M, R, 2.0, True)
+    d = rel_err(exact, ref)[0]
+    e = _report(f"1024-tall M{M} R{R} frames vs reference restatement",
+                _hip.esrrf_frames(xd, M, R, 2.0, True).cpu().numpy(), ref)
+    ref_avg = oracle_built.temporal(ref, "AVG")
+    d_avg = rel_err(fp64.temporal(exact, "AVG"), ref_avg)[0]
+    e_avg = _report(f"1024-tall M{M} R{R} AVG vs reference restatement",
+                    _hip.esrrf_temporal(xd, M, R, 2.0, True, ("AVG",))["AVG"].cpu().numpy(), ref_avg)
+    print(f"exact geometry vs reference: frames {d:.3e}, AVG {d_avg:.3e}", flush=True)
+    assert e <= 3e-5 and e < d / 3 and e_avg <= 3e-5 and e_avg < d_avg / 3
+
+
 @pytest.mark.parametrize("M,R", [(3, 1.5), (4, 2.0)])
 def test_single_frame_temporal(torch, oracle_built, M, R):
     """One frame: AVG is the frame's map, VAR is 0 and TAC2 is NaN everywhere, as the reference's NumPy reduction
